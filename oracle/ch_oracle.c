@@ -1,0 +1,623 @@
+/*
+ * ch_oracle.c -- FP64 CPU restatement of the cahn-hilliard time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  It is loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+ * product library.  Loops follow the reference's loop order so that the
+ * single-core timing is representative of the reference CPU integrators.
+ *
+ * Pins (what checks each function; the tests live in tests/test_oracle_*.py):
+ *   drand48 ............ glibc drand48 (python ctypes) + known first values
+ *   init ............... reference fixture header/first values (6 digits)
+ *   euler, euler_mob ... oracle/_ref golden vectors (reference sources, 1e-12),
+ *                        reference .dat fixtures (5e-6), Fourier eigenmode,
+ *                        brute-force dense operator on tiny grids
+ *   pseudospectral ..... brute DFT, per-mode closed form, reference .dat
+ *                        fixtures (5e-6); FFTW itself is unavailable here
+ *   observables ........ closed forms (uniform field, single cosine)
+ *   S(q) ............... numpy.fft on random fields
+ *   adjoint ............ central finite differences of och_run + dot-product test
+ */
+#include "ch_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SQR(x) ((x) * (x))
+
+static const double OCH_PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------ */
+/* drand48: X_{n+1} = (0x5DEECE66D X_n + 0xB) mod 2^48, srand48 sets   */
+/* X = (seed << 16) | 0x330E (POSIX).  Seeded at src/main.cpp:36.       */
+/* ------------------------------------------------------------------ */
+void och_srand48(och_rng *r, int64_t seed) {
+    uint64_t s = (uint64_t)(uint32_t)seed;
+    r->x = ((s << 16) | 0x330Eull) & 0xFFFFFFFFFFFFull;
+}
+
+double och_drand48(och_rng *r) {
+    r->x = (0x5DEECE66Dull * r->x + 0xBull) & 0xFFFFFFFFFFFFull;
+    return (double)r->x / 281474976710656.0; /* 2^48 */
+}
+
+static int64_t ipow(int N, int dims) {
+    int64_t b = 1;
+    for(int i = 0; i < dims; i++) b *= N;
+    return b;
+}
+
+/* src/CahnHilliard.cpp:133-151 (one species, unit scaling factor 1) */
+void och_init_density(int dims, int N, double density, double A, int N_peaks, int64_t seed, double *rho) {
+    och_rng r;
+    och_srand48(&r, seed);
+    int64_t bins = ipow(N, dims);
+    double initial_k = 2 * OCH_PI * N_peaks / (double)N;
+    for(int64_t bin = 0; bin < bins; bin++) {
+        double modulation = A * cos(initial_k * bin);
+        double random_factor = (N_peaks == 0) ? (och_drand48(&r) - 0.5) : 1.0 + 0.02 * (och_drand48(&r) - 0.5);
+        if(density != 0.0) {
+            rho[bin] = density * (1.0 + 2.0 * modulation * random_factor);
+        }
+        else {
+            rho[bin] = 2.0 * modulation * random_factor;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* models                                                              */
+/* ------------------------------------------------------------------ */
+static double two_valence_delta(const och_model *m) { return 2.0 * m->valence * m->delta; }
+
+/* src/models/SimpleWertheim.cpp:88-90 */
+double och_wertheim_X(const och_model *m, double rho) {
+    double tvd = two_valence_delta(m);
+    return (-1.0 + sqrt(1.0 + 2.0 * tvd * rho)) / (tvd * rho);
+}
+
+/* src/models/Landau.cpp:54-57, src/models/SimpleWertheim.cpp:71-86 */
+double och_bulk_free_energy(const och_model *m, double rho) {
+    if(m->model == OCH_LANDAU) {
+        return -0.5 * m->epsilon * SQR(rho) + 0.25 * SQR(SQR(rho));
+    }
+    double reg = m->regularisation_delta;
+    double log_delta = log(reg);
+    double f_ref = m->B2 * SQR(rho);
+    if(rho < reg) {
+        double a = 1 / (2.0 * reg);
+        double b = log_delta - 1.0;
+        double c = reg * (log_delta - 1.0) - a * SQR(reg) - b * reg;
+        f_ref += a * SQR(rho) + b * rho + c;
+    }
+    else {
+        f_ref += rho * (log(rho) - 1.0);
+    }
+    double X = och_wertheim_X(m, rho);
+    double f_bond = (rho > 0.) ? m->valence * rho * (log(X) + 0.5 * (1. - X)) : 0.0;
+    return f_ref + f_bond;
+}
+
+/* src/models/Landau.cpp:46-52, src/models/SimpleWertheim.cpp:40-53 */
+double och_der_bulk_free_energy(const och_model *m, double rho) {
+    if(m->model == OCH_LANDAU) {
+        return -m->epsilon * rho + rho * rho * rho;
+    }
+    double reg = m->regularisation_delta;
+    double der_f_ref = 2 * m->B2 * rho;
+    if(rho < reg) {
+        der_f_ref += log(reg) + (rho - reg) / reg;
+    }
+    else {
+        der_f_ref += log(rho);
+    }
+    double der_f_bond = (rho > 0.) ? m->valence * log(och_wertheim_X(m, rho)) : 0.0;
+    return der_f_ref + der_f_bond;
+}
+
+/* d^2 f / d rho^2.  Landau: -eps + 3 psi^2.  SimpleWertheim (rho > reg):
+ * 2 B2 + 1/rho + valence X'/X with X' from the mass-action law
+ * c rho X^2 + X - 1 = 0 (c = tvd/2): X' = -c X^2 / (2 c rho X + 1). */
+double och_second_der_bulk_free_energy(const och_model *m, double rho) {
+    if(m->model == OCH_LANDAU) {
+        return -m->epsilon + 3.0 * rho * rho;
+    }
+    double reg = m->regularisation_delta;
+    double d = 2 * m->B2;
+    d += (rho < reg) ? 1.0 / reg : 1.0 / rho;
+    if(rho > 0.) {
+        double c = 0.5 * two_valence_delta(m);
+        double X = och_wertheim_X(m, rho);
+        double Xp = -c * X * X / (2.0 * c * rho * X + 1.0);
+        d += m->valence * Xp / X;
+    }
+    return d;
+}
+
+/* src/models/Landau.cpp:65-70 */
+double och_pressure(const och_model *m, double rho) {
+    return rho * och_der_bulk_free_energy(m, rho) - och_bulk_free_energy(m, rho);
+}
+
+void och_der_bulk_free_energy_field(const och_model *m, const double *rho, double *out, int64_t n) {
+    for(int64_t i = 0; i < n; i++) out[i] = och_der_bulk_free_energy(m, rho[i]);
+}
+
+/* src/mobility/ConstantMobility.h:19-23, FreeEnergyMobility.h:25-34 with
+ * SimpleWertheim.cpp:104-111, RegularisedMobility.h:26-33; "linear" is new:
+ * M rho / rho_ref. */
+void och_update_mobility(const och_params *p, const och_model *m, const double *rho, double *mob, int64_t n) {
+    for(int64_t i = 0; i < n; i++) {
+        switch(p->mobility) {
+        case OCH_MOB_CONSTANT: mob[i] = p->M; break;
+        case OCH_MOB_FREE_ENERGY: {
+            double X = och_wertheim_X(m, rho[i]);
+            double D = p->M * pow(X, (double)m->valence);
+            mob[i] = rho[i] * D;
+            break;
+        }
+        case OCH_MOB_REGULARISED: mob[i] = p->M * rho[i] / (rho[i] + p->rho_min); break;
+        case OCH_MOB_LINEAR: mob[i] = p->M * rho[i] / p->rho_ref; break;
+        default: mob[i] = NAN;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* grid helpers, src/integrators/EulerCPU.cpp:39-88                    */
+/* ------------------------------------------------------------------ */
+static int ilog2(int N) { int b = 0; while((1 << b) < N) b++; return b; }
+
+static void fill_coords(int dims, int N, int64_t idx, int *c) {
+    int bits = ilog2(N);
+    for(int d = 0; d < dims; d++) { c[d] = (int)(idx & (N - 1)); idx >>= bits; }
+}
+
+static int64_t cell_idx(int dims, int N, const int *c) {
+    int bits = ilog2(N);
+    int64_t idx = 0, mul = 1;
+    for(int d = 0; d < dims; d++) { idx += c[d] * mul; mul <<= bits; }
+    return idx;
+}
+
+/* periodic (2d+1)-point Laplacian, neighbours summed as -x,+x,-y,+y,-z,+z */
+static double cell_laplacian(int dims, int N, double dx, const double *f, int64_t idx) {
+    if(dims == 1) {
+        int64_t m = (idx - 1 + N) & (N - 1), pp = (idx + 1) & (N - 1);
+        return (f[m] + f[pp] - 2.0 * f[idx]) / SQR(dx);
+    }
+    int c[3], cn[3];
+    fill_coords(dims, N, idx, c);
+    memcpy(cn, c, sizeof(c));
+    double sum = 0.0;
+    for(int d = 0; d < dims; d++) {
+        cn[d] = (c[d] - 1 + N) & (N - 1);
+        sum += f[cell_idx(dims, N, cn)];
+        cn[d] = (c[d] + 1) & (N - 1);
+        sum += f[cell_idx(dims, N, cn)];
+        cn[d] = c[d];
+    }
+    return (sum - 2.0 * dims * f[idx]) / SQR(dx);
+}
+
+/* src/integrators/EulerCPU.cpp:18-36 */
+void och_euler_step(const och_params *p, const och_model *m, double *rho) {
+    int64_t bins = ipow(p->N, p->dims);
+    double *rho_der = (double *)malloc(sizeof(double) * bins);
+    for(int64_t i = 0; i < bins; i++) rho_der[i] = och_der_bulk_free_energy(m, rho[i]);
+    for(int64_t i = 0; i < bins; i++) rho_der[i] -= 2 * p->k * cell_laplacian(p->dims, p->N, p->dx, rho, i);
+    double M = p->M; /* constant mobility: mobility(0, 0) */
+    for(int64_t i = 0; i < bins; i++) {
+        double total_derivative = M * cell_laplacian(p->dims, p->N, p->dx, rho_der, i);
+        rho[i] += total_derivative * p->dt;
+    }
+    free(rho_der);
+}
+
+/* src/integrators/EulerMobilityCPU.cpp:31-83 (no noise) with the mobility
+ * refreshed from rho first, as CahnHilliard::evolve does (src/CahnHilliard.cpp:279-282) */
+void och_euler_mobility_step(const och_params *p, const och_model *m, double *rho) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double *mob = (double *)malloc(sizeof(double) * bins);
+    double *mu = (double *)malloc(sizeof(double) * bins);
+    double *flux = (double *)malloc(sizeof(double) * bins * dims);
+    och_update_mobility(p, m, rho, mob, bins);
+    for(int64_t i = 0; i < bins; i++) mu[i] = och_der_bulk_free_energy(m, rho[i]);
+    for(int64_t i = 0; i < bins; i++) mu[i] -= 2.0 * p->k * cell_laplacian(dims, N, p->dx, rho, i);
+    for(int64_t i = 0; i < bins; i++) {
+        int c[3], cp[3];
+        fill_coords(dims, N, i, c);
+        for(int d = 0; d < dims; d++) {
+            memcpy(cp, c, sizeof(c));
+            cp[d] = (c[d] + 1) & (N - 1);
+            int64_t ip = cell_idx(dims, N, cp);
+            double M_face = 0.5 * (mob[i] + mob[ip]);
+            flux[i * dims + d] = M_face * (mu[ip] - mu[i]) / p->dx;
+        }
+    }
+    for(int64_t i = 0; i < bins; i++) {
+        /* src/integrators/EulerCPU.cpp:114-128 */
+        int c[3], cm[3];
+        fill_coords(dims, N, i, c);
+        memcpy(cm, c, sizeof(c));
+        double res = 0;
+        for(int d = 0; d < dims; d++) {
+            cm[d] = (c[d] - 1 + N) & (N - 1);
+            int64_t im = cell_idx(dims, N, cm);
+            res += (flux[i * dims + d] - flux[im * dims + d]) / p->dx;
+            cm[d] = c[d];
+        }
+        rho[i] += res * p->dt;
+    }
+    free(mob); free(mu); free(flux);
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: iterative radix-2, twiddles from exact integer indices         */
+/* ------------------------------------------------------------------ */
+static void twiddle(int n, int j, int inverse, double *c, double *s) {
+    double ang = 2.0 * OCH_PI * (double)j / (double)n;
+    *c = cos(ang);
+    *s = inverse ? sin(ang) : -sin(ang);
+}
+
+/* unnormalised DFT, forward sign e^{-i}, inverse e^{+i} (FFTW convention) */
+void och_fft_c2c(int n, int inverse, double *a, int stride) {
+    /* gather into a contiguous buffer */
+    double *b = (double *)malloc(sizeof(double) * 2 * n);
+    for(int j = 0; j < n; j++) { b[2 * j] = a[2 * (int64_t)j * stride]; b[2 * j + 1] = a[2 * (int64_t)j * stride + 1]; }
+    int bits = ilog2(n);
+    for(int j = 0; j < n; j++) {
+        int r = 0;
+        for(int q = 0; q < bits; q++) if(j & (1 << q)) r |= 1 << (bits - 1 - q);
+        if(r > j) {
+            double t0 = b[2 * j], t1 = b[2 * j + 1];
+            b[2 * j] = b[2 * r]; b[2 * j + 1] = b[2 * r + 1];
+            b[2 * r] = t0; b[2 * r + 1] = t1;
+        }
+    }
+    for(int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1;
+        for(int i = 0; i < n; i += len) {
+            for(int j = 0; j < half; j++) {
+                double wr, wi;
+                twiddle(len, j, inverse, &wr, &wi);
+                double *u = b + 2 * (i + j), *v = b + 2 * (i + j + half);
+                double vr = v[0] * wr - v[1] * wi, vi = v[0] * wi + v[1] * wr;
+                v[0] = u[0] - vr; v[1] = u[1] - vi;
+                u[0] += vr; u[1] += vi;
+            }
+        }
+    }
+    for(int j = 0; j < n; j++) { a[2 * (int64_t)j * stride] = b[2 * j]; a[2 * (int64_t)j * stride + 1] = b[2 * j + 1]; }
+    free(b);
+}
+
+void och_dft_brute(int n, int inverse, const double *in, double *out) {
+    for(int k = 0; k < n; k++) {
+        double sr = 0, si = 0;
+        for(int j = 0; j < n; j++) {
+            double wr, wi;
+            twiddle(n, (int)(((int64_t)j * k) % n), inverse, &wr, &wi);
+            sr += in[2 * j] * wr - in[2 * j + 1] * wi;
+            si += in[2 * j] * wi + in[2 * j + 1] * wr;
+        }
+        out[2 * k] = sr; out[2 * k + 1] = si;
+    }
+}
+
+/* r2c over a dims-dimensional N^dims real array into N^(dims-1) x (N/2+1) complex */
+static void r2c(int dims, int N, const double *in, double *out) {
+    int Nh = N / 2 + 1;
+    int64_t lines = ipow(N, dims - 1);
+    double *line = (double *)malloc(sizeof(double) * 2 * N);
+    for(int64_t l = 0; l < lines; l++) {
+        for(int x = 0; x < N; x++) { line[2 * x] = in[l * N + x]; line[2 * x + 1] = 0.0; }
+        och_fft_c2c(N, 0, line, 1);
+        memcpy(out + 2 * l * Nh, line, sizeof(double) * 2 * Nh);
+    }
+    free(line);
+    if(dims >= 2) { /* y axis: stride Nh */
+        int64_t planes = ipow(N, dims - 2);
+        for(int64_t pz = 0; pz < planes; pz++)
+            for(int kx = 0; kx < Nh; kx++) och_fft_c2c(N, 0, out + 2 * (pz * N * Nh + kx), Nh);
+    }
+    if(dims == 3) { /* z axis: stride N*Nh */
+        for(int y = 0; y < N; y++)
+            for(int kx = 0; kx < Nh; kx++) och_fft_c2c(N, 0, out + 2 * ((int64_t)y * Nh + kx), N * Nh);
+    }
+}
+
+/* c2r, unnormalised; imaginary parts of the kx = 0 and kx = N/2 entries of
+ * each x-line are dropped (the Hermitian-completion semantics of a c2r). */
+static void c2r(int dims, int N, double *in, double *out) {
+    int Nh = N / 2 + 1;
+    if(dims == 3) {
+        for(int y = 0; y < N; y++)
+            for(int kx = 0; kx < Nh; kx++) och_fft_c2c(N, 1, in + 2 * ((int64_t)y * Nh + kx), N * Nh);
+    }
+    if(dims >= 2) {
+        int64_t planes = ipow(N, dims - 2);
+        for(int64_t pz = 0; pz < planes; pz++)
+            for(int kx = 0; kx < Nh; kx++) och_fft_c2c(N, 1, in + 2 * (pz * N * Nh + kx), Nh);
+    }
+    int64_t lines = ipow(N, dims - 1);
+    double *line = (double *)malloc(sizeof(double) * 2 * N);
+    for(int64_t l = 0; l < lines; l++) {
+        const double *h = in + 2 * l * Nh;
+        for(int k = 0; k < Nh; k++) { line[2 * k] = h[2 * k]; line[2 * k + 1] = h[2 * k + 1]; }
+        line[1] = 0.0;
+        line[2 * (N / 2) + 1] = 0.0;
+        for(int k = 1; k < N / 2; k++) { line[2 * (N - k)] = h[2 * k]; line[2 * (N - k) + 1] = -h[2 * k + 1]; }
+        och_fft_c2c(N, 1, line, 1);
+        for(int x = 0; x < N; x++) out[l * N + x] = line[2 * x];
+    }
+    free(line);
+}
+
+/* ------------------------------------------------------------------ */
+/* pseudospectral, src/integrators/PseudospectralCPU.cpp:13-136        */
+/* ------------------------------------------------------------------ */
+struct och_ps {
+    och_params p;
+    och_model m;
+    int64_t bins, hat;
+    double *rho_hat, *rho_hat_copy, *f_der, *f_der_hat, *k2, *dealiaser;
+};
+
+och_ps *och_ps_create(const och_params *p, const och_model *m, const double *rho) {
+    och_ps *s = (och_ps *)calloc(1, sizeof(och_ps));
+    s->p = *p; s->m = *m;
+    int N = p->N, dims = p->dims, Nh = N / 2 + 1;
+    s->bins = ipow(N, dims);
+    s->hat = ipow(N, dims - 1) * Nh;
+    s->rho_hat = (double *)malloc(sizeof(double) * 2 * s->hat);
+    s->rho_hat_copy = (double *)malloc(sizeof(double) * 2 * s->hat);
+    s->f_der_hat = (double *)malloc(sizeof(double) * 2 * s->hat);
+    s->f_der = (double *)malloc(sizeof(double) * s->bins);
+    s->k2 = (double *)malloc(sizeof(double) * s->hat);
+    s->dealiaser = (double *)malloc(sizeof(double) * s->hat);
+    double k_cut = N * OCH_PI / (N * p->dx) * 2.0 / 3.0;
+    int64_t k_idx = 0;
+    if(dims == 1) {
+        for(int i = 0; i < Nh; i++) {
+            double k = 2.0 * OCH_PI * i / (N * p->dx);
+            s->dealiaser[k_idx] = (k <= k_cut) ? 1.0 : 0.0;
+            s->k2[k_idx++] = SQR(k);
+        }
+    }
+    else if(dims == 2) {
+        for(int kx_idx = 0; kx_idx < N; kx_idx++) {
+            int kx = (kx_idx < N / 2) ? kx_idx : N - kx_idx;
+            for(int ky = 0; ky < Nh; ky++) {
+                double k = 2.0 * OCH_PI * sqrt((double)(SQR(kx) + SQR(ky))) / (N * p->dx);
+                s->dealiaser[k_idx] = (k < k_cut) ? 1.0 : 0.0;
+                s->k2[k_idx++] = SQR(k);
+            }
+        }
+    }
+    else {
+        for(int kx_idx = 0; kx_idx < N; kx_idx++) {
+            int kx_i = (kx_idx < N / 2) ? kx_idx : (kx_idx - N);
+            double kx = 2.0 * OCH_PI * kx_i / (N * p->dx);
+            for(int ky_idx = 0; ky_idx < N; ky_idx++) {
+                int ky_i = (ky_idx < N / 2) ? ky_idx : (ky_idx - N);
+                double ky = 2.0 * OCH_PI * ky_i / (N * p->dx);
+                for(int kz_idx = 0; kz_idx < Nh; kz_idx++) {
+                    double kz = 2.0 * OCH_PI * kz_idx / (N * p->dx);
+                    double k2 = SQR(kx) + SQR(ky) + SQR(kz);
+                    double k = sqrt(k2);
+                    s->dealiaser[k_idx] = (k <= k_cut) ? 1.0 : 0.0;
+                    s->k2[k_idx++] = k2;
+                }
+            }
+        }
+    }
+    r2c(dims, N, rho, s->rho_hat);
+    memcpy(s->rho_hat_copy, s->rho_hat, sizeof(double) * 2 * s->hat);
+    return s;
+}
+
+void och_ps_step(och_ps *s, double *rho) {
+    const och_params *p = &s->p;
+    double M = p->M;
+    for(int64_t i = 0; i < s->bins; i++) s->f_der[i] = och_der_bulk_free_energy(&s->m, rho[i]);
+    r2c(p->dims, p->N, s->f_der, s->f_der_hat);
+    for(int64_t k = 0; k < s->hat; k++) {
+        double *fh = s->f_der_hat + 2 * k, *rh = s->rho_hat + 2 * k, *rc = s->rho_hat_copy + 2 * k;
+        if(p->use_dealias) { fh[0] *= s->dealiaser[k]; fh[1] *= s->dealiaser[k]; }
+        double k2 = s->k2[k];
+        double k4 = SQR(k2);
+        double denom = 1.0 + p->dt * M * (p->S * k2 + 2.0 * p->k * k4);
+        double re = (rh[0] - p->dt * M * k2 * (fh[0] - p->S * rh[0])) / denom;
+        double im = (rh[1] - p->dt * M * k2 * (fh[1] - p->S * rh[1])) / denom;
+        rh[0] = rc[0] = re;
+        rh[1] = rc[1] = im;
+        rc[0] /= s->bins;
+        rc[1] /= s->bins;
+    }
+    c2r(p->dims, p->N, s->rho_hat_copy, rho);
+}
+
+int64_t och_ps_hat_size(const och_ps *s) { return s->hat; }
+void och_ps_get_hat(const och_ps *s, double *re_im) { memcpy(re_im, s->rho_hat, sizeof(double) * 2 * s->hat); }
+
+void och_ps_destroy(och_ps *s) {
+    if(!s) return;
+    free(s->rho_hat); free(s->rho_hat_copy); free(s->f_der_hat); free(s->f_der); free(s->k2); free(s->dealiaser);
+    free(s);
+}
+
+int och_run(int integrator, const och_params *p, const och_model *m, double *rho, int64_t steps) {
+    if(integrator == 0) {
+        for(int64_t t = 0; t < steps; t++) och_euler_step(p, m, rho);
+    }
+    else if(integrator == 1) {
+        for(int64_t t = 0; t < steps; t++) och_euler_mobility_step(p, m, rho);
+    }
+    else if(integrator == 2) {
+        och_ps *s = och_ps_create(p, m, rho);
+        for(int64_t t = 0; t < steps; t++) och_ps_step(s, rho);
+        och_ps_destroy(s);
+    }
+    else return -1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* observables, src/CahnHilliard.cpp:256-319 (V_bin = dx^3, :163)      */
+/* ------------------------------------------------------------------ */
+double och_average_mass(const och_params *p, const double *rho) {
+    int64_t bins = ipow(p->N, p->dims);
+    double mass = 0.;
+    for(int64_t i = 0; i < bins; i++) mass += rho[i];
+    double V_bin = p->dx * p->dx * p->dx;
+    return mass * V_bin / bins;
+}
+
+double och_average_free_energy(const och_params *p, const och_model *m, const double *rho) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double fe = 0.;
+    for(int64_t i = 0; i < bins; i++) {
+        int c[3], cp[3];
+        fill_coords(dims, N, i, c);
+        double interfacial = 0.;
+        for(int d = 0; d < dims; d++) {
+            memcpy(cp, c, sizeof(c));
+            cp[d] = (c[d] + 1) & (N - 1);
+            double g = (rho[cell_idx(dims, N, cp)] - rho[i]) / p->dx;
+            interfacial += p->k * g * g;
+        }
+        fe += och_bulk_free_energy(m, rho[i]) + interfacial;
+    }
+    double V_bin = p->dx * p->dx * p->dx;
+    return fe * V_bin / bins;
+}
+
+/* ------------------------------------------------------------------ */
+/* S(q), utils/sq_from_traj.py:72-103                                  */
+/* ------------------------------------------------------------------ */
+static int fftfreq_index(int i, int L) { return (i < (L + 1) / 2) ? i : i - L; }
+
+static int shell_of(int L, double dx, int a, int b) {
+    double k = 2.0 * OCH_PI / (L * dx);
+    double kx = k * fftfreq_index(a, L), ky = k * fftfreq_index(b, L);
+    double K = sqrt(kx * kx + ky * ky);
+    double dq = 2.0 * OCH_PI / (L * dx);
+    return (int)rint(K / dq);
+}
+
+int och_sq_nshells(int L) {
+    int mx = 0;
+    for(int a = 0; a < L; a++)
+        for(int b = 0; b < L; b++) { int s = shell_of(L, 1.0, a, b); if(s > mx) mx = s; }
+    return mx + 1;
+}
+
+static void sq_spectrum(int L, const double *field, double *F) {
+    int64_t n = (int64_t)L * L;
+    double mean = 0;
+    for(int64_t i = 0; i < n; i++) mean += field[i];
+    mean /= n;
+    for(int64_t i = 0; i < n; i++) { F[2 * i] = field[i] - mean; F[2 * i + 1] = 0; }
+    for(int r = 0; r < L; r++) och_fft_c2c(L, 0, F + 2 * (int64_t)r * L, 1);
+    for(int c = 0; c < L; c++) och_fft_c2c(L, 0, F + 2 * c, L);
+}
+
+void och_radial_sq(int L, double dx, const double *field, double *Sq) {
+    int ns = och_sq_nshells(L);
+    int64_t n = (int64_t)L * L;
+    double *F = (double *)malloc(sizeof(double) * 2 * n);
+    double *cnt = (double *)calloc(ns, sizeof(double));
+    sq_spectrum(L, field, F);
+    for(int s = 0; s < ns; s++) Sq[s] = 0;
+    for(int a = 0; a < L; a++)
+        for(int b = 0; b < L; b++) {
+            int64_t i = (int64_t)a * L + b;
+            int s = shell_of(L, dx, a, b);
+            Sq[s] += (F[2 * i] * F[2 * i] + F[2 * i + 1] * F[2 * i + 1]) / n;
+            cnt[s] += 1;
+        }
+    for(int s = 0; s < ns; s++) Sq[s] = cnt[s] > 0 ? Sq[s] / cnt[s] : 0.0;
+    free(F); free(cnt);
+}
+
+/* loss = sum_s (S_s - T_s)^2 and its gradient with respect to the field */
+double och_sq_loss(int L, double dx, const double *field, const double *T, double *grad) {
+    int ns = och_sq_nshells(L);
+    int64_t n = (int64_t)L * L;
+    double *Sq = (double *)malloc(sizeof(double) * ns);
+    double *cnt = (double *)calloc(ns, sizeof(double));
+    double *F = (double *)malloc(sizeof(double) * 2 * n);
+    och_radial_sq(L, dx, field, Sq);
+    double loss = 0;
+    for(int s = 0; s < ns; s++) loss += SQR(Sq[s] - T[s]);
+    if(grad) {
+        sq_spectrum(L, field, F);
+        for(int a = 0; a < L; a++)
+            for(int b = 0; b < L; b++) cnt[shell_of(L, dx, a, b)] += 1;
+        /* G_k = 2 (S_s - T_s) / (c_s n); dL/dphi = 2 Re(sum_k G_k F_k e^{+ikx}) */
+        for(int a = 0; a < L; a++)
+            for(int b = 0; b < L; b++) {
+                int64_t i = (int64_t)a * L + b;
+                int s = shell_of(L, dx, a, b);
+                double G = 2.0 * (Sq[s] - T[s]) / (cnt[s] * n);
+                F[2 * i] *= G; F[2 * i + 1] *= G;
+            }
+        for(int r = 0; r < L; r++) och_fft_c2c(L, 1, F + 2 * (int64_t)r * L, 1);
+        for(int c = 0; c < L; c++) och_fft_c2c(L, 1, F + 2 * c, L);
+        double mean = 0;
+        for(int64_t i = 0; i < n; i++) { grad[i] = 2.0 * F[2 * i]; mean += grad[i]; }
+        mean /= n;
+        for(int64_t i = 0; i < n; i++) grad[i] -= mean;
+    }
+    free(Sq); free(cnt); free(F);
+    return loss;
+}
+
+/* ------------------------------------------------------------------ */
+/* adjoint of the Landau Euler step                                    */
+/* J^T l = l + dt M (f''(psi) (L l) - 2k L (L l))   (L symmetric)       */
+/* ------------------------------------------------------------------ */
+void och_euler_adjoint_step(const och_params *p, const och_model *m, const double *psi, double *lambda, double *grad) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double *w = (double *)malloc(sizeof(double) * bins);
+    double *lp = (double *)malloc(sizeof(double) * bins);
+    double *mu = (double *)malloc(sizeof(double) * bins);
+    for(int64_t i = 0; i < bins; i++) w[i] = cell_laplacian(dims, N, p->dx, lambda, i);
+    for(int64_t i = 0; i < bins; i++) lp[i] = cell_laplacian(dims, N, p->dx, psi, i);
+    for(int64_t i = 0; i < bins; i++) mu[i] = och_der_bulk_free_energy(m, psi[i]) - 2.0 * p->k * lp[i];
+    double ge = 0, gk = 0, gM = 0;
+    for(int64_t i = 0; i < bins; i++) {
+        ge += w[i] * psi[i];
+        gk += w[i] * lp[i];
+        gM += w[i] * mu[i];
+    }
+    grad[0] += -p->dt * p->M * ge;
+    grad[1] += -2.0 * p->dt * p->M * gk;
+    grad[2] += p->dt * gM;
+    for(int64_t i = 0; i < bins; i++) {
+        double Lw = cell_laplacian(dims, N, p->dx, w, i);
+        lambda[i] += p->dt * p->M * (och_second_der_bulk_free_energy(m, psi[i]) * w[i] - 2.0 * p->k * Lw);
+    }
+    free(w); free(lp); free(mu);
+}
+
+void och_euler_adjoint(const och_params *p, const och_model *m, const double *psi0, int64_t steps,
+                       const double *dL_dpsiT, double *dL_dpsi0, double *grad) {
+    int64_t bins = ipow(p->N, p->dims);
+    double *tape = (double *)malloc(sizeof(double) * bins * (steps + 1));
+    memcpy(tape, psi0, sizeof(double) * bins);
+    for(int64_t t = 0; t < steps; t++) {
+        memcpy(tape + (t + 1) * bins, tape + t * bins, sizeof(double) * bins);
+        och_euler_step(p, m, tape + (t + 1) * bins);
+    }
+    memcpy(dL_dpsi0, dL_dpsiT, sizeof(double) * bins);
+    grad[0] = grad[1] = grad[2] = 0;
+    for(int64_t t = steps - 1; t >= 0; t--) och_euler_adjoint_step(p, m, tape + t * bins, dL_dpsi0, grad);
+    free(tape);
+}

@@ -1,0 +1,105 @@
+/*
+ * ch_oracle.h -- FP64 CPU restatement of the cahn-hilliard time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product library never links
+ * or calls it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates.
+ * Layout everywhere: one species (Landau and SimpleWertheim are single-species
+ * models, src/models/Landau.cpp:40-42, src/models/SimpleWertheim.h:21-23),
+ * idx = x + N*y + N*N*z (src/CahnHilliard.cpp:245-253, MultiField.h:62).
+ */
+#ifndef CH_ORACLE_H
+#define CH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OCH_LANDAU = 0, OCH_SIMPLE_WERTHEIM = 1 };
+enum { OCH_MOB_CONSTANT = 0, OCH_MOB_FREE_ENERGY = 1, OCH_MOB_REGULARISED = 2, OCH_MOB_LINEAR = 3 };
+
+typedef struct och_model {
+    int model;                 /* OCH_LANDAU / OCH_SIMPLE_WERTHEIM */
+    double epsilon;            /* landau.epsilon                        */
+    double B2;                 /* wertheim.B2                           */
+    double delta;              /* wertheim.delta                        */
+    double regularisation_delta;
+    int valence;               /* wertheim.valence                      */
+} och_model;
+
+typedef struct och_params {
+    int dims, N;
+    double dt, dx, k;          /* k: the TOML "k" (the step uses 2k)    */
+    int mobility;              /* OCH_MOB_*                             */
+    double M;                  /* mobility.M                            */
+    double rho_min;            /* mobility.rho_min (regularised)        */
+    double rho_ref;            /* mobility.rho_ref (linear)             */
+    double S;                  /* pseudospectral.S                      */
+    int use_dealias;           /* pseudospectral.use_dealias            */
+} och_params;
+
+/* ---- POSIX drand48 (the reference seeds it at src/main.cpp:36) ---- */
+typedef struct och_rng { uint64_t x; } och_rng;
+void och_srand48(och_rng *r, int64_t seed);
+double och_drand48(och_rng *r);
+
+/* initial field, src/CahnHilliard.cpp:133-151 */
+void och_init_density(int dims, int N, double density, double A, int N_peaks, int64_t seed, double *rho);
+
+/* ---- models ---- */
+double och_bulk_free_energy(const och_model *m, double rho);
+double och_der_bulk_free_energy(const och_model *m, double rho);
+double och_second_der_bulk_free_energy(const och_model *m, double rho);
+double och_wertheim_X(const och_model *m, double rho);
+double och_pressure(const och_model *m, double rho);
+void och_der_bulk_free_energy_field(const och_model *m, const double *rho, double *out, int64_t n);
+
+/* mobility field from rho, src/mobility/*.h */
+void och_update_mobility(const och_params *p, const och_model *m, const double *rho, double *mob, int64_t n);
+
+/* ---- integrators: one step, in place ---- */
+void och_euler_step(const och_params *p, const och_model *m, double *rho);
+void och_euler_mobility_step(const och_params *p, const och_model *m, double *rho);
+
+/* pseudospectral: opaque state carrying rho_hat (src/integrators/PseudospectralCPU.h:45-53) */
+typedef struct och_ps och_ps;
+och_ps *och_ps_create(const och_params *p, const och_model *m, const double *rho);
+void och_ps_step(och_ps *s, double *rho);
+void och_ps_destroy(och_ps *s);
+int64_t och_ps_hat_size(const och_ps *s);
+void och_ps_get_hat(const och_ps *s, double *re_im);
+
+/* n steps of the selected integrator (0 euler, 1 euler_mobility, 2 pseudospectral) */
+int och_run(int integrator, const och_params *p, const och_model *m, double *rho, int64_t steps);
+
+/* ---- observables, src/CahnHilliard.cpp:285-319 ---- */
+double och_average_mass(const och_params *p, const double *rho);
+double och_average_free_energy(const och_params *p, const och_model *m, const double *rho);
+
+/* ---- FFT (FP64, radix-2, exact-index twiddles) ---- */
+void och_fft_c2c(int n, int inverse, double *re_im, int stride);
+void och_dft_brute(int n, int inverse, const double *in_re_im, double *out_re_im);
+
+/* ---- S(q), utils/sq_from_traj.py:72-103, 2D L x L ---- */
+int och_sq_nshells(int L);
+void och_radial_sq(int L, double dx, const double *field, double *Sq);
+double och_sq_loss(int L, double dx, const double *field, const double *Sq_target, double *grad_field);
+
+/* ---- FP64 adjoint of the Landau Euler step (constant mobility) ----
+ * forward: psi' = psi + dt*M*Lap(mu), mu = f'(psi) - 2k Lap(psi)
+ * given lambda' = dLoss/dpsi', returns lambda = dLoss/dpsi in place and
+ * accumulates dLoss/d(eps, k, M) into grad[0..2]. */
+void och_euler_adjoint_step(const och_params *p, const och_model *m, const double *psi, double *lambda, double *grad);
+/* full forward + reverse sweep with every state stored: loss = <w, psi_T> */
+void och_euler_adjoint(const och_params *p, const och_model *m, const double *psi0, int64_t steps,
+                       const double *dL_dpsiT, double *dL_dpsi0, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
