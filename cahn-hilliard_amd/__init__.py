@@ -1,0 +1,15 @@
+"""cahn-hilliard_amd: MI355X-native time-stepping for the cahn-hilliard phase-field code.
+
+The compute path is libch_hip.so (hand-written gfx950 HIP kernels behind the C
+ABI in include/ch_hip.h).  This Python package is a thin ctypes layer used by
+the tests, the benchmark and the autograd wrapper; the drop-in C++ host code
+lives in host/.
+"""
+from . import _lib
+from ._lib import (CHError, EULER, EULER_MOBILITY, FP32, FP64, LANDAU, PSEUDOSPECTRAL, SIMPLE_WERTHEIM,
+                   header_functions)
+from .system import Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility
+
+__all__ = ["CHError", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
+           "set_mobility", "header_functions", "EULER", "EULER_MOBILITY", "PSEUDOSPECTRAL", "LANDAU",
+           "SIMPLE_WERTHEIM", "FP32", "FP64"]

@@ -1,0 +1,97 @@
+"""ctypes bindings of include/ch_hip.h (libch_hip.so, built for gfx950).
+
+There is no CPU fallback: if the shared library is missing, or no GPU is
+visible, the calls raise.
+"""
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libch_hip.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "ch_hip.h")
+
+CH_OK, CH_EINVAL, CH_EHIP, CH_ENAN, CH_EUNSUPPORTED, CH_ENOMEM, CH_ESTATE = range(7)
+FP64, FP32 = 0, 1
+EULER, EULER_MOBILITY, PSEUDOSPECTRAL = 0, 1, 2
+LANDAU, SIMPLE_WERTHEIM = 0, 1
+MOB_CONSTANT, MOB_FREE_ENERGY, MOB_REGULARISED, MOB_LINEAR = 0, 1, 2, 3
+
+INTEGRATORS = {"euler": EULER, "euler_mobility": EULER_MOBILITY, "pseudospectral": PSEUDOSPECTRAL}
+MODELS = {"landau": LANDAU, "simple_wertheim": SIMPLE_WERTHEIM}
+MOBILITIES = {"constant": MOB_CONSTANT, "free_energy": MOB_FREE_ENERGY, "regularised": MOB_REGULARISED,
+              "linear": MOB_LINEAR}
+
+
+class ChDesc(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32), ("N", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("integrator", ctypes.c_int32), ("model", ctypes.c_int32), ("mobility", ctypes.c_int32),
+                ("valence", ctypes.c_int32), ("use_dealias", ctypes.c_int32),
+                ("epsilon", ctypes.c_double), ("B2", ctypes.c_double), ("delta", ctypes.c_double),
+                ("regularisation_delta", ctypes.c_double), ("M", ctypes.c_double), ("rho_min", ctypes.c_double),
+                ("rho_ref", ctypes.c_double), ("dt", ctypes.c_double), ("dx", ctypes.c_double),
+                ("k", ctypes.c_double), ("S", ctypes.c_double)]
+
+
+class CHError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"ch status {status}: {msg}")
+        self.status = status
+
+
+def header_functions(path=HEADER):
+    """Names of every function the public header declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ch_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    # torch ships its own libamdhip64.so.7 (same SONAME as /opt/rocm's).  Loading
+    # it first makes this library bind to that runtime, so the process holds one
+    # HIP/HSA runtime; loading ours first would make torch open a second one,
+    # which then sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I64, I32, D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    DP = ctypes.POINTER(ChDesc)
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    DD = ctypes.POINTER(ctypes.c_double)
+    sigs = {
+        "ch_abi_version": (ctypes.c_int, []),
+        "ch_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "ch_ctx_create": (ctypes.c_int, [ctypes.c_int, PP]),
+        "ch_ctx_destroy": (None, [P]),
+        "ch_ctx_last_error": (ctypes.c_char_p, [P]),
+        "ch_ctx_synchronize": (ctypes.c_int, [P]),
+        "ch_ctx_stream": (P, [P]),
+        "ch_sys_create": (ctypes.c_int, [P, DP, PP]),
+        "ch_sys_destroy": (None, [P]),
+        "ch_sys_upload": (ctypes.c_int, [P, P]),
+        "ch_sys_download": (ctypes.c_int, [P, P]),
+        "ch_sys_upload_device": (ctypes.c_int, [P, P]),
+        "ch_sys_download_device": (ctypes.c_int, [P, P]),
+        "ch_sys_download_mobility": (ctypes.c_int, [P, P]),
+        "ch_sys_evolve": (ctypes.c_int, [P, I64]),
+        "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
+        "ch_der_bulk_free_energy": (ctypes.c_int, [P, DP, P, P, I64]),
+        "ch_set_mobility": (ctypes.c_int, [P, DP, P, P, I64]),
+    }
+    for name, (res, args) in sigs.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+    _lib = L
+    return L

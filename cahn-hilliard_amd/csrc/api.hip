@@ -1,0 +1,439 @@
+// api.hip -- the C ABI (include/ch_hip.h): contexts, systems, step loop.
+//
+// A ch_sys owns its device buffers (the reference's CUDAIntegrator owns
+// _d_rho, _d_rho_der and _d_mobility, CUDAIntegrator.cu:13-39).  The step
+// loop replays a captured hipGraph of G steps (ping-pong buffers, G even) so
+// the host issues one launch per G steps; the remainder runs eagerly.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ch_internal.h"
+
+struct ch_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct ch_sys {
+    ch_ctx *ctx = nullptr;
+    ch_desc d{};
+    chd::Params p{};
+    int64_t bins = 0, hat = 0;
+    size_t elt = 8;
+    void *psi[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *work = nullptr, *rho_hat = nullptr, *twiddles = nullptr;
+    void *stage = nullptr;   // bins doubles: host transfers of FP32 fields, mobility
+    void *stage2 = nullptr;  // bins elements of T: mobility
+    double *partials = nullptr, *result = nullptr;
+    int graph_steps = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+};
+
+static ch_status fail(ch_ctx *ctx, ch_status st, const char *fmt, ...) {
+    if(ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return st;
+}
+
+#define HIPCHK(ctx, call)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if(e_ != hipSuccess) return fail((ctx), CH_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while(0)
+
+extern "C" int ch_abi_version(void) { return CH_ABI_VERSION; }
+
+extern "C" const char *ch_status_string(ch_status s) {
+    switch(s) {
+    case CH_OK: return "ok";
+    case CH_EINVAL: return "invalid argument";
+    case CH_EHIP: return "HIP runtime error";
+    case CH_ENAN: return "NaN encountered";
+    case CH_EUNSUPPORTED: return "unsupported";
+    case CH_ENOMEM: return "out of device memory";
+    case CH_ESTATE: return "invalid call order";
+    }
+    return "unknown";
+}
+
+extern "C" ch_status ch_ctx_create(int device, ch_ctx **out) {
+    if(!out) return CH_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess || n == 0) return CH_EHIP;
+    if(device < 0 || device >= n) return CH_EINVAL;
+    ch_ctx *c = new ch_ctx();
+    c->device = device;
+    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CH_EHIP;
+    }
+    *out = c;
+    return CH_OK;
+}
+
+extern "C" void ch_ctx_destroy(ch_ctx *ctx) {
+    if(!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if(ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+extern "C" const char *ch_ctx_last_error(const ch_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" ch_status ch_ctx_synchronize(ch_ctx *ctx) {
+    if(!ctx) return CH_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CH_OK;
+}
+
+extern "C" void *ch_ctx_stream(ch_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ------------------------------------------------------------------ helpers
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
+    if(!d) return fail(ctx, CH_EINVAL, "null descriptor");
+    if(d->dims < 1 || d->dims > 3) return fail(ctx, CH_EINVAL, "dims must be 1, 2 or 3 (got %d)", d->dims);
+    if(!is_pow2(d->N)) return fail(ctx, CH_EINVAL, "N should be a power of 2 (got %d)", d->N);
+    if(d->N < 4) return fail(ctx, CH_EUNSUPPORTED, "N must be at least 4 (got %d)", d->N);
+    if(d->precision != CH_FP64 && d->precision != CH_FP32) return fail(ctx, CH_EINVAL, "bad precision %d", d->precision);
+    if(d->model != CH_LANDAU && d->model != CH_SIMPLE_WERTHEIM) return fail(ctx, CH_EINVAL, "bad model %d", d->model);
+    if(d->integrator < CH_EULER || d->integrator > CH_PSEUDOSPECTRAL)
+        return fail(ctx, CH_EINVAL, "bad integrator %d", d->integrator);
+    if(d->mobility < CH_MOB_CONSTANT || d->mobility > CH_MOB_LINEAR)
+        return fail(ctx, CH_EINVAL, "bad mobility type %d", d->mobility);
+    // Integrator::validate (src/integrators/Integrator.cpp:42-46)
+    if(d->integrator != CH_EULER_MOBILITY && d->mobility != CH_MOB_CONSTANT)
+        return fail(ctx, CH_EINVAL, "The selected integrator only supports constant mobility");
+    // FreeEnergyModel::set_mobility default (src/models/FreeEnergyModel.h:51-53)
+    if(d->mobility == CH_MOB_FREE_ENERGY && d->model != CH_SIMPLE_WERTHEIM)
+        return fail(ctx, CH_EUNSUPPORTED, "this model does not support mobility coupling");
+    if(d->mobility == CH_MOB_LINEAR && d->rho_ref == 0.0) return fail(ctx, CH_EINVAL, "linear mobility needs rho_ref != 0");
+    if(d->model == CH_SIMPLE_WERTHEIM && (d->valence <= 0 || !(d->delta > 0.0)))
+        return fail(ctx, CH_EINVAL, "simple_wertheim needs valence > 0 and delta > 0");
+    if(!(d->dt > 0.0) || !(d->dx > 0.0)) return fail(ctx, CH_EINVAL, "dt and dx must be positive");
+    if(d->integrator == CH_PSEUDOSPECTRAL && d->N > 4096)
+        return fail(ctx, CH_EUNSUPPORTED, "pseudospectral supports N <= 4096 (got %d)", d->N);
+    int64_t bins = 1;
+    for(int i = 0; i < d->dims; i++) bins *= d->N;
+    if(bins > ((int64_t)1 << 31)) return fail(ctx, CH_EUNSUPPORTED, "grid too large (%lld cells)", (long long)bins);
+    return CH_OK;
+}
+
+static chd::Params make_params(const ch_desc *d) {
+    chd::Params p{};
+    p.N = d->N;
+    p.mask = d->N - 1;
+    p.log2N = 0;
+    while((1 << p.log2N) < d->N) p.log2N++;
+    p.dims = d->dims;
+    p.valence = d->valence;
+    p.use_dealias = d->use_dealias;
+    p.eps = d->epsilon;
+    p.B2 = d->B2;
+    p.reg = d->regularisation_delta;
+    p.log_reg = std::log(d->regularisation_delta);
+    p.tvd = 2.0 * d->valence * d->delta;
+    p.M = d->M;
+    p.rho_min = d->rho_min;
+    p.rho_ref = d->rho_ref;
+    p.k = d->k;
+    p.two_k = 2.0 * d->k;
+    p.dx = d->dx;
+    p.inv_dx = 1.0 / d->dx;
+    p.inv_dx2 = 1.0 / (d->dx * d->dx);
+    p.dt = d->dt;
+    p.S = d->S;
+    double bins = std::pow((double)d->N, d->dims);
+    p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
+    p.inv_bins = 1.0 / bins;
+    return p;
+}
+
+// e^{-2 pi i m / N} from the integer index: octant reduction, long double
+// sin/cos on [0, pi/4]; exact 0 and +-1 at the axes.
+static void twiddle(int m, int N, long double &c, long double &s) {
+    const long double PI = 3.141592653589793238462643383279502884L;
+    m = ((m % N) + N) % N;
+    int sign = 1;
+    if(m >= N / 2) { m -= N / 2; sign = -1; }
+    long double cc, ss;
+    if(4 * m <= N / 2) {  // [0, pi/4]
+        long double a = 2 * PI * m / N;
+        cc = cosl(a); ss = sinl(a);
+    }
+    else if(4 * m < N) {  // (pi/4, pi/2)
+        long double a = 2 * PI * (N / 4.0L - m) / N;
+        cc = sinl(a); ss = cosl(a);
+    }
+    else if(4 * m == N) { cc = 0; ss = 1; }
+    else if(4 * m <= 3 * N / 2) {  // (pi/2, 3pi/4]
+        long double a = 2 * PI * (m - N / 4.0L) / N;
+        cc = -sinl(a); ss = cosl(a);
+    }
+    else {  // (3pi/4, pi)
+        long double a = 2 * PI * (N / 2.0L - m) / N;
+        cc = -cosl(a); ss = sinl(a);
+    }
+    c = sign * cc;
+    s = -sign * ss;  // forward sign
+}
+
+static void free_sys(ch_sys *s) {
+    for(int i = 0; i < 2; i++) {
+        if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
+        if(s->psi[i]) (void)hipFree(s->psi[i]);
+    }
+    void *bufs[] = {s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result};
+    for(void *b : bufs)
+        if(b) (void)hipFree(b);
+    delete s;
+}
+
+extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **out) {
+    if(!ctx || !out) return CH_EINVAL;
+    *out = nullptr;
+    ch_status st = validate(ctx, desc);
+    if(st != CH_OK) return st;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ch_sys *s = new ch_sys();
+    s->ctx = ctx;
+    s->d = *desc;
+    s->p = make_params(desc);
+    s->bins = 1;
+    for(int i = 0; i < desc->dims; i++) s->bins *= desc->N;
+    s->elt = desc->precision == CH_FP64 ? 8 : 4;
+    const int N = desc->N;
+    s->hat = s->bins / N * (N / 2 + 1);
+    const char *g = std::getenv("CH_GRAPH_STEPS");
+    s->graph_steps = g ? std::atoi(g) : 64;
+    if(s->graph_steps < 0) s->graph_steps = 0;
+    s->graph_steps &= ~1;
+
+    auto alloc = [&](void **p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
+    bool ok = alloc(&s->psi[0], s->bins * s->elt) && alloc(&s->stage, s->bins * 8) &&
+              alloc(&s->stage2, s->bins * s->elt) && alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
+              alloc((void **)&s->result, 2 * sizeof(double));
+    if(ok && desc->integrator != CH_PSEUDOSPECTRAL) ok = alloc(&s->psi[1], s->bins * s->elt);
+    if(ok && desc->integrator == CH_PSEUDOSPECTRAL) {
+        ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->rho_hat, s->hat * 2 * s->elt) &&
+             alloc(&s->twiddles, (size_t)N * 2 * s->elt);
+        if(ok) {
+            std::vector<double> twd(2 * N);
+            std::vector<float> twf(2 * N);
+            for(int m = 0; m < N; m++) {
+                long double c, sn;
+                twiddle(m, N, c, sn);
+                twd[2 * m] = (double)c; twd[2 * m + 1] = (double)sn;
+                twf[2 * m] = (float)twd[2 * m]; twf[2 * m + 1] = (float)twd[2 * m + 1];
+            }
+            const void *src = desc->precision == CH_FP64 ? (const void *)twd.data() : (const void *)twf.data();
+            ok = hipMemcpy(s->twiddles, src, (size_t)N * 2 * s->elt, hipMemcpyHostToDevice) == hipSuccess;
+        }
+    }
+    if(!ok) {
+        free_sys(s);
+        return fail(ctx, CH_ENOMEM, "device allocation failed for N=%d dims=%d", desc->N, desc->dims);
+    }
+    *out = s;
+    return CH_OK;
+}
+
+extern "C" void ch_sys_destroy(ch_sys *s) {
+    if(!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_sys(s);
+}
+
+static chd::SpectralArgs spectral_args(ch_sys *s) {
+    chd::SpectralArgs a;
+    a.p = s->p;
+    a.precision = s->d.precision;
+    a.model = s->d.model;
+    a.rho = s->psi[0];
+    a.work = s->work;
+    a.rho_hat = s->rho_hat;
+    a.twiddles = s->twiddles;
+    return a;
+}
+
+static ch_status after_upload(ch_sys *s) {
+    ch_ctx *c = s->ctx;
+    s->cur = 0;
+    if(s->d.integrator == CH_PSEUDOSPECTRAL) {
+        chd::SpectralArgs a = spectral_args(s);
+        HIPCHK(c, chd::ps_run(a, c->stream, true));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_upload(ch_sys *s, const double *rho_host) {
+    if(!s || !rho_host) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if(s->d.precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_host, s->bins * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    else {
+        HIPCHK(c, hipMemcpyAsync(s->stage, rho_host, s->bins * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, chd::convert_d2f((const double *)s->stage, (float *)s->psi[0], s->bins, c->stream));
+    }
+    return after_upload(s);
+}
+
+extern "C" ch_status ch_sys_upload_device(ch_sys *s, const void *rho_dev) {
+    if(!s || !rho_dev) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_dev, s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
+    return after_upload(s);
+}
+
+extern "C" ch_status ch_sys_download(ch_sys *s, double *rho_host) {
+    if(!s || !rho_host) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if(s->d.precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(rho_host, s->psi[s->cur], s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    else {
+        HIPCHK(c, chd::convert_f2d((const float *)s->psi[s->cur], (double *)s->stage, s->bins, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rho_host, s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_download_device(ch_sys *s, void *rho_dev) {
+    if(!s || !rho_dev) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(rho_dev, s->psi[s->cur], s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_download_mobility(ch_sys *s, double *mob_host) {
+    if(!s || !mob_host) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, chd::pointwise_mobility(s->p, s->d.precision, s->d.mobility, s->psi[s->cur], s->stage2, s->bins, c->stream));
+    if(s->d.precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(mob_host, s->stage2, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    else {
+        HIPCHK(c, chd::convert_f2d((const float *)s->stage2, (double *)s->stage, s->bins, c->stream));
+        HIPCHK(c, hipMemcpyAsync(mob_host, s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+// one time step queued on the stream
+static hipError_t one_step(ch_sys *s, int &cur) {
+    hipStream_t st = s->ctx->stream;
+    if(s->d.integrator == CH_PSEUDOSPECTRAL) {
+        chd::SpectralArgs a = spectral_args(s);
+        return chd::ps_run(a, st, false);
+    }
+    const void *in = s->psi[cur];
+    void *out = s->psi[cur ^ 1];
+    hipError_t e;
+    switch(s->d.dims) {
+    case 1: e = chd::stencil_step<1>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
+    case 2: e = chd::stencil_step<2>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
+    default: e = chd::stencil_step<3>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
+    }
+    cur ^= 1;
+    return e;
+}
+
+extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
+    if(!s || nsteps < 0) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int G = s->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        const int par = s->cur;
+        if(!s->gexec[par]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int cur = s->cur;
+            hipError_t e = hipSuccess;
+            for(int i = 0; i < G && e == hipSuccess; i++) e = one_step(s, cur);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&s->gexec[par], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[par], c->stream));
+        nsteps -= reps * G;  // G is even: the ping-pong parity is unchanged
+    }
+    for(int64_t t = 0; t < nsteps; t++) HIPCHK(c, one_step(s, s->cur));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_energy) {
+    if(!s) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts, s->result,
+                               c->stream));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, s->result, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double m = h[0] * s->p.vbin_over_bins, f = h[1] * s->p.vbin_over_bins;
+    if(mass) *mass = m;
+    if(free_energy) *free_energy = f;
+    if(std::isnan(h[0])) return fail(c, CH_ENAN, "Encountered a nan while computing the total mass");
+    if(std::isnan(h[1])) return fail(c, CH_ENAN, "Encountered a nan while computing the total free energy");
+    return CH_OK;
+}
+
+extern "C" ch_status ch_der_bulk_free_energy(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev, void *out_dev,
+                                             int64_t n) {
+    if(!ctx || !desc || !rho_dev || !out_dev || n < 0) return CH_EINVAL;
+    if(desc->model != CH_LANDAU && desc->model != CH_SIMPLE_WERTHEIM) return fail(ctx, CH_EINVAL, "bad model");
+    if(desc->precision != CH_FP64 && desc->precision != CH_FP32) return fail(ctx, CH_EINVAL, "bad precision");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    chd::Params p = make_params(desc);
+    HIPCHK(ctx, chd::pointwise_fprime(p, desc->precision, desc->model, rho_dev, out_dev, n, ctx->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_set_mobility(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev, void *mob_dev, int64_t n) {
+    if(!ctx || !desc || !rho_dev || !mob_dev || n < 0) return CH_EINVAL;
+    if(desc->mobility < CH_MOB_CONSTANT || desc->mobility > CH_MOB_LINEAR) return fail(ctx, CH_EINVAL, "bad mobility");
+    if(desc->precision != CH_FP64 && desc->precision != CH_FP32) return fail(ctx, CH_EINVAL, "bad precision");
+    if(desc->mobility == CH_MOB_FREE_ENERGY && desc->model != CH_SIMPLE_WERTHEIM)
+        return fail(ctx, CH_EUNSUPPORTED, "this model does not support mobility coupling");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    chd::Params p = make_params(desc);
+    HIPCHK(ctx, chd::pointwise_mobility(p, desc->precision, desc->mobility, rho_dev, mob_dev, n, ctx->stream));
+    return CH_OK;
+}

@@ -1,0 +1,39 @@
+// ch_internal.h -- host-side declarations shared by the library's .hip files.
+#pragma once
+
+#include "ch_device.h"
+
+namespace chd {
+
+// one fused Euler / EulerMobility step, in -> out (distinct buffers)
+template <int DIMS>
+hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, const void *in, void *out,
+                        hipStream_t s);
+
+// pseudospectral pipeline
+struct SpectralArgs {
+    Params p;
+    int precision;
+    int model;
+    void *rho;            // N^d real field (T)
+    void *work;           // half spectrum scratch (T2)
+    void *rho_hat;        // half spectrum state (T2)
+    const void *twiddles; // W_N^m, m = 0..N-1 (T2)
+};
+// init = true: rho_hat = r2c(rho); otherwise one time step
+hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init);
+
+// FP64 observables: partials -> result[0] = sum rho, result[1] = sum f-density
+hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
+                       double *result, hipStream_t s);
+constexpr int kObsParts = 1024;
+
+hipError_t pointwise_fprime(const Params &p, int precision, int model, const void *in, void *out, int64_t n,
+                            hipStream_t s);
+hipError_t pointwise_mobility(const Params &p, int precision, int mob, const void *in, void *out, int64_t n,
+                              hipStream_t s);
+// double <-> float conversions on device
+hipError_t convert_d2f(const double *in, float *out, int64_t n, hipStream_t s);
+hipError_t convert_f2d(const float *in, double *out, int64_t n, hipStream_t s);
+
+}  // namespace chd

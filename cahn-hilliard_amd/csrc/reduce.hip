@@ -1,0 +1,161 @@
+// reduce.hip -- FP64 observables and pointwise model kernels.
+//
+// average_mass and average_free_energy (src/CahnHilliard.cpp:285-319) are
+// evaluated on the device in FP64 whatever the field precision: each thread
+// accumulates a grid-stride slice, waves combine with 64-lane shuffles, waves
+// of a block through LDS, and a single-block pass sums the per-block partials
+// in a fixed order (deterministic, no atomics).  Only two doubles return to
+// the host, instead of the reference's blocking copy of the whole field and a
+// serial host loop (CUDAIntegrator.cu:55-70).
+#include "ch_internal.h"
+
+namespace chd {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T, int MODEL>
+__global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, Params p, double *__restrict__ partials) {
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    const int N = p.N, mask = p.mask, lg = p.log2N;
+    double m = 0, f = 0;
+    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256) {
+        const double r = (double)rho[i];
+        m += r;
+        // forward-difference gradient (CahnHilliard.cpp:256-276)
+        const int x = (int)(i & mask);
+        double inter = 0.;
+        {
+            const int64_t ip = (i - x) + ((x + 1) & mask);
+            const double g = ((double)rho[ip] - r) * p.inv_dx;
+            inter += p.k * g * g;
+        }
+        if(p.dims >= 2) {
+            const int y = (int)((i >> lg) & mask);
+            const int64_t ip = i + ((int64_t)(((y + 1) & mask) - y) << lg);
+            const double g = ((double)rho[ip] - r) * p.inv_dx;
+            inter += p.k * g * g;
+        }
+        if(p.dims == 3) {
+            const int z = (int)(i >> (2 * lg));
+            const int64_t ip = i + ((int64_t)(((z + 1) & mask) - z) << (2 * lg));
+            const double g = ((double)rho[ip] - r) * p.inv_dx;
+            inter += p.k * g * g;
+        }
+        (void)N;
+        f += bulk<MODEL>(p, r) + inter;
+    }
+    __shared__ double sm[4], sf[4];
+    m = wave_sum(m);
+    f = wave_sum(f);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if(lane == 0) { sm[w] = m; sf[w] = f; }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+        partials[2 * blockIdx.x + 1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nparts,
+                                                  double *__restrict__ result) {
+    double m = 0, f = 0;
+    for(int i = threadIdx.x; i < nparts; i += 256) {
+        m += partials[2 * i];
+        f += partials[2 * i + 1];
+    }
+    __shared__ double sm[4], sf[4];
+    m = wave_sum(m);
+    f = wave_sum(f);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if(lane == 0) { sm[w] = m; sf[w] = f; }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        result[0] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+        result[1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+    }
+}
+
+hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
+                       double *result, hipStream_t s) {
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    int nb = (int)std::min<int64_t>(nparts, (bins + 255) / 256);
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<double, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
+        else hipLaunchKernelGGL((k_observables<double, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
+    }
+    else {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<float, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
+        else hipLaunchKernelGGL((k_observables<float, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, result);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- pointwise
+template <typename T, int MODEL>
+__global__ void k_fprime(const T *__restrict__ in, T *__restrict__ out, Params p, int64_t n) {
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = fprime<T, MODEL>(p, in[i]);
+}
+
+template <typename T, int MOB>
+__global__ void k_mobility(const T *__restrict__ in, T *__restrict__ out, Params p, int64_t n) {
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = mobility<T, MOB>(p, in[i]);
+}
+
+static int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
+
+hipError_t pointwise_fprime(const Params &p, int precision, int model, const void *in, void *out, int64_t n,
+                            hipStream_t s) {
+    const int g = grid_for(n);
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_fprime<double, CH_LANDAU>), dim3(g), dim3(256), 0, s, (const double *)in, (double *)out, p, n);
+        else hipLaunchKernelGGL((k_fprime<double, CH_SIMPLE_WERTHEIM>), dim3(g), dim3(256), 0, s, (const double *)in, (double *)out, p, n);
+    }
+    else {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_fprime<float, CH_LANDAU>), dim3(g), dim3(256), 0, s, (const float *)in, (float *)out, p, n);
+        else hipLaunchKernelGGL((k_fprime<float, CH_SIMPLE_WERTHEIM>), dim3(g), dim3(256), 0, s, (const float *)in, (float *)out, p, n);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static void launch_mob(const Params &p, int mob, const T *in, T *out, int64_t n, hipStream_t s) {
+    const int g = grid_for(n);
+    switch(mob) {
+    case CH_MOB_CONSTANT: hipLaunchKernelGGL((k_mobility<T, CH_MOB_CONSTANT>), dim3(g), dim3(256), 0, s, in, out, p, n); break;
+    case CH_MOB_FREE_ENERGY: hipLaunchKernelGGL((k_mobility<T, CH_MOB_FREE_ENERGY>), dim3(g), dim3(256), 0, s, in, out, p, n); break;
+    case CH_MOB_REGULARISED: hipLaunchKernelGGL((k_mobility<T, CH_MOB_REGULARISED>), dim3(g), dim3(256), 0, s, in, out, p, n); break;
+    default: hipLaunchKernelGGL((k_mobility<T, CH_MOB_LINEAR>), dim3(g), dim3(256), 0, s, in, out, p, n); break;
+    }
+}
+
+hipError_t pointwise_mobility(const Params &p, int precision, int mob, const void *in, void *out, int64_t n,
+                              hipStream_t s) {
+    if(precision == CH_FP64) launch_mob<double>(p, mob, (const double *)in, (double *)out, n, s);
+    else launch_mob<float>(p, mob, (const float *)in, (float *)out, n, s);
+    return hipGetLastError();
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ in, B *__restrict__ out, int64_t n) {
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (B)in[i];
+}
+
+hipError_t convert_d2f(const double *in, float *out, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t convert_f2d(const float *in, double *out, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace chd

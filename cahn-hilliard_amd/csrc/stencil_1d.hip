@@ -1,0 +1,10 @@
+// stencil_1d.hip -- 1D instantiations of the fused stencil step.
+#include "ch_internal.h"
+#include "stencil_dispatch.h"
+
+namespace chd {
+template <> hipError_t stencil_step<1>(const Params &p, int precision, int integrator, int model, int mob,
+                                         const void *in, void *out, hipStream_t s) {
+    return stencil_step_dims<1>(p, precision, integrator, model, mob, in, out, s);
+}
+}  // namespace chd

@@ -1,0 +1,10 @@
+// stencil_2d.hip -- 2D instantiations of the fused stencil step.
+#include "ch_internal.h"
+#include "stencil_dispatch.h"
+
+namespace chd {
+template <> hipError_t stencil_step<2>(const Params &p, int precision, int integrator, int model, int mob,
+                                         const void *in, void *out, hipStream_t s) {
+    return stencil_step_dims<2>(p, precision, integrator, model, mob, in, out, s);
+}
+}  // namespace chd

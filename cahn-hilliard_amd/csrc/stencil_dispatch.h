@@ -1,0 +1,56 @@
+// stencil_dispatch.h -- runtime descriptor -> k_stencil instantiation.
+#pragma once
+
+#include "stencil.h"
+
+namespace chd {
+
+// Tile shapes (TX, TY, TZ) per dimension; the largest that fits N is used.
+//   1D: 256 | 64 | 16 | 4      2D: 64x16 | 16x16 | 4x4      3D: 16x8x8 | 4x4x4
+template <typename T, int DIMS, int MODEL, int MOB, bool EM, int K>
+hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t s) {
+    const int N = p.N;
+    if constexpr(DIMS == 1) {
+        if(N >= 256) return launch_stencil_cfg<T, 1, MODEL, MOB, EM, 256, 1, 1, K>(p, in, out, s);
+        if(N >= 64) return launch_stencil_cfg<T, 1, MODEL, MOB, EM, 64, 1, 1, K>(p, in, out, s);
+        if(N >= 16) return launch_stencil_cfg<T, 1, MODEL, MOB, EM, 16, 1, 1, K>(p, in, out, s);
+        return launch_stencil_cfg<T, 1, MODEL, MOB, EM, 4, 1, 1, K>(p, in, out, s);
+    }
+    else if constexpr(DIMS == 2) {
+        if(N >= 64) return launch_stencil_cfg<T, 2, MODEL, MOB, EM, 64, 16, 1, K>(p, in, out, s);
+        if(N >= 16) return launch_stencil_cfg<T, 2, MODEL, MOB, EM, 16, 16, 1, K>(p, in, out, s);
+        return launch_stencil_cfg<T, 2, MODEL, MOB, EM, 4, 4, 1, K>(p, in, out, s);
+    }
+    else {
+        if(N >= 16) return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 16, 8, 8, K>(p, in, out, s);
+        return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 4, 4, 4, K>(p, in, out, s);
+    }
+}
+
+template <typename T, int DIMS, int MODEL>
+hipError_t stencil_model(const Params &p, int integrator, int mob, const void *in, void *out, hipStream_t s) {
+    if(integrator == CH_EULER) return stencil_tiles<T, DIMS, MODEL, CH_MOB_CONSTANT, false, 1>(p, in, out, s);
+    switch(mob) {
+    case CH_MOB_CONSTANT: return stencil_tiles<T, DIMS, MODEL, CH_MOB_CONSTANT, true, 1>(p, in, out, s);
+    case CH_MOB_REGULARISED: return stencil_tiles<T, DIMS, MODEL, CH_MOB_REGULARISED, true, 1>(p, in, out, s);
+    case CH_MOB_LINEAR: return stencil_tiles<T, DIMS, MODEL, CH_MOB_LINEAR, true, 1>(p, in, out, s);
+    case CH_MOB_FREE_ENERGY:
+        if constexpr(MODEL == CH_SIMPLE_WERTHEIM)
+            return stencil_tiles<T, DIMS, MODEL, CH_MOB_FREE_ENERGY, true, 1>(p, in, out, s);
+        return hipErrorInvalidValue;
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int DIMS>
+hipError_t stencil_step_dims(const Params &p, int precision, int integrator, int model, int mob, const void *in,
+                             void *out, hipStream_t s) {
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) return stencil_model<double, DIMS, CH_LANDAU>(p, integrator, mob, in, out, s);
+        return stencil_model<double, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, out, s);
+    }
+    if(model == CH_LANDAU) return stencil_model<float, DIMS, CH_LANDAU>(p, integrator, mob, in, out, s);
+    return stencil_model<float, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, out, s);
+}
+
+}  // namespace chd

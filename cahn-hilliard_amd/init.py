@@ -1,0 +1,49 @@
+"""Initial fields with the reference's recipe (src/CahnHilliard.cpp:133-151).
+
+srand48(seed) (src/main.cpp:36), then one drand48() draw per bin, x fastest:
+    rho = density (1 + 2 A cos(k0 bin) r)   or   2 A cos(k0 bin) r   (density == 0)
+    r = drand48() - 0.5 (initial_N_peaks == 0) else 1 + 0.02 (drand48() - 0.5)
+drand48 is the POSIX 48-bit LCG x' = (0x5DEECE66D x + 0xB) mod 2^48.  The stream
+is generated with numpy in blocks of B values using the B-step jump
+x_{n+B} = a^B x_n + c (a^B - 1)/(a - 1) mod 2^48; uint64 products wrap mod 2^64,
+which 2^48 divides, so masking gives the exact residue.
+"""
+import math
+
+import numpy as np
+
+_A, _C, _MASK = 0x5DEECE66D, 0xB, (1 << 48) - 1
+
+
+def drand48_stream(seed, n, block=4096):
+    x = (((seed & 0xFFFFFFFF) << 16) | 0x330E) & _MASK
+    first = np.empty(min(block, n), dtype=np.uint64)
+    for i in range(first.size):
+        x = (_A * x + _C) & _MASK
+        first[i] = x
+    out = np.empty(n, dtype=np.uint64)
+    out[:first.size] = first
+    if n > block:
+        aB, cB = 1, 0
+        for _ in range(block):  # compose x -> a x + c, block times
+            aB, cB = (_A * aB) & _MASK, (_A * cB + _C) & _MASK
+        aB, cB, mask = np.uint64(aB), np.uint64(cB), np.uint64(_MASK)
+        cur = first
+        pos = block
+        while pos < n:
+            cur = (cur * aB + cB) & mask
+            m = min(block, n - pos)
+            out[pos:pos + m] = cur[:m]
+            pos += m
+    return out.astype(np.float64) / float(1 << 48)
+
+
+def initial_density(dims, N, density=0.0, A=1e-2, N_peaks=0, seed=123456):
+    bins = N ** dims
+    d = drand48_stream(seed, bins)
+    r = (d - 0.5) if N_peaks == 0 else 1.0 + 0.02 * (d - 0.5)
+    k0 = 2 * math.pi * N_peaks / N
+    modulation = A * np.cos(k0 * np.arange(bins, dtype=np.float64))
+    if density != 0.0:
+        return density * (1.0 + 2.0 * modulation * r)
+    return 2.0 * modulation * r

@@ -1,0 +1,192 @@
+"""Python handles over the C ABI: Context (device + stream) and System.
+
+`System` plays the role of the reference's CahnHilliard<dims> object driving an
+Integrator (src/CahnHilliard.h:24-62): evolve() == CahnHilliard::evolve called
+n times, sync()/download() == Integrator::sync, average_mass() /
+average_free_energy() == the FP64 observables (src/CahnHilliard.cpp:285-319).
+`desc_from_config` reads the reference's TOML keys (README.md:24-53 and the
+_config_* calls in Integrator.cpp:11-28, Landau.cpp:20-33,
+SimpleWertheim.cpp:16-33, IMobility.cpp:18-57, PseudospectralCPU.cpp:23-25).
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _check(ctx_handle, status):
+    if status != L.CH_OK:
+        msg = L.lib().ch_ctx_last_error(ctx_handle) if ctx_handle else b""
+        raise L.CHError(status, (msg or b"").decode() or L.lib().ch_status_string(status).decode())
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        st = L.lib().ch_ctx_create(device, ctypes.byref(h))
+        if st != L.CH_OK:
+            raise L.CHError(st, f"ch_ctx_create(device={device}) failed (is a GPU visible?)")
+        self.handle = h
+
+    def synchronize(self):
+        _check(self.handle, L.lib().ch_ctx_synchronize(self.handle))
+
+    @property
+    def stream(self):
+        return L.lib().ch_ctx_stream(self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().ch_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64", dt=0.01, dx=1.0, k=1.0,
+              epsilon=0.9, B2=1.0, delta=1e4, valence=4, regularisation_delta=0.0, mobility="constant", M=1.0,
+              rho_min=0.0, rho_ref=1.0, S=0.0, use_dealias=False):
+    d = L.ChDesc()
+    d.dims, d.N = dims, N
+    d.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision] if isinstance(precision, str) else precision
+    d.integrator = L.INTEGRATORS[integrator] if isinstance(integrator, str) else integrator
+    d.model = L.MODELS[free_energy] if isinstance(free_energy, str) else free_energy
+    d.mobility = L.MOBILITIES[mobility] if isinstance(mobility, str) else mobility
+    d.valence, d.use_dealias = valence, int(bool(use_dealias))
+    d.epsilon, d.B2, d.delta, d.regularisation_delta = epsilon, B2, delta, regularisation_delta
+    d.M, d.rho_min, d.rho_ref = M, rho_min, rho_ref
+    d.dt, d.dx, d.k, d.S = dt, dx, k, S
+    return d
+
+
+def desc_from_config(cfg, dims, precision="fp64"):
+    """ch_desc from a reference TOML table (already parsed to nested dicts)."""
+    def get(path, default=None, required=False):
+        node = cfg
+        for part in path.split("."):
+            if not isinstance(node, dict) or part not in node:
+                if required:
+                    raise KeyError(f"mandatory option '{path}' is missing")
+                return default
+            node = node[part]
+        return node
+
+    u2i = 1.0 / get("distance_scaling_factor", 1.0)
+    model = get("free_energy", required=True)
+    kw = {}
+    if model == "landau":
+        eps = get("landau.epsilon")
+        if eps is None:
+            T = get("landau.T")
+            if T is None:
+                raise KeyError("The Landau free energy requires either 'epsilon' or 'T' to be set")
+            eps = 1 - T
+        if u2i != 1.0:
+            raise ValueError("The Landau free energy does not support distance_scaling_factor values different from 1.0")
+        kw["epsilon"] = eps
+    elif model == "simple_wertheim":
+        delta = get("wertheim.delta", required=True)
+        if isinstance(delta, dict):  # src/utils/Delta.cpp:15-42
+            value = delta.get("value", 0.0)
+            if value == 0.0:
+                T, dH, dS = delta["T"], delta["deltaH"], delta["deltaS"]
+                salt, L_DNA = delta.get("salt", 1.0), delta.get("sticky_size", 6)
+                dS_salt = 0.368 * (L_DNA - 1.0) * math.log(salt)
+                dG = dH - T * (dS + dS_salt)
+                value = 1.6606 * math.exp(-dG / (1.9872036 * T))
+            delta = value
+        kw.update(valence=int(get("wertheim.valence", required=True)), B2=get("wertheim.B2", required=True) * u2i ** 3,
+                  delta=delta * u2i ** 3, regularisation_delta=get("wertheim.regularisation_delta", 0.0) / u2i ** 3)
+    else:
+        raise ValueError(f"Unsupported free energy model '{model}'")
+    k = get("k", required=True)
+    if isinstance(k, list):
+        k = k[0]
+    mob = get("mobility.type", "constant")
+    return make_desc(dims=dims, N=int(get("N", required=True)), integrator=get("integrator", "euler"),
+                     free_energy=model, precision=precision, dt=get("dt", required=True),
+                     dx=get("dx", 1.0) * u2i, k=k * u2i ** 5, mobility=mob, M=get("mobility.M", 1.0) / u2i,
+                     rho_min=get("mobility.rho_min", 0.0) / u2i, rho_ref=get("mobility.rho_ref", 1.0),
+                     S=get("pseudospectral.S", 0.0) * u2i ** 3, use_dealias=get("pseudospectral.use_dealias", False),
+                     **kw)
+
+
+class System:
+    """One Cahn-Hilliard system resident on the GPU."""
+
+    def __init__(self, ctx, desc):
+        self.ctx, self.desc = ctx, desc
+        h = ctypes.c_void_p()
+        _check(ctx.handle, L.lib().ch_sys_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        self.bins = desc.N ** desc.dims
+
+    def _chk(self, st):
+        _check(self.ctx.handle, st)
+
+    def upload(self, rho):
+        a = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        if a.size != self.bins:
+            raise ValueError(f"expected {self.bins} values, got {a.size}")
+        self._chk(L.lib().ch_sys_upload(self.handle, a.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self):
+        out = np.empty(self.bins, dtype=np.float64)
+        self._chk(L.lib().ch_sys_download(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    sync = download
+
+    def download_mobility(self):
+        out = np.empty(self.bins, dtype=np.float64)
+        self._chk(L.lib().ch_sys_download_mobility(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def upload_device(self, ptr):
+        self._chk(L.lib().ch_sys_upload_device(self.handle, ctypes.c_void_p(ptr)))
+
+    def download_device(self, ptr):
+        self._chk(L.lib().ch_sys_download_device(self.handle, ctypes.c_void_p(ptr)))
+
+    def evolve(self, nsteps=1):
+        self._chk(L.lib().ch_sys_evolve(self.handle, int(nsteps)))
+
+    def observables(self):
+        m, f = ctypes.c_double(), ctypes.c_double()
+        self._chk(L.lib().ch_sys_observables(self.handle, ctypes.byref(m), ctypes.byref(f)))
+        return m.value, f.value
+
+    def average_mass(self):
+        return self.observables()[0]
+
+    def average_free_energy(self):
+        return self.observables()[1]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().ch_sys_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def der_bulk_free_energy(ctx, desc, rho_ptr, out_ptr, n):
+    """FreeEnergyModel::der_bulk_free_energy on device buffers (ints: device addresses)."""
+    _check(ctx.handle, L.lib().ch_der_bulk_free_energy(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(rho_ptr),
+                                                       ctypes.c_void_p(out_ptr), int(n)))
+
+
+def set_mobility(ctx, desc, rho_ptr, mob_ptr, n):
+    """IMobility::update_mobility's pointwise rule on device buffers."""
+    _check(ctx.handle, L.lib().ch_set_mobility(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(rho_ptr),
+                                               ctypes.c_void_p(mob_ptr), int(n)))

@@ -1,0 +1,131 @@
+/*
+ * ch_hip.h -- C ABI of the MI355X (gfx950) Cahn-Hilliard time-stepping library.
+ *
+ * The drop-in boundary for the reference's in-process plugin API:
+ *   Integrator<dims>          src/integrators/Integrator.h:10-41
+ *   CUDAIntegrator<dims>      src/CUDA/integrators/CUDAIntegrator.{h,cu}
+ *   FreeEnergyModel (device)  src/models/FreeEnergyModel.h:29,48
+ *   IMobility<dims>           src/mobility/IMobility.h:17-30
+ *   CahnHilliard observables  src/CahnHilliard.cpp:285-319
+ * Plain C types only: pointers, sizes, status codes.  Nothing here calls
+ * exit(); every failure is a ch_status plus a message from ch_ctx_last_error.
+ *
+ * Layout of every field: one species, idx = x + N*y + N*N*z (x fastest), the
+ * reference's MultiField layout (src/utils/MultiField.h:62).  Host arrays are
+ * always double; device buffers are double (CH_FP64) or float (CH_FP32).
+ *
+ * Threading: calls on one ch_ctx are serialised by the caller; work is queued
+ * on the context's stream (default priority).  Distinct contexts are
+ * independent.
+ */
+#ifndef CH_HIP_H
+#define CH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CH_ABI_VERSION 1
+
+typedef enum ch_status {
+    CH_OK = 0,
+    CH_EINVAL = 1,        /* bad argument or descriptor                         */
+    CH_EHIP = 2,          /* HIP runtime error (message in ch_ctx_last_error)   */
+    CH_ENAN = 3,          /* NaN met in a reduction (CahnHilliard.cpp:290-293)  */
+    CH_EUNSUPPORTED = 4,  /* valid request this build does not implement        */
+    CH_ENOMEM = 5,        /* device allocation failed                           */
+    CH_ESTATE = 6         /* call out of order (e.g. adjoint before record)     */
+} ch_status;
+
+enum { CH_FP64 = 0, CH_FP32 = 1 };
+/* `integrator` key: src/CahnHilliard.cpp:169-220 */
+enum { CH_EULER = 0, CH_EULER_MOBILITY = 1, CH_PSEUDOSPECTRAL = 2 };
+/* `free_energy` key: src/main.cpp:44-62 */
+enum { CH_LANDAU = 0, CH_SIMPLE_WERTHEIM = 1 };
+/* `mobility.type` key: src/mobility/IMobility.cpp:18-57; LINEAR (M rho / rho_ref) is new */
+enum { CH_MOB_CONSTANT = 0, CH_MOB_FREE_ENERGY = 1, CH_MOB_REGULARISED = 2, CH_MOB_LINEAR = 3 };
+
+/* One system.  All values in internal units (the caller applies
+ * distance_scaling_factor as src/integrators/Integrator.cpp:11-28 does). */
+typedef struct ch_desc {
+    int32_t dims;          /* 1, 2 or 3                                           */
+    int32_t N;             /* cells per side, a power of two >= 4                 */
+    int32_t precision;     /* CH_FP64 or CH_FP32 (field arithmetic)              */
+    int32_t integrator;    /* CH_EULER, CH_EULER_MOBILITY, CH_PSEUDOSPECTRAL     */
+    int32_t model;         /* CH_LANDAU, CH_SIMPLE_WERTHEIM                      */
+    int32_t mobility;      /* CH_MOB_*                                            */
+    int32_t valence;       /* wertheim.valence                                    */
+    int32_t use_dealias;   /* pseudospectral.use_dealias                          */
+    double epsilon;        /* landau.epsilon (or 1 - landau.T)                    */
+    double B2;             /* wertheim.B2                                         */
+    double delta;          /* wertheim.delta (already resolved by Delta)          */
+    double regularisation_delta; /* wertheim.regularisation_delta                 */
+    double M;              /* mobility.M                                          */
+    double rho_min;        /* mobility.rho_min (regularised)                      */
+    double rho_ref;        /* mobility.rho_ref (linear)                           */
+    double dt, dx, k;      /* dt, dx, k (the step uses mu = f' - 2 k lap)         */
+    double S;              /* pseudospectral.S                                    */
+} ch_desc;
+
+typedef struct ch_ctx ch_ctx;
+typedef struct ch_sys ch_sys;
+typedef struct ch_batch ch_batch;
+
+int ch_abi_version(void);
+const char *ch_status_string(ch_status s);
+
+/* device + stream; replaces the implicit CUDA context and legacy default
+ * stream of src/CUDA/integrators/CUDAIntegrator.cu:13-33 */
+ch_status ch_ctx_create(int device, ch_ctx **out);
+void ch_ctx_destroy(ch_ctx *ctx);
+const char *ch_ctx_last_error(const ch_ctx *ctx);
+ch_status ch_ctx_synchronize(ch_ctx *ctx);
+/* the hipStream_t work is queued on (for interop, e.g. events) */
+void *ch_ctx_stream(ch_ctx *ctx);
+
+/* == CUDAIntegrator ctor + EulerCUDA / EulerMobilityCUDA / PseudospectralCUDA
+ * ctor (device buffers, constants, FFT set-up): CUDAIntegrator.cu:13-33,
+ * EulerCUDA.cu:37-55, PseudospectralCUDA.cu:190-230 */
+ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **out);
+void ch_sys_destroy(ch_sys *sys);
+
+/* host -> device copy of rho (N^dims doubles); == CUDAIntegrator::_CPU_GPU
+ * (CUDAIntegrator.cu:43-53).  For the pseudospectral integrator it also sets
+ * rho_hat = r2c(rho) (PseudospectralCPU.cpp:98-101). */
+ch_status ch_sys_upload(ch_sys *sys, const double *rho_host);
+/* device -> host copy of rho; == Integrator::sync() (CUDAIntegrator.cu:55-70) */
+ch_status ch_sys_download(ch_sys *sys, double *rho_host);
+/* device-to-device variants; `rho_dev` holds N^dims elements of the system's
+ * precision (double for CH_FP64, float for CH_FP32) */
+ch_status ch_sys_upload_device(ch_sys *sys, const void *rho_dev);
+ch_status ch_sys_download_device(ch_sys *sys, void *rho_dev);
+/* the mobility field used by the last step (N^dims doubles); the host copy
+ * CUDAIntegrator::sync() refreshes (CUDAIntegrator.cu:61-66) */
+ch_status ch_sys_download_mobility(ch_sys *sys, double *mob_host);
+
+/* nsteps x { IMobility::update_mobility(); Integrator::evolve(); }
+ * == CahnHilliard::evolve (src/CahnHilliard.cpp:279-282) called nsteps times.
+ * Asynchronous on the context stream. */
+ch_status ch_sys_evolve(ch_sys *sys, int64_t nsteps);
+
+/* FP64 reductions of the current field: average_mass and average_free_energy
+ * (src/CahnHilliard.cpp:285-319, V_bin = dx^3 as :163).  Synchronous.
+ * Returns CH_ENAN when either sum is NaN. */
+ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
+
+/* == FreeEnergyModel::der_bulk_free_energy(field_type*, float*, int) and
+ * FreeEnergyModel::set_mobility(field_type*, double, field_type*, int)
+ * (src/models/FreeEnergyModel.h:29,48; Landau.cu:5-19, SimpleWertheim.cu:10-42):
+ * pointwise kernels on caller-owned device buffers of n elements of
+ * desc->precision.  set_mobility evaluates desc->mobility (not only
+ * free_energy), so every IMobility type has a device form. */
+ch_status ch_der_bulk_free_energy(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev, void *out_dev, int64_t n);
+ch_status ch_set_mobility(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev, void *mob_dev, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CH_HIP_H */

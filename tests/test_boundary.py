@@ -1,0 +1,56 @@
+"""CPU-side checks of the drop-in boundary: the C-ABI library loads and exports
+every symbol include/ch_hip.h declares; descriptors built from reference TOML
+keys carry the reference's unit conventions.  No GPU calls."""
+import ctypes
+import os
+
+import pytest
+
+from conftest import REPO
+
+
+def test_library_exports_every_header_symbol(ch):
+    L = ch._lib.lib()
+    names = ch.header_functions()
+    assert "ch_sys_evolve" in names and len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_every_header_in_include_is_exported(ch):
+    L = ch._lib.lib()
+    inc = os.path.join(REPO, "include")
+    for h in os.listdir(inc):
+        if h.endswith(".h"):
+            for n in ch.header_functions(os.path.join(inc, h)):
+                assert hasattr(L, n), (h, n)
+
+
+def test_abi_version_and_status_strings_without_gpu(ch):
+    L = ch._lib.lib()
+    assert L.ch_abi_version() == 1
+    assert L.ch_status_string(3) == b"NaN encountered"
+
+
+def test_ctx_create_rejects_null_out(ch):
+    L = ch._lib.lib()
+    assert L.ch_ctx_create(0, None) == ch._lib.CH_EINVAL
+
+
+def test_desc_from_reference_config(ch):
+    cfg = {"N": 64, "dt": 0.001, "dx": 1.0, "k": 1.0, "free_energy": "landau", "integrator": "pseudospectral",
+           "landau": {"T": 0.1}, "pseudospectral": {"S": 0.5}}
+    d = ch.desc_from_config(cfg, dims=2)
+    assert (d.dims, d.N, d.integrator, d.model) == (2, 64, ch.PSEUDOSPECTRAL, ch.LANDAU)
+    assert d.epsilon == pytest.approx(0.9) and d.S == 0.5 and d.M == 1.0
+    w = {"N": 32, "dt": 1e-4, "k": [2.0], "free_energy": "simple_wertheim", "distance_scaling_factor": 2.0,
+         "mobility": {"type": "free_energy", "M": 3.0},
+         "wertheim": {"valence": 4, "B2": 8.0, "delta": {"T": 300, "deltaH": -42200, "deltaS": -119.1, "salt": 0.5,
+                                                        "sticky_size": 6}}}
+    d = ch.desc_from_config(w, dims=3)
+    # src/integrators/Integrator.cpp:24-27 (k ~ m^5), SimpleWertheim.cpp:29-31 (B2 ~ m^3),
+    # IMobility FreeEnergyMobility.h:21 (M ~ m^-1)
+    assert d.k == pytest.approx(2.0 / 32) and d.B2 == pytest.approx(1.0) and d.M == pytest.approx(6.0)
+    assert d.mobility == 1 and d.valence == 4 and d.delta > 0
+    with pytest.raises(KeyError):
+        ch.desc_from_config({"N": 8, "dt": 0.1, "free_energy": "landau", "landau": {}}, dims=2)

@@ -1,0 +1,229 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle and the
+reference-held / reference-built fixtures.
+
+Tolerances (stated per result):
+  FP64 fields: <= 1e-10 relative L-inf against the reference-built golden
+               vectors and the oracle (up to 1000 steps); mass drift <= 1e-12.
+  FP32 fields: relative L-inf against the FP64 result <= 2e-4 (1000 Euler
+               steps) / 1e-4 (pseudospectral), FP64 reductions.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import CASES, GOLD, desc_for, golden, oracle_setup, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(ch):
+    c = ch.Context(0)
+    yield c
+    c.close()
+
+
+def run_gpu(ch, ctx, desc, rho0, steps):
+    s = ch.System(ctx, desc)
+    s.upload(rho0)
+    s.evolve(steps)
+    out = s.download()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("c", CASES, ids=[c["name"] for c in CASES])
+def test_reference_golden_vectors_fp64(ch, ctx, c):
+    """Reference EulerCPU / EulerMobilityCPU outputs (oracle/_ref) at 1 and T steps."""
+    g = golden(c["name"])
+    d = desc_for(ch, c)
+    one = run_gpu(ch, ctx, d, g["rho0"], 1)
+    assert rel_linf(one, g["rho1"]) < 1e-13
+    out = run_gpu(ch, ctx, d, g["rho0"], c["steps"])
+    assert rel_linf(out, g["rhoT"]) < 1e-10
+    assert abs(out.sum() - g["rho0"].sum()) < 1e-12 * max(1.0, np.abs(g["rho0"]).sum())
+
+
+@pytest.mark.parametrize("c", CASES, ids=[c["name"] for c in CASES])
+def test_fp32_against_fp64(ch, ctx, c):
+    g = golden(c["name"])
+    out = run_gpu(ch, ctx, desc_for(ch, c, "fp32"), g["rho0"], c["steps"])
+    err = rel_linf(out, g["rhoT"])
+    print(f"{c['name']} fp32 rel L-inf vs fp64 = {err:.3e}")
+    assert err < 2e-4
+
+
+@pytest.mark.parametrize("integrator,name", [("euler", "euler"), ("pseudospectral", "pseudospectral")])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_reference_regression_fixtures(ch, ctx, integrator, name, dims):
+    """The reference's own regression test (tests/test_landau.py:55-65): 100 steps of
+    tests/data/simple_landau.toml, compared at the fixtures' 6 printed digits."""
+    fx = np.load(os.path.join(GOLD, "reference_landau_fixtures.npz"))
+    ref = fx[f"{name}_{dims}D"]
+    rho0 = O.init_density(dims, 64, 0.01, 0.01, 0, 123456)
+    d = ch.make_desc(dims=dims, N=64, integrator=integrator, dt=0.001, dx=1.0, k=1.0, epsilon=0.9)
+    out = run_gpu(ch, ctx, d, rho0, 100)
+    mism = sum(a != b for a, b in zip(("%g" % v for v in out), ("%g" % v for v in ref)))
+    assert mism == 0
+    assert abs(rho0.sum() - out.sum()) < 1e-4
+
+
+@pytest.mark.parametrize("dims,N,steps", [(1, 256, 300), (2, 128, 300), (3, 32, 100)])
+def test_pseudospectral_against_oracle(ch, ctx, dims, N, steps):
+    rho0 = O.init_density(dims, N, 0.0, 0.3, 0, 123456)
+    kw = dict(dims=dims, N=N, dt=0.1, dx=1.0, k=1.0)
+    p = O.Params(**kw)
+    ref = O.run(O.PSEUDOSPECTRAL, p, O.Model(epsilon=0.9), rho0, steps)
+    out = run_gpu(ch, ctx, ch.make_desc(integrator="pseudospectral", epsilon=0.9, **kw), rho0, steps)
+    assert rel_linf(out, ref) < 1e-10
+    assert abs(out.sum() - rho0.sum()) < 1e-12 * N ** dims
+    out32 = run_gpu(ch, ctx, ch.make_desc(integrator="pseudospectral", epsilon=0.9, precision="fp32", **kw), rho0, steps)
+    err = rel_linf(out32, ref)
+    print(f"pseudospectral {dims}D fp32 rel L-inf vs fp64 = {err:.3e}")
+    assert err < 1e-4
+
+
+def test_pseudospectral_options_against_oracle(ch, ctx):
+    """S splitting, dealiasing, dx != 1 and M != 1 (PseudospectralCPU.cpp:23-25,119-133)."""
+    for dims in (1, 2, 3):
+        N = 16 if dims == 3 else 64
+        kw = dict(dims=dims, N=N, dt=0.05, dx=0.7, k=0.8, M=1.3, S=0.4, use_dealias=1)
+        rho0 = O.init_density(dims, N, 0.1, 0.5, 0, 17)
+        ref = O.run(O.PSEUDOSPECTRAL, O.Params(**kw), O.Model(epsilon=0.6), rho0, 50)
+        out = run_gpu(ch, ctx, ch.make_desc(integrator="pseudospectral", epsilon=0.6, **kw), rho0, 50)
+        assert rel_linf(out, ref) < 1e-11, dims
+
+
+def test_euler_c1_config_against_oracle(ch, ctx):
+    """Landau 2D N=128, dt=0.01, eps=0.9, k=1, 1000 steps (the C1 workload, FP64)."""
+    rho0 = O.init_density(2, 128, 0.0, 0.01, 0, 123456)
+    kw = dict(dims=2, N=128, dt=0.01, dx=1.0, k=1.0)
+    ref = O.run(O.EULER, O.Params(**kw), O.Model(epsilon=0.9), rho0, 1000)
+    out = run_gpu(ch, ctx, ch.make_desc(integrator="euler", epsilon=0.9, **kw), rho0, 1000)
+    assert rel_linf(out, ref) < 1e-10
+    assert abs(out.sum() - rho0.sum()) < 1e-12
+
+
+@pytest.mark.parametrize("integrator", ["euler", "euler_mobility", "pseudospectral"])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_observables_against_oracle(ch, ctx, integrator, dims):
+    N = {1: 128, 2: 64, 3: 16}[dims]
+    for model in ("landau", "simple_wertheim"):
+        # the nonlinear term is explicit: Wertheim's f'' ~ 1/rho needs a small dt
+        dt = 0.1 if (integrator == "pseudospectral" and model == "landau") else 0.001
+        if model == "landau":
+            om, dens, A = O.Model(O.LANDAU, epsilon=0.9), 0.0, 0.3
+            mk = dict(epsilon=0.9)
+        else:
+            om, dens, A = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4), 0.05, 0.05
+            mk = dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4)
+        kw = dict(dims=dims, N=N, dt=dt, dx=1.2, k=1.1)
+        rho0 = O.init_density(dims, N, dens, A, 0, 3)
+        s = ch.System(ctx, ch.make_desc(integrator=integrator, **kw, **mk))
+        s.upload(rho0)
+        s.evolve(10)
+        rho = s.download()
+        m, f = s.observables()
+        p = O.Params(**kw)
+        assert m == pytest.approx(O.average_mass(p, rho), rel=1e-12, abs=1e-15)
+        assert f == pytest.approx(O.average_free_energy(p, om, rho), rel=1e-12)
+        s.close()
+
+
+def test_observables_nan_status(ch, ctx):
+    s = ch.System(ctx, ch.make_desc(dims=2, N=16))
+    rho = np.zeros(256)
+    rho[37] = np.nan
+    s.upload(rho)
+    with pytest.raises(ch.CHError) as e:
+        s.observables()
+    assert e.value.status == 3  # CH_ENAN (src/CahnHilliard.cpp:290-293 exits instead)
+    s.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-15), ("fp32", 1e-6)])
+def test_pointwise_model_and_mobility(ch, ctx, precision, tol):
+    import torch
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    rng = np.random.default_rng(0)
+    rho = rng.uniform(0.001, 0.9, 4096)
+    r = torch.tensor(rho, dtype=dt, device="cuda")
+    out = torch.empty_like(r)
+    models = [(O.Model(O.LANDAU, epsilon=0.9), dict(epsilon=0.9)),
+              (O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4, regularisation_delta=0.01),
+               dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4, regularisation_delta=0.01))]
+    for om, mk in models:
+        d = ch.make_desc(dims=1, N=4096, precision=precision, **mk)
+        torch.cuda.synchronize()
+        ch.der_bulk_free_energy(ctx, d, r.data_ptr(), out.data_ptr(), r.numel())
+        ctx.synchronize()
+        ref = O.der_bulk(om, r.double().cpu().numpy())
+        assert np.abs(out.double().cpu().numpy() - ref).max() <= tol * max(1.0, np.abs(ref).max()) * 4
+    om = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4)
+    for mob, kw in (("constant", {}), ("free_energy", {}), ("regularised", dict(rho_min=0.1)),
+                    ("linear", dict(rho_ref=0.05))):
+        d = ch.make_desc(dims=1, N=4096, precision=precision, free_energy="simple_wertheim", B2=1.0, delta=1e4,
+                         valence=4, mobility=mob, M=2.0, **kw)
+        ch.set_mobility(ctx, d, r.data_ptr(), out.data_ptr(), r.numel())
+        ctx.synchronize()
+        p = O.Params(dims=1, N=4096, mobility=O.MOB_CONSTANT if mob == "constant" else
+                     {"free_energy": 1, "regularised": 2, "linear": 3}[mob], M=2.0, **kw)
+        ref = O.mobility(p, om, r.double().cpu().numpy())
+        assert np.abs(out.double().cpu().numpy() - ref).max() <= tol * max(1.0, np.abs(ref).max()) * 4, mob
+
+
+def test_linear_mobility_against_oracle(ch, ctx):
+    """mobility.type = "linear" (M rho / rho_ref, new) with SimpleWertheim in 3D (C5 shape, small N)."""
+    kw = dict(dims=3, N=16, dt=1e-4, dx=1.0, k=1.0, M=1.0, rho_ref=0.05)
+    rho0 = O.init_density(3, 16, 0.05, 0.05, 0, 123456)
+    om = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4)
+    ref = O.run(O.EULER_MOBILITY, O.Params(mobility=O.MOB_LINEAR, **kw), om, rho0, 200)
+    d = ch.make_desc(integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4,
+                     mobility="linear", **kw)
+    out = run_gpu(ch, ctx, d, rho0, 200)
+    assert rel_linf(out - rho0, ref - rho0) < 1e-9
+    assert rel_linf(out, ref) < 1e-12
+
+
+def test_graph_replay_equals_eager(ch, ctx, monkeypatch):
+    """The captured-graph step loop is bit-identical to step-by-step launches."""
+    rho0 = O.init_density(2, 64, 0.0, 0.3, 0, 9)
+    d = ch.make_desc(dims=2, N=64, dt=0.01)
+    s = ch.System(ctx, d)
+    s.upload(rho0)
+    s.evolve(64 * 3 + 5)
+    a = s.download()
+    s.upload(rho0)
+    for _ in range(64 * 3 + 5):
+        s.evolve(1)
+    b = s.download()
+    s.close()
+    assert np.array_equal(a, b)
+
+
+def test_full_size_properties(ch, ctx):
+    """Landau 2D N=1024 FP64 (the headline workload): exact psi -> -psi antisymmetry,
+    exact translation equivariance and mass conservation after 2000 steps."""
+    N = 1024
+    rho0 = O.init_density(2, N, 0.0, 0.01, 0, 123456)
+    d = ch.make_desc(dims=2, N=N, dt=0.01, epsilon=0.9, k=1.0)
+    a = run_gpu(ch, ctx, d, rho0, 2000)
+    b = run_gpu(ch, ctx, d, -rho0, 2000)
+    assert np.array_equal(a, -b)
+    sh = lambda f: np.roll(f.reshape(N, N), (37, 501), axis=(0, 1)).ravel()
+    c = run_gpu(ch, ctx, d, sh(rho0), 2000)
+    assert np.array_equal(c, sh(a))
+    assert abs(a.sum() - rho0.sum()) < 1e-12 * N
+    assert np.abs(a).max() > 0.0
+
+
+def test_invalid_descriptors(ch, ctx):
+    bad = [dict(dims=2, N=48), dict(dims=4, N=16), dict(dims=2, N=2),
+           dict(dims=2, N=16, integrator="euler", mobility="regularised"),
+           dict(dims=2, N=16, integrator="euler_mobility", mobility="free_energy"),
+           dict(dims=2, N=16, dt=-1.0)]
+    for kw in bad:
+        with pytest.raises(ch.CHError):
+            ch.System(ctx, ch.make_desc(**kw))
