@@ -118,7 +118,7 @@ def main():
     s.upload(rho0)
     wall = timed_run(ch, ctx, s, args.steps, args.warmup, args.inner)
     value = cells * args.inner * args.steps / wall / 1e9
-    spl = ch.steps_per_launch(desc) if hasattr(ch, "steps_per_launch") else 1
+    spl = s.steps_per_launch
     t_launch = kernel_time_per_launch(ctx, s, max(1000, 10 * args.inner), spl)
     bytes_per_launch = 16.0 * cells * spl
     achieved = bytes_per_launch / t_launch / 1e9

@@ -16,6 +16,9 @@
 
 #include "ch_internal.h"
 
+// default temporal-blocking depth (time steps per stencil launch)
+static constexpr int kDefaultTB = 4;
+
 struct ch_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -35,6 +38,7 @@ struct ch_sys {
     void *stage2 = nullptr;  // bins elements of T: mobility
     double *partials = nullptr, *result = nullptr;
     int graph_steps = 0;
+    int tb = 1;  // time steps per stencil launch (temporal blocking depth)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
@@ -162,6 +166,8 @@ static chd::Params make_params(const ch_desc *d) {
     p.inv_dx = 1.0 / d->dx;
     p.inv_dx2 = 1.0 / (d->dx * d->dx);
     p.dt = d->dt;
+    p.Mdt = d->M * d->dt;
+    p.wave_rpw = 16;
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
@@ -224,10 +230,19 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     s->elt = desc->precision == CH_FP64 ? 8 : 4;
     const int N = desc->N;
     s->hat = s->bins / N * (N / 2 + 1);
+    // temporal blocking depth: kDefaultTB, or CH_TB_STEPS (environment) when
+    // set, rounded down to a depth the shape has (1 for the pseudospectral path)
+    const int max_k = desc->integrator == CH_PSEUDOSPECTRAL ? 1 : chd::stencil_max_k(desc->dims, desc->N);
+    int want = kDefaultTB;
+    if(const char *tb = std::getenv("CH_TB_STEPS")) want = std::atoi(tb);
+    s->tb = 1;
+    while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
+    if(const char *r = std::getenv("CH_WAVE_RPW")) s->p.wave_rpw = std::atoi(r) == 32 ? 32 : 16;
+    // graph chunk: a multiple of 2 tb launches' worth of steps (parity restored)
     const char *g = std::getenv("CH_GRAPH_STEPS");
     s->graph_steps = g ? std::atoi(g) : 64;
     if(s->graph_steps < 0) s->graph_steps = 0;
-    s->graph_steps &= ~1;
+    s->graph_steps -= s->graph_steps % (2 * s->tb);
 
     auto alloc = [&](void **p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
     bool ok = alloc(&s->psi[0], s->bins * s->elt) && alloc(&s->stage, s->bins * 8) &&
@@ -351,8 +366,8 @@ extern "C" ch_status ch_sys_download_mobility(ch_sys *s, double *mob_host) {
     return CH_OK;
 }
 
-// one time step queued on the stream
-static hipError_t one_step(ch_sys *s, int &cur) {
+// K time steps (one launch) queued on the stream
+static hipError_t launch_steps(ch_sys *s, int &cur, int K) {
     hipStream_t st = s->ctx->stream;
     if(s->d.integrator == CH_PSEUDOSPECTRAL) {
         chd::SpectralArgs a = spectral_args(s);
@@ -360,11 +375,12 @@ static hipError_t one_step(ch_sys *s, int &cur) {
     }
     const void *in = s->psi[cur];
     void *out = s->psi[cur ^ 1];
+    const ch_desc &d = s->d;
     hipError_t e;
-    switch(s->d.dims) {
-    case 1: e = chd::stencil_step<1>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
-    case 2: e = chd::stencil_step<2>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
-    default: e = chd::stencil_step<3>(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, in, out, st); break;
+    switch(d.dims) {
+    case 1: e = chd::stencil_step<1>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
+    case 2: e = chd::stencil_step<2>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
+    default: e = chd::stencil_step<3>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     }
     cur ^= 1;
     return e;
@@ -382,7 +398,7 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int cur = s->cur;
             hipError_t e = hipSuccess;
-            for(int i = 0; i < G && e == hipSuccess; i++) e = one_step(s, cur);
+            for(int i = 0; i < G && e == hipSuccess; i += s->tb) e = launch_steps(s, cur, s->tb);
             hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
             if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
             if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
@@ -392,11 +408,14 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
         }
         const int64_t reps = nsteps / G;
         for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[par], c->stream));
-        nsteps -= reps * G;  // G is even: the ping-pong parity is unchanged
+        nsteps -= reps * G;  // G / tb launches, an even number: parity unchanged
     }
-    for(int64_t t = 0; t < nsteps; t++) HIPCHK(c, one_step(s, s->cur));
+    for(; nsteps >= s->tb; nsteps -= s->tb) HIPCHK(c, launch_steps(s, s->cur, s->tb));
+    for(; nsteps > 0; nsteps--) HIPCHK(c, launch_steps(s, s->cur, 1));
     return CH_OK;
 }
+
+extern "C" int32_t ch_sys_steps_per_launch(const ch_sys *s) { return s ? s->tb : 0; }
 
 extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_energy) {
     if(!s) return CH_EINVAL;

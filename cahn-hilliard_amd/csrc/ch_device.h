@@ -17,13 +17,22 @@ struct Params {
     int N, mask, log2N, dims;
     int valence;
     int use_dealias;
+    int wave_rpw;          // rows per wave of the 2D wave kernel (16 or 32)
     double eps;            // Landau epsilon
     double B2, reg, log_reg, tvd;   // SimpleWertheim: tvd = 2 valence delta
     double M, rho_min, rho_ref;     // mobility
     double k, two_k, dx, inv_dx, inv_dx2, dt, S;
+    double Mdt;                     // M dt (constant-mobility Euler update)
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
 };
+
+// Every per-cell expression below is written with contraction off and
+// explicit fused multiply-adds, so that all kernels which share these
+// functions (any tile shape, any number of steps per launch) produce
+// bit-identical fields.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // ---------------------------------------------------------------- models
 // Landau: f' = -eps psi + psi^3 (src/models/Landau.cpp:46-52), evaluated in T.
@@ -36,8 +45,9 @@ __device__ __forceinline__ double wertheim_X(const Params &p, double r) {
 
 template <typename T, int MODEL>
 __device__ __forceinline__ T fprime(const Params &p, T r) {
+#pragma clang fp contract(off)
     if constexpr(MODEL == CH_LANDAU) {
-        return -(T)p.eps * r + r * r * r;
+        return fma_t(r * r, r, -(T)p.eps * r);
     }
     else {
         double rd = (double)r;
@@ -96,6 +106,53 @@ __device__ __forceinline__ T mobility(const Params &p, T r) {
     else {
         return (T)p.M * r / (T)p.rho_ref;
     }
+}
+
+// ---------------------------------------------------------------- cells
+// Periodic (2d+1)-point Laplacian with the neighbours summed in the
+// reference's order -x, +x, -y, +y, -z, +z (EulerCPU.cpp:71-84);
+// m[d] / q[d] are the -e_d / +e_d neighbours.
+template <typename T, int DIMS>
+__device__ __forceinline__ T laplacian(T c, const T (&m)[DIMS], const T (&q)[DIMS], T inv_dx2) {
+#pragma clang fp contract(off)
+    T s = m[0] + q[0];
+#pragma unroll
+    for(int d = 1; d < DIMS; d++) {
+        s = s + m[d];
+        s = s + q[d];
+    }
+    return fma_t((T)(-2 * DIMS), c, s) * inv_dx2;
+}
+
+// mu = f'(psi) - 2 k lap(psi)  (EulerCPU.cpp:20-26)
+template <typename T, int DIMS, int MODEL>
+__device__ __forceinline__ T chem_pot(const Params &p, T c, const T (&m)[DIMS], const T (&q)[DIMS]) {
+#pragma clang fp contract(off)
+    return fma_t(-(T)p.two_k, laplacian<T, DIMS>(c, m, q, (T)p.inv_dx2), fprime<T, MODEL>(p, c));
+}
+
+// psi + dt M lap(mu)  (EulerCPU.cpp:29-35), constant mobility
+template <typename T, int DIMS>
+__device__ __forceinline__ T euler_update(const Params &p, T psi, T mu, const T (&m)[DIMS], const T (&q)[DIMS]) {
+#pragma clang fp contract(off)
+    return fma_t((T)p.Mdt, laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2), psi);
+}
+
+// psi + dt div J with face fluxes J_d(i) = (M_i + M_{i+e_d})/2 (mu_{i+e_d} - mu_i)/dx
+// (EulerMobilityCPU.cpp:44-82, divergence EulerCPU.cpp:114-128)
+template <typename T, int DIMS>
+__device__ __forceinline__ T mobility_update(const Params &p, T psi, T mu, const T (&mum)[DIMS], const T (&muq)[DIMS],
+                                             T M, const T (&Mm)[DIMS], const T (&Mq)[DIMS]) {
+#pragma clang fp contract(off)
+    const T half = (T)0.5, inv_dx = (T)p.inv_dx;
+    T res = 0;
+#pragma unroll
+    for(int d = 0; d < DIMS; d++) {
+        const T Jq = half * (M + Mq[d]) * (muq[d] - mu) * inv_dx;
+        const T Jm = half * (Mm[d] + M) * (mu - mum[d]) * inv_dx;
+        res = res + (Jq - Jm) * inv_dx;
+    }
+    return fma_t(res, (T)p.dt, psi);
 }
 
 // XCD-aware block order: gfx950 hands workgroups to its 8 XCDs round-robin by

@@ -5,10 +5,12 @@
 
 namespace chd {
 
-// one fused Euler / EulerMobility step, in -> out (distinct buffers)
+// K fused Euler / EulerMobility steps in one launch, in -> out (distinct
+// buffers).  K = 1 always exists; K in {2, 4, 8} for 2D grids with N >= 64.
 template <int DIMS>
-hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, const void *in, void *out,
-                        hipStream_t s);
+hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, int K, const void *in,
+                        void *out, hipStream_t s);
+inline int stencil_max_k(int dims, int N) { return (dims == 2 && N >= 64) ? 8 : 1; }
 
 // pseudospectral pipeline
 struct SpectralArgs {

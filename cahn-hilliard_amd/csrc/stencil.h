@@ -28,13 +28,6 @@ struct TileGeom {
     static constexpr int R = RXY * RZ;
 };
 
-template <typename T, int DIMS, int RX, int RXY>
-__device__ __forceinline__ T nsum(const T *__restrict__ a, int li) {
-    T s = a[li - 1] + a[li + 1];
-    if constexpr(DIMS >= 2) s = s + a[li - RX] + a[li + RX];
-    if constexpr(DIMS == 3) s = s + a[li - RXY] + a[li + RXY];
-    return s;
-}
 
 template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K, int NT>
 __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__restrict__ out, Params p,
@@ -61,8 +54,8 @@ __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__r
     }
     __syncthreads();
 
-    const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, dt = (T)p.dt, Mc = (T)p.M;
-    const T inv_dx = (T)p.inv_dx, half = (T)0.5;
+    constexpr int off[3] = {1, RX, RXY};
+    const T Mc = (T)p.M;
 
 #pragma unroll
     for(int s = 0; s < K; s++) {
@@ -74,9 +67,11 @@ __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__r
             for(int c = threadIdx.x; c < EX * EY * EZ; c += NT) {
                 const int ex = c % EX, ey = (c / EX) % EY, ez = c / (EX * EY);
                 const int li = (lo + ex) + (DIMS >= 2 ? (lo + ey) * RX : 0) + (DIMS == 3 ? (lo + ez) * RXY : 0);
+                T m[DIMS], q[DIMS];
+#pragma unroll
+                for(int d = 0; d < DIMS; d++) { m[d] = A[li - off[d]]; q[d] = A[li + off[d]]; }
                 const T a = A[li];
-                const T lap = (nsum<T, DIMS, RX, RXY>(A, li) - (T)(2 * DIMS) * a) * inv_dx2;
-                B[li] = fprime<T, MODEL>(p, a) - two_k * lap;
+                B[li] = chem_pot<T, DIMS, MODEL>(p, a, m, q);
                 if constexpr(EM && MOB != CH_MOB_CONSTANT) C[li] = mobility<T, MOB>(p, a);
             }
         }
@@ -89,25 +84,26 @@ __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__r
             for(int c = threadIdx.x; c < EX * EY * EZ; c += NT) {
                 const int ex = c % EX, ey = (c / EX) % EY, ez = c / (EX * EY);
                 const int li = (lo + ex) + (DIMS >= 2 ? (lo + ey) * RX : 0) + (DIMS == 3 ? (lo + ez) * RXY : 0);
+                T m[DIMS], q[DIMS];
+#pragma unroll
+                for(int d = 0; d < DIMS; d++) { m[d] = B[li - off[d]]; q[d] = B[li + off[d]]; }
                 const T mu = B[li];
                 if constexpr(!EM) {
-                    const T lapmu = (nsum<T, DIMS, RX, RXY>(B, li) - (T)(2 * DIMS) * mu) * inv_dx2;
-                    A[li] = A[li] + (Mc * lapmu) * dt;
+                    A[li] = euler_update<T, DIMS>(p, A[li], mu, m, q);
                 }
                 else {
-                    constexpr int off[3] = {1, RX, RXY};
-                    T res = 0;
+                    T Mm[DIMS], Mq[DIMS], Mi;
+                    if constexpr(MOB == CH_MOB_CONSTANT) {
+                        Mi = Mc;
 #pragma unroll
-                    for(int d = 0; d < DIMS; d++) {
-                        const int o = off[d];
-                        T Mi, Mp, Mm;
-                        if constexpr(MOB == CH_MOB_CONSTANT) { Mi = Mp = Mm = Mc; }
-                        else { Mi = C[li]; Mp = C[li + o]; Mm = C[li - o]; }
-                        const T Jp = half * (Mi + Mp) * (B[li + o] - mu) * inv_dx;
-                        const T Jm = half * (Mm + Mi) * (mu - B[li - o]) * inv_dx;
-                        res += (Jp - Jm) * inv_dx;
+                        for(int d = 0; d < DIMS; d++) { Mm[d] = Mc; Mq[d] = Mc; }
                     }
-                    A[li] = A[li] + res * dt;
+                    else {
+                        Mi = C[li];
+#pragma unroll
+                        for(int d = 0; d < DIMS; d++) { Mm[d] = C[li - off[d]]; Mq[d] = C[li + off[d]]; }
+                    }
+                    A[li] = mobility_update<T, DIMS>(p, A[li], mu, m, q, Mi, Mm, Mq);
                 }
             }
         }

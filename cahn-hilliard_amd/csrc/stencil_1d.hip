@@ -3,8 +3,8 @@
 #include "stencil_dispatch.h"
 
 namespace chd {
-template <> hipError_t stencil_step<1>(const Params &p, int precision, int integrator, int model, int mob,
+template <> hipError_t stencil_step<1>(const Params &p, int precision, int integrator, int model, int mob, int K,
                                          const void *in, void *out, hipStream_t s) {
-    return stencil_step_dims<1>(p, precision, integrator, model, mob, in, out, s);
+    return stencil_step_dims<1>(p, precision, integrator, model, mob, K, in, out, s);
 }
 }  // namespace chd

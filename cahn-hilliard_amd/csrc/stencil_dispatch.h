@@ -2,6 +2,7 @@
 #pragma once
 
 #include "stencil.h"
+#include "stencil_wave.h"
 
 namespace chd {
 
@@ -27,30 +28,55 @@ hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t
     }
 }
 
+// 2D grids with N >= 64 use the register/DPP wave kernel (4 waves per block,
+// p.wave_rpw rows per wave); K in {1, 2, 4, 8} steps per launch.
+template <typename T, int MODEL, int MOB, bool EM, int RPW>
+hipError_t wave_k(const Params &p, int K, const void *in, void *out, hipStream_t s) {
+    switch(K) {
+    case 1: return launch_wave_cfg<T, MODEL, MOB, EM, 1, RPW, 4>(p, in, out, s);
+    case 2: return launch_wave_cfg<T, MODEL, MOB, EM, 2, RPW, 4>(p, in, out, s);
+    case 4: return launch_wave_cfg<T, MODEL, MOB, EM, 4, RPW, 4>(p, in, out, s);
+    case 8: return launch_wave_cfg<T, MODEL, MOB, EM, 8, RPW, 4>(p, in, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int DIMS, int MODEL, int MOB, bool EM>
+hipError_t stencil_k(const Params &p, int K, const void *in, void *out, hipStream_t s) {
+    if constexpr(DIMS == 2) {
+        if(p.N >= 64) {
+            if(p.wave_rpw == 32) return wave_k<T, MODEL, MOB, EM, 32>(p, K, in, out, s);
+            return wave_k<T, MODEL, MOB, EM, 16>(p, K, in, out, s);
+        }
+    }
+    if(K != 1) return hipErrorInvalidValue;
+    return stencil_tiles<T, DIMS, MODEL, MOB, EM, 1>(p, in, out, s);
+}
+
 template <typename T, int DIMS, int MODEL>
-hipError_t stencil_model(const Params &p, int integrator, int mob, const void *in, void *out, hipStream_t s) {
-    if(integrator == CH_EULER) return stencil_tiles<T, DIMS, MODEL, CH_MOB_CONSTANT, false, 1>(p, in, out, s);
+hipError_t stencil_model(const Params &p, int integrator, int mob, int K, const void *in, void *out, hipStream_t s) {
+    if(integrator == CH_EULER) return stencil_k<T, DIMS, MODEL, CH_MOB_CONSTANT, false>(p, K, in, out, s);
     switch(mob) {
-    case CH_MOB_CONSTANT: return stencil_tiles<T, DIMS, MODEL, CH_MOB_CONSTANT, true, 1>(p, in, out, s);
-    case CH_MOB_REGULARISED: return stencil_tiles<T, DIMS, MODEL, CH_MOB_REGULARISED, true, 1>(p, in, out, s);
-    case CH_MOB_LINEAR: return stencil_tiles<T, DIMS, MODEL, CH_MOB_LINEAR, true, 1>(p, in, out, s);
+    case CH_MOB_CONSTANT: return stencil_k<T, DIMS, MODEL, CH_MOB_CONSTANT, true>(p, K, in, out, s);
+    case CH_MOB_REGULARISED: return stencil_k<T, DIMS, MODEL, CH_MOB_REGULARISED, true>(p, K, in, out, s);
+    case CH_MOB_LINEAR: return stencil_k<T, DIMS, MODEL, CH_MOB_LINEAR, true>(p, K, in, out, s);
     case CH_MOB_FREE_ENERGY:
         if constexpr(MODEL == CH_SIMPLE_WERTHEIM)
-            return stencil_tiles<T, DIMS, MODEL, CH_MOB_FREE_ENERGY, true, 1>(p, in, out, s);
+            return stencil_k<T, DIMS, MODEL, CH_MOB_FREE_ENERGY, true>(p, K, in, out, s);
         return hipErrorInvalidValue;
     }
     return hipErrorInvalidValue;
 }
 
 template <int DIMS>
-hipError_t stencil_step_dims(const Params &p, int precision, int integrator, int model, int mob, const void *in,
-                             void *out, hipStream_t s) {
+hipError_t stencil_step_dims(const Params &p, int precision, int integrator, int model, int mob, int K,
+                             const void *in, void *out, hipStream_t s) {
     if(precision == CH_FP64) {
-        if(model == CH_LANDAU) return stencil_model<double, DIMS, CH_LANDAU>(p, integrator, mob, in, out, s);
-        return stencil_model<double, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, out, s);
+        if(model == CH_LANDAU) return stencil_model<double, DIMS, CH_LANDAU>(p, integrator, mob, K, in, out, s);
+        return stencil_model<double, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, K, in, out, s);
     }
-    if(model == CH_LANDAU) return stencil_model<float, DIMS, CH_LANDAU>(p, integrator, mob, in, out, s);
-    return stencil_model<float, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, out, s);
+    if(model == CH_LANDAU) return stencil_model<float, DIMS, CH_LANDAU>(p, integrator, mob, K, in, out, s);
+    return stencil_model<float, DIMS, CH_SIMPLE_WERTHEIM>(p, integrator, mob, K, in, out, s);
 }
 
 }  // namespace chd

@@ -157,6 +157,10 @@ class System:
     def evolve(self, nsteps=1):
         self._chk(L.lib().ch_sys_evolve(self.handle, int(nsteps)))
 
+    @property
+    def steps_per_launch(self):
+        return int(L.lib().ch_sys_steps_per_launch(self.handle))
+
     def observables(self):
         m, f = ctypes.c_double(), ctypes.c_double()
         self._chk(L.lib().ch_sys_observables(self.handle, ctypes.byref(m), ctypes.byref(f)))

@@ -110,6 +110,12 @@ ch_status ch_sys_download_mobility(ch_sys *sys, double *mob_host);
  * Asynchronous on the context stream. */
 ch_status ch_sys_evolve(ch_sys *sys, int64_t nsteps);
 
+/* Time steps one stencil launch advances (temporal blocking depth; 1 for
+ * the pseudospectral integrator).  No reference counterpart: introspection for
+ * measurement (the reference always does one step per launch,
+ * EulerCUDA.cu:60-69).  Override with the CH_TB_STEPS environment variable. */
+int32_t ch_sys_steps_per_launch(const ch_sys *sys);
+
 /* FP64 reductions of the current field: average_mass and average_free_energy
  * (src/CahnHilliard.cpp:285-319, V_bin = dx^3 as :163).  Synchronous.
  * Returns CH_ENAN when either sum is NaN. */

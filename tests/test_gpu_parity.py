@@ -203,6 +203,32 @@ def test_graph_replay_equals_eager(ch, ctx, monkeypatch):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("integrator,mob", [("euler", "constant"), ("euler_mobility", "regularised")])
+def test_temporal_blocking_variants_bitwise(ch, ctx, monkeypatch, integrator, mob):
+    """Every steps-per-launch depth and rows-per-wave split of the 2D wave kernel
+    gives the same bits as one step per launch, and matches the oracle."""
+    N, steps = 128, 37
+    rho0 = O.init_density(2, N, 0.5, 0.1, 0, 21)
+    kw = dict(dims=2, N=N, dt=0.01, dx=1.0, k=1.0, M=1.0, rho_min=0.1)
+    outs = {}
+    for rpw in ("16", "32"):
+        for tb in ("1", "2", "4", "8"):
+            monkeypatch.setenv("CH_TB_STEPS", tb)
+            monkeypatch.setenv("CH_WAVE_RPW", rpw)
+            s = ch.System(ctx, ch.make_desc(integrator=integrator, mobility=mob, epsilon=0.9, **kw))
+            assert s.steps_per_launch == int(tb)
+            s.upload(rho0)
+            s.evolve(steps)
+            outs[(rpw, tb)] = s.download()
+            s.close()
+    base = outs[("16", "1")]
+    for key, v in outs.items():
+        assert np.array_equal(v, base), key
+    p = O.Params(mobility=O.MOB_CONSTANT if mob == "constant" else O.MOB_REGULARISED, **kw)
+    ref = O.run(O.EULER if integrator == "euler" else O.EULER_MOBILITY, p, O.Model(epsilon=0.9), rho0, steps)
+    assert rel_linf(base, ref) < 1e-12
+
+
 def test_full_size_properties(ch, ctx):
     """Landau 2D N=1024 FP64 (the headline workload): exact psi -> -psi antisymmetry,
     exact translation equivariance and mass conservation after 2000 steps."""
