@@ -10,10 +10,14 @@
 //   last     W -> FFT -> spectral update of rho_hat -> /N^d -> IFFT  -> W   (fused)
 //   y pass   W -> IFFT_y                                            -> W   (3D only)
 //   x pass   W -> c2r (merge + length-N/2 IFFT)                     -> rho
-// 1D uses x pass, an element-wise update kernel and the inverse x pass.
+// 1D uses the x pass, an element-wise update kernel and the inverse x pass.
+//
 // Every line FFT is a Stockham radix-4 (plus one radix-2 pass when log2 n is
-// odd) in LDS; twiddles come from an FP64 table W_N^m = e^{-2 pi i m / N}
-// computed on the host from exact integer indices.
+// odd) in LDS with the length a template parameter: each thread owns one
+// radix-4 butterfly per pass (4 elements), all of its twiddles are loaded
+// into registers before the first pass, and lines of n <= 256 are packed
+// several per block.  Twiddles come from an FP64 table W_N^m = e^{-2 pi i m/N}
+// built on the host from exact integer indices (rounded once for FP32).
 #include "ch_internal.h"
 
 namespace chd {
@@ -29,70 +33,90 @@ __device__ __forceinline__ T2 csub(T2 a, T2 b) { return T2{a.x - b.x, a.y - b.y}
 template <typename T2>
 __device__ __forceinline__ T2 conj_(T2 a) { return T2{a.x, -a.y}; }
 
-// In-place FFT of `E / n` lines of length n stored line-major in LDS.
-// NT threads, EPT = E / NT elements per thread.  INV: e^{+i}, unnormalised.
-template <typename T2, bool INV, int EPT, int NT>
-__device__ void lds_fft(T2 *buf, int n, int log2n, const T2 *__restrict__ tw, int twstride) {
-    const int tid = threadIdx.x;
-    int Ns = 1;
-    if(log2n & 1) {
-        T2 v0[EPT / 2], v1[EPT / 2];
-        const int h = n >> 1;
+// Block geometry of a length-2^LOG2N line FFT: E elements, NT = E / 4 threads.
+template <int LOG2N, int NLINES>
+struct FftGeom {
+    static constexpr int n = 1 << LOG2N;
+    static constexpr int NL = NLINES;
+    static constexpr int E = n * NL;
+    static constexpr int NT = E / 4;
+    static constexpr bool R2 = (LOG2N & 1) != 0;
+    static constexpr int NP4 = LOG2N / 2;  // radix-4 passes
+};
+
+// lines of n <= 256 are packed four-per-1024 elements (NT = 256)
+template <int LOG2N>
+struct PackedLines { static constexpr int value = LOG2N >= 8 ? 1 : (1 << (10 - LOG2N)); };
+
+template <typename T2, int LOG2N, int NLINES>
+struct LineFFT {
+    using G = FftGeom<LOG2N, NLINES>;
+    T2 tw[G::NP4 > 0 ? G::NP4 : 1][3];
+
+    // table: W_Ntab^m; twstride = Ntab / n
+    __device__ __forceinline__ void prefetch(const T2 *__restrict__ table, int twstride) {
+        constexpr int n = G::n, qn = n >> 2;
+        const int j = (int)threadIdx.x % (qn > 0 ? qn : 1);
 #pragma unroll
-        for(int i = 0; i < EPT / 2; i++) {
-            const int q = tid + i * NT;
-            const int line = q / h, j = q - line * h;
-            v0[i] = buf[line * n + j];
-            v1[i] = buf[line * n + j + h];
-        }
-        __syncthreads();
+        for(int ps = 0; ps < G::NP4; ps++) {
+            const int Ns = (G::R2 ? 2 : 1) << (2 * ps);
+            const int tscale = (n / (4 * Ns)) * twstride;
+            const int jm = j & (Ns - 1);
 #pragma unroll
-        for(int i = 0; i < EPT / 2; i++) {
-            const int q = tid + i * NT;
-            const int line = q / h, j = q - line * h;
-            buf[line * n + 2 * j] = cadd(v0[i], v1[i]);
-            buf[line * n + 2 * j + 1] = csub(v0[i], v1[i]);
+            for(int r = 1; r < 4; r++) tw[ps][r - 1] = table[r * jm * tscale];
         }
-        __syncthreads();
-        Ns = 2;
     }
-    const int qn = n >> 2;
-    for(; Ns < n; Ns <<= 2) {
-        T2 v[EPT / 4][4];
-        const int tscale = (n / (4 * Ns)) * twstride;
+
+    // in-place transform of NL lines stored line-major in buf; INV: e^{+i}
+    template <bool INV>
+    __device__ __forceinline__ void run(T2 *buf) const {
+        constexpr int n = G::n, NT = G::NT;
+        const int tid = threadIdx.x;
+        if constexpr(G::R2) {
+            constexpr int h = n >> 1;
+            T2 v0[2], v1[2];
 #pragma unroll
-        for(int i = 0; i < EPT / 4; i++) {
-            const int q = tid + i * NT;
-            const int line = q / qn, j = q - line * qn;
-            const int jm = j & (Ns - 1);
-#pragma unroll
-            for(int r = 0; r < 4; r++) v[i][r] = buf[line * n + j + r * qn];
-#pragma unroll
-            for(int r = 1; r < 4; r++) {
-                T2 w = tw[r * jm * tscale];
-                if(INV) w.y = -w.y;
-                v[i][r] = cmul(v[i][r], w);
+            for(int i = 0; i < 2; i++) {
+                const int q = tid + i * NT;
+                const int line = q / h, j = q % h;
+                v0[i] = buf[line * n + j];
+                v1[i] = buf[line * n + j + h];
             }
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll
-        for(int i = 0; i < EPT / 4; i++) {
-            const int q = tid + i * NT;
-            const int line = q / qn, j = q - line * qn;
+            for(int i = 0; i < 2; i++) {
+                const int q = tid + i * NT;
+                const int line = q / h, j = q % h;
+                buf[line * n + 2 * j] = cadd(v0[i], v1[i]);
+                buf[line * n + 2 * j + 1] = csub(v0[i], v1[i]);
+            }
+            __syncthreads();
+        }
+        constexpr int qn = n >> 2;
+        const int line = tid / (qn > 0 ? qn : 1), j = tid % (qn > 0 ? qn : 1);
+#pragma unroll
+        for(int ps = 0; ps < G::NP4; ps++) {
+            const int Ns = (G::R2 ? 2 : 1) << (2 * ps);
             const int jm = j & (Ns - 1);
-            const T2 a0 = cadd(v[i][0], v[i][2]), a1 = csub(v[i][0], v[i][2]);
-            const T2 a2 = cadd(v[i][1], v[i][3]);
-            const T2 d = csub(v[i][1], v[i][3]);
-            const T2 a3 = INV ? T2{-d.y, d.x} : T2{d.y, -d.x};  // (+-i) (v1 - v3)
+            T2 v[4];
+#pragma unroll
+            for(int r = 0; r < 4; r++) v[r] = buf[line * n + j + r * qn];
+#pragma unroll
+            for(int r = 1; r < 4; r++) v[r] = cmul(v[r], INV ? conj_(tw[ps][r - 1]) : tw[ps][r - 1]);
+            __syncthreads();
+            const T2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+            const T2 a2 = cadd(v[1], v[3]);
+            const T2 d = csub(v[1], v[3]);
+            const T2 a3 = INV ? T2{-d.y, d.x} : T2{d.y, -d.x};  // (+-i)(v1 - v3)
             const int o = line * n + (j - jm) * 4 + jm;
             buf[o] = cadd(a0, a2);
             buf[o + Ns] = cadd(a1, a3);
             buf[o + 2 * Ns] = csub(a0, a2);
             buf[o + 3 * Ns] = csub(a1, a3);
+            __syncthreads();
         }
-        __syncthreads();
     }
-}
+};
 
 // ---- spectral update (PseudospectralCPU.cpp:119-133), mode -> q^2 exactly as
 // the reference builds its tables (PseudospectralCPU.cpp:34-83)
@@ -140,31 +164,37 @@ __device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, i
     return T2{nw.x * inv_bins, nw.y * inv_bins};
 }
 
-// ---- x pass, forward: rows of N reals -> N/2+1 complex (r2c)
-template <typename T, int MODEL, bool FPRIME, int EPT, int NT>
-__global__ __launch_bounds__(NT) void k_xfwd(const T *__restrict__ in, typename Complex<T>::type *__restrict__ out,
-                                            const typename Complex<T>::type *__restrict__ tw, Params p, int rows) {
+// ---- x pass, forward: rows of N = 2n reals -> n + 1 complex (r2c).
+// LOG2N is log2 of the complex length n = N / 2.
+template <typename T, int MODEL, bool FPRIME, int LOG2N>
+__global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xfwd(
+    const T *__restrict__ in, typename Complex<T>::type *__restrict__ out, const typename Complex<T>::type *__restrict__ tw,
+    Params p, int rows) {
     using T2 = typename Complex<T>::type;
-    constexpr int E = EPT * NT;
-    __shared__ T2 buf[E];
-    const int N = p.N, n = N >> 1, nl = E / n;
+    using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
+    constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT;
+    __shared__ T2 buf[G::E];
+    LineFFT<T2, LOG2N, nl> fft;
+    fft.prefetch(tw, 2);
     const int row0 = blockIdx.x * nl;
     T *bufT = reinterpret_cast<T *>(buf);
-    for(int e = threadIdx.x; e < nl * N; e += NT) {
-        const int line = e / N, x = e - line * N;
+#pragma unroll
+    for(int i = 0; i < 8; i++) {
+        const int e = threadIdx.x + i * NT;  // nl * N = 8 NT reals
+        const int line = e / N, x = e % N;
         const int row = row0 + line;
         T v = 0;
         if(row < rows) {
             v = in[(size_t)row * N + x];
             if constexpr(FPRIME) v = fprime<T, MODEL>(p, v);
         }
-        bufT[line * N + x] = v;
+        bufT[e] = v;
     }
     __syncthreads();
-    lds_fft<T2, false, EPT, NT>(buf, n, p.log2N - 1, tw, 2);
-    const int Nh = n + 1;
+    fft.template run<false>(buf);
+    constexpr int Nh = n + 1;
     for(int e = threadIdx.x; e < nl * Nh; e += NT) {
-        const int line = e / Nh, k = e - line * Nh;
+        const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
         if(row >= rows) continue;
         const T2 zk = buf[line * n + (k == n ? 0 : k)];
@@ -173,32 +203,34 @@ __global__ __launch_bounds__(NT) void k_xfwd(const T *__restrict__ in, typename 
         const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
         const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
         const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
-        const T2 w = tw[k];
-        out[(size_t)row * Nh + k] = cadd(Ev, cmul(w, Ov));
+        out[(size_t)row * Nh + k] = cadd(Ev, cmul(tw[k], Ov));
     }
 }
 
-// ---- x pass, inverse: N/2+1 complex -> N reals (c2r, unnormalised).  The
+// ---- x pass, inverse: n + 1 complex -> N = 2n reals (c2r, unnormalised).  The
 // imaginary parts of X_0 and X_{N/2} are dropped (Hermitian completion).
-template <typename T, int EPT, int NT>
-__global__ __launch_bounds__(NT) void k_xinv(const typename Complex<T>::type *__restrict__ in, T *__restrict__ out,
-                                            const typename Complex<T>::type *__restrict__ tw, Params p, int rows) {
+template <typename T, int LOG2N>
+__global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xinv(
+    const typename Complex<T>::type *__restrict__ in, T *__restrict__ out, const typename Complex<T>::type *__restrict__ tw,
+    Params p, int rows) {
     using T2 = typename Complex<T>::type;
-    constexpr int E = EPT * NT;
-    __shared__ T2 buf[E + E / 2];  // nl (n + 1) <= E + E/2 since n >= 2
-    const int N = p.N, n = N >> 1, nl = E / n, Nh = n + 1;
+    using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
+    constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
+    __shared__ T2 buf[G::E + nl];
+    LineFFT<T2, LOG2N, nl> fft;
+    fft.prefetch(tw, 2);
     const int row0 = blockIdx.x * nl;
     for(int e = threadIdx.x; e < nl * Nh; e += NT) {
-        const int line = e / Nh, k = e - line * Nh;
+        const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
         buf[e] = (row < rows) ? in[(size_t)row * Nh + k] : T2{0, 0};
     }
     __syncthreads();
-    T2 z[EPT];
+    T2 z[4];
 #pragma unroll
-    for(int i = 0; i < EPT; i++) {
+    for(int i = 0; i < 4; i++) {
         const int q = threadIdx.x + i * NT;
-        const int line = q / n, k = q - line * n;
+        const int line = q / n, k = q % n;
         T2 xk = buf[line * Nh + k];
         T2 xnk = buf[line * Nh + n - k];
         if(k == 0) { xk.y = 0; xnk.y = 0; }
@@ -209,66 +241,69 @@ __global__ __launch_bounds__(NT) void k_xinv(const typename Complex<T>::type *__
     }
     __syncthreads();
 #pragma unroll
-    for(int i = 0; i < EPT; i++) buf[threadIdx.x + i * NT] = z[i];
+    for(int i = 0; i < 4; i++) buf[threadIdx.x + i * NT] = z[i];
     __syncthreads();
-    lds_fft<T2, true, EPT, NT>(buf, n, p.log2N - 1, tw, 2);
+    fft.template run<true>(buf);
     const T *bufT = reinterpret_cast<const T *>(buf);
-    for(int e = threadIdx.x; e < nl * N; e += NT) {
-        const int line = e / N, x = e - line * N;
+#pragma unroll
+    for(int i = 0; i < 8; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int line = e / N, x = e % N;
         const int row = row0 + line;
-        if(row < rows) out[(size_t)row * N + x] = bufT[line * N + x];
+        if(row < rows) out[(size_t)row * N + x] = bufT[e];
     }
 }
 
-// ---- strided axis pass over the half spectrum.  Line l starts at
-// (l / Nh) * P + (l % Nh) and steps by `stride`.  MODE 0: forward, 1: inverse,
-// 2: forward + spectral update of rho_hat + /N^d + inverse (the last axis).
-template <typename T, int MODE, int EPT, int NT>
-__global__ __launch_bounds__(NT) void k_col(const typename Complex<T>::type *__restrict__ src,
-                                           typename Complex<T>::type *__restrict__ dst,
-                                           typename Complex<T>::type *__restrict__ rho_hat,
-                                           const typename Complex<T>::type *__restrict__ tw, Params p, int lines,
-                                           long P, long stride, int axis) {
+// ---- strided axis pass over the half spectrum, NLC lines per block.  Line l
+// starts at (l / Nh) * P + (l % Nh) and steps by `stride`.  MODE 0: forward,
+// 1: inverse, 2: forward + spectral update of rho_hat + /N^d + inverse.
+template <typename T, int MODE, int LOG2N, int NLC>
+__global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
+    const typename Complex<T>::type *__restrict__ src, typename Complex<T>::type *__restrict__ dst,
+    typename Complex<T>::type *__restrict__ rho_hat, const typename Complex<T>::type *__restrict__ tw, Params p,
+    int lines, long P, long stride, int axis) {
     using T2 = typename Complex<T>::type;
-    constexpr int E = EPT * NT;
-    __shared__ T2 buf[E];
-    const int N = p.N, n = N, nl = E / n, Nh = N / 2 + 1;
+    using G = FftGeom<LOG2N, NLC>;
+    constexpr int n = G::n, NT = G::NT, nl = NLC;
+    __shared__ T2 buf[G::E];
+    LineFFT<T2, LOG2N, nl> fft;
+    fft.prefetch(tw, 1);
+    const int Nh = p.N / 2 + 1;
     const int l0 = blockIdx.x * nl;
-    for(int e = threadIdx.x; e < nl * n; e += NT) {
-        const int j = e / nl, lb = e - j * nl;
+#pragma unroll
+    for(int i = 0; i < 4; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int j = e / nl, lb = e % nl;
         const int l = l0 + lb;
         T2 v{0, 0};
         if(l < lines) v = src[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride];
         buf[lb * n + j] = v;
     }
     __syncthreads();
-    if constexpr(MODE == 1) {
-        lds_fft<T2, true, EPT, NT>(buf, n, p.log2N, tw, 1);
-    }
-    else {
-        lds_fft<T2, false, EPT, NT>(buf, n, p.log2N, tw, 1);
-    }
+    fft.template run<MODE == 1>(buf);
     if constexpr(MODE == 2) {
         const T inv_bins = (T)p.inv_bins;
 #pragma unroll
-        for(int i = 0; i < EPT; i++) {
+        for(int i = 0; i < 4; i++) {
             const int q = threadIdx.x + i * NT;
-            const int lb = q / n, j = q - lb * n;
+            const int lb = q / n, j = q % n;
             const int l = l0 + lb;
             if(l < lines) {
                 const size_t gi = (size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride;
                 const int ix = l % Nh;
                 int iy, iz;
-                if(axis == 1) { iy = j; iz = l / Nh; }   // y pass (2D last axis)
-                else { iy = l / Nh; iz = j; }             // z pass (3D last axis)
+                if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
+                else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
                 buf[q] = spectral_update<T, T2>(p, ix, iy, iz, buf[q], rho_hat + gi, inv_bins);
             }
         }
         __syncthreads();
-        lds_fft<T2, true, EPT, NT>(buf, n, p.log2N, tw, 1);
+        fft.template run<true>(buf);
     }
-    for(int e = threadIdx.x; e < nl * n; e += NT) {
-        const int j = e / nl, lb = e - j * nl;
+#pragma unroll
+    for(int i = 0; i < 4; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int j = e / nl, lb = e % nl;
         const int l = l0 + lb;
         if(l < lines) dst[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride] = buf[lb * n + j];
     }
@@ -285,29 +320,35 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
 }
 
 // ------------------------------------------------------------------ host side
-template <typename T, int EPT>
+// columns per block of the strided passes: NLC lines of n = N, NT = NLC n / 4
+// threads (<= 1024)
+template <int LOG2N>
+struct ColLines { static constexpr int value = LOG2N >= 11 ? 1 : (LOG2N == 10 ? 2 : (LOG2N >= 8 ? 4 : (1024 >> LOG2N))); };
+
+template <typename T, int LOG2N>
 static hipError_t ps_run_t(const SpectralArgs &a, hipStream_t s, bool init) {
     using T2 = typename Complex<T>::type;
-    constexpr int NT = 256, E = EPT * NT;
     const Params &p = a.p;
-    const int N = p.N, dims = p.dims, Nh = N / 2 + 1;
+    constexpr int N = 1 << LOG2N, Nh = N / 2 + 1;
+    constexpr int LX = LOG2N - 1;                     // x pass: complex length N / 2
+    using GX = FftGeom<LX, PackedLines<LX>::value>;
+    constexpr int NLC = ColLines<LOG2N>::value;
+    using GC = FftGeom<LOG2N, NLC>;
+    const int dims = p.dims;
     const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
     const T2 *tw2 = (const T2 *)a.twiddles;
     T *rho = (T *)a.rho;
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
-    const int nlx = E / (N / 2);
-    const int bx = (rows + nlx - 1) / nlx;
-    const int nlc = E / N;
-    // forward x pass
+    const int bx = (rows + GX::NL - 1) / GX::NL;
     if(init) {
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, EPT, NT>), dim3(bx), dim3(NT), 0, s, rho,
-                           dims == 1 ? RH : W, tw2, p, rows);
+        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, dims == 1 ? RH : W,
+                           tw2, p, rows);
     }
     else if(a.model == CH_LANDAU) {
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, EPT, NT>), dim3(bx), dim3(NT), 0, s, rho, W, tw2, p, rows);
+        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows);
     }
     else {
-        hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, EPT, NT>), dim3(bx), dim3(NT), 0, s, rho, W, tw2, p,
+        hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p,
                            rows);
     }
     if(dims == 1) {
@@ -316,44 +357,56 @@ static hipError_t ps_run_t(const SpectralArgs &a, hipStream_t s, bool init) {
     }
     else if(dims == 2) {
         const int lines = Nh;
-        const int bc = (lines + nlc - 1) / nlc;
+        const int bc = (lines + NLC - 1) / NLC;
         if(init) {
-            hipLaunchKernelGGL((k_col<T, 0, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, RH, RH, tw2, p, lines, (long)N * Nh,
-                               (long)Nh, 1);
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, RH, RH, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
             return hipGetLastError();
         }
-        hipLaunchKernelGGL((k_col<T, 2, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, W, RH, tw2, p, lines, (long)N * Nh,
-                           (long)Nh, 1);
+        hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                           (long)N * Nh, (long)Nh, 1);
     }
     else {
         const int lines = N * Nh;
-        const int bc = (lines + nlc - 1) / nlc;
+        const int bc = (lines + NLC - 1) / NLC;
         // y forward (line l: plane l / Nh, column l % Nh), then the fused z pass
-        hipLaunchKernelGGL((k_col<T, 0, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, W, RH, tw2, p, lines, (long)N * Nh,
-                           (long)Nh, 1);
+        hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                           (long)N * Nh, (long)Nh, 1);
         if(init) {
-            hipLaunchKernelGGL((k_col<T, 0, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, RH, RH, tw2, p, lines, (long)Nh,
-                               (long)N * Nh, 2);
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, RH, RH, tw2, p, lines,
+                               (long)Nh, (long)N * Nh, 2);
             return hipGetLastError();
         }
-        hipLaunchKernelGGL((k_col<T, 2, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, W, RH, tw2, p, lines, (long)Nh,
+        hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines, (long)Nh,
                            (long)N * Nh, 2);
-        hipLaunchKernelGGL((k_col<T, 1, EPT, NT>), dim3(bc), dim3(NT), 0, s, W, W, RH, tw2, p, lines, (long)N * Nh,
-                           (long)Nh, 1);
+        hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                           (long)N * Nh, (long)Nh, 1);
     }
-    hipLaunchKernelGGL((k_xinv<T, EPT, NT>), dim3(bx), dim3(NT), 0, s, W, rho, tw2, p, rows);
+    hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, rho, tw2, p, rows);
     return hipGetLastError();
 }
 
-// E = EPT * 256 complex elements per block must hold one line: n <= E.
-hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init) {
-    const int N = a.p.N;
-    if(a.precision == CH_FP64) {
-        if(N <= 2048) return ps_run_t<double, 8>(a, s, init);
-        return ps_run_t<double, 16>(a, s, init);
+template <typename T>
+static hipError_t ps_run_p(const SpectralArgs &a, hipStream_t s, bool init) {
+    switch(a.p.log2N) {
+    case 2: return ps_run_t<T, 2>(a, s, init);
+    case 3: return ps_run_t<T, 3>(a, s, init);
+    case 4: return ps_run_t<T, 4>(a, s, init);
+    case 5: return ps_run_t<T, 5>(a, s, init);
+    case 6: return ps_run_t<T, 6>(a, s, init);
+    case 7: return ps_run_t<T, 7>(a, s, init);
+    case 8: return ps_run_t<T, 8>(a, s, init);
+    case 9: return ps_run_t<T, 9>(a, s, init);
+    case 10: return ps_run_t<T, 10>(a, s, init);
+    case 11: return ps_run_t<T, 11>(a, s, init);
+    case 12: return ps_run_t<T, 12>(a, s, init);
     }
-    if(N <= 2048) return ps_run_t<float, 8>(a, s, init);
-    return ps_run_t<float, 16>(a, s, init);
+    return hipErrorInvalidValue;
+}
+
+hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init) {
+    if(a.precision == CH_FP64) return ps_run_p<double>(a, s, init);
+    return ps_run_p<float>(a, s, init);
 }
 
 }  // namespace chd

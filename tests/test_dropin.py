@@ -1,0 +1,110 @@
+"""The drop-in drivers ch_{1,2,3}D (cahn-hilliard_amd/host) run as the
+reference's own binaries do in its test-suite (tests/test_landau.py:26-65):
+the same TOML input, the same output files, the same comparisons.
+
+CPU tests cover configuration errors that stop before any device work; the
+GPU tests rerun the reference's regression and mass tests on our binaries.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+BIN = os.path.join(REPO, "cahn-hilliard_amd", "bin")
+GOLD = os.path.join(REPO, "tests", "golden")
+
+
+def run_bin(dims, cfg_text, tmp_path, check=True):
+    cfg = tmp_path / "input.toml"
+    cfg.write_text(cfg_text)
+    exe = os.path.join(BIN, f"ch_{dims}D")
+    return subprocess.run([exe, str(cfg)], cwd=tmp_path, capture_output=True, text=True, check=check, timeout=300)
+
+
+def simple_landau(integrator):
+    with open(os.path.join(GOLD, "simple_landau.toml")) as f:
+        return f'\nintegrator = "{integrator}"\n' + f.read()
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("steps = 10\nN = 48\ndt = 0.1\nk = 1.0\nfree_energy = \"landau\"\ninitial_density = 0.0\n[landau]\nepsilon = 0.9\n",
+     "N should be a power of 2"),
+    ("steps = 10\nN = 16\ndt = 0.1\nk = 1.0\nfree_energy = \"landau\"\n[landau]\nepsilon = 0.9\n",
+     "Either 'initial_density' or 'load_from' should be specified"),
+    ("steps = 10\nN = 16\ndt = 0.1\nfree_energy = \"landau\"\ninitial_density = 0.0\n[landau]\nepsilon = 0.9\n",
+     "Mandatory option 'k' not found"),
+    ("steps = 10\nN = 16\ndt = 0.1\nk = 1.0\nfree_energy = \"landau\"\ninitial_density = 0.0\n[landau]\n",
+     "requires either 'epsilon' or 'T'"),
+    ("steps = 10\nN = 16\ndt = 0.1\nk = 1.0\nfree_energy = \"foo\"\ninitial_density = 0.0\n",
+     "Unsupported free energy model 'foo'"),
+    ("steps = 10\nN = 16 16\n", "Parsing failed"),
+])
+def test_config_errors_before_device_work(tmp_path, text, msg):
+    r = run_bin(2, text, tmp_path, check=False)
+    assert r.returncode == 1
+    assert msg in (r.stderr + r.stdout)
+
+
+def test_toml_subset(tmp_path):
+    """Dotted keys, inline tables, arrays, 1e7-style integers and comments parse
+    (the run then stops at the power-of-two check, before any device work)."""
+    text = ("steps = 1e7  # float literal for an integer option\nN = 24\ndt = 1e-3\nk = [1.0]\n"
+            "free_energy = \"simple_wertheim\"\ninitial_density = 5e-5\nwertheim.valence = 4\n"
+            "[wertheim]\nB2 = 2190\ndelta = { T = 300, deltaH = -42200, deltaS = -119.1, salt = 0.5, sticky_size = 6 }\n")
+    r = run_bin(3, text, tmp_path, check=False)
+    assert r.returncode == 1 and "N should be a power of 2" in r.stderr
+    assert "valence = 4" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", ["euler", "pseudospectral"])
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_reference_regression_on_dropin_binary(tmp_path, integrator, dims):
+    """tests/test_landau.py:41-65 of the reference, pointed at ch_{dims}D."""
+    run_bin(dims, simple_landau(integrator), tmp_path)
+    phi0 = np.loadtxt(tmp_path / "init_0.dat")
+    phi1 = np.loadtxt(tmp_path / "last_0.dat")
+    assert abs(phi0.sum() - phi1.sum()) < 1e-4
+    ref = np.load(os.path.join(GOLD, "reference_landau_fixtures.npz"))[f"{integrator}_{dims}D"]
+    np.testing.assert_allclose(phi1.ravel(), ref, rtol=1e-12, atol=1e-14)
+    # the header line the reference writes (src/utils/Printer.cpp:183)
+    head = (tmp_path / "last_0.dat").read_text().split("\n", 1)[0]
+    assert head.startswith("# step = 100, t = 0.1, size = 64x") and head.endswith("| density")
+
+
+@pytest.mark.gpu
+def test_energy_file_and_restart(tmp_path):
+    """energy.dat cadence (a line *before* every print_every-th step), FP64
+    observables, and a load_from restart continuing the step counter."""
+    text = ("integrator = \"euler\"\nsteps = 300\nprint_every = 100\nprint_last_every = 150\nN = 32\ndt = 0.01\n"
+            "k = 1.0\nfree_energy = \"landau\"\ninitial_density = 0.0\ninitial_A = 0.3\nseed = 123456\n"
+            "[landau]\nepsilon = 0.9\n")
+    r = run_bin(2, text, tmp_path)
+    lines = (tmp_path / "energy.dat").read_text().split("\n")
+    lines = [l for l in lines if l]
+    assert [l.split()[3] for l in lines] == ["0", "100", "200"]
+    assert lines[0].split()[0] == "0.0" and lines[1].split()[0] == "1.0"
+    assert r.stdout.split("\n")[0] == lines[0]
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    p = O.Params(dims=2, N=32, dt=0.01, k=1.0)
+    rho0 = O.init_density(2, 32, 0.0, 0.3, 0, 123456)
+    F0, m0 = float(lines[0].split()[1]), float(lines[0].split()[2])
+    assert F0 == pytest.approx(O.average_free_energy(p, O.Model(epsilon=0.9), rho0), rel=1e-7)
+    assert m0 == pytest.approx(O.average_mass(p, rho0), rel=1e-4, abs=1e-9)
+    # last_0.dat written at t = 150 (print_last_every) then overwritten at the end
+    head = (tmp_path / "last_0.dat").read_text().split("\n", 1)[0]
+    assert head.startswith("# step = 300,")
+    # restart from it: step counter continues at 300
+    rs = tmp_path / "restart"
+    rs.mkdir()
+    shutil.copy(tmp_path / "last_0.dat", rs / "start.dat")
+    text2 = text.replace("initial_density = 0.0\n", "load_from = \"start.dat\"\n").replace("steps = 300", "steps = 100")
+    run_bin(2, text2, rs)
+    e2 = [l for l in (rs / "energy.dat").read_text().split("\n") if l]
+    assert e2[0].split()[3] == "300"
