@@ -87,6 +87,9 @@ def lib():
         "ch_sys_evolve": (ctypes.c_int, [P, I64]),
         "ch_sys_steps_per_launch": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
+        "ch_sys_record": (ctypes.c_int, [P, I64, I32]),
+        "ch_sys_adjoint": (ctypes.c_int, [P, P, P, P]),
+        "ch_sys_adjoint_device": (ctypes.c_int, [P, P, P, P]),
         "ch_der_bulk_free_energy": (ctypes.c_int, [P, DP, P, P, I64]),
         "ch_set_mobility": (ctypes.c_int, [P, DP, P, P, I64]),
     }

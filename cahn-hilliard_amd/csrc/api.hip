@@ -13,6 +13,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "ch_internal.h"
 
@@ -37,6 +38,12 @@ struct ch_sys {
     void *stage = nullptr;   // bins doubles: host transfers of FP32 fields, mobility
     void *stage2 = nullptr;  // bins elements of T: mobility
     double *partials = nullptr, *result = nullptr;
+    // reverse mode: checkpoints, the window tape, lambda ping-pong, gradient slots
+    void *ckpt = nullptr, *tape = nullptr, *lam[2] = {nullptr, nullptr};
+    double *gacc = nullptr;
+    int64_t ckpt_cap = 0, tape_cap = 0;  // capacities in fields
+    int64_t rec_steps = 0, rec_window = 0;
+    bool rec_valid = false, rec_full_tape = false;
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -209,7 +216,8 @@ static void free_sys(ch_sys *s) {
         if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
-    void *bufs[] = {s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result};
+    void *bufs[] = {s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
+                    s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
     delete s;
@@ -431,6 +439,165 @@ extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_en
     if(free_energy) *free_energy = f;
     if(std::isnan(h[0])) return fail(c, CH_ENAN, "Encountered a nan while computing the total mass");
     if(std::isnan(h[1])) return fail(c, CH_ENAN, "Encountered a nan while computing the total free energy");
+    return CH_OK;
+}
+
+// ------------------------------------------------------------------ reverse mode
+static bool grow(void **buf, int64_t &cap, int64_t fields, size_t field_bytes) {
+    if(fields <= cap && *buf) return true;
+    if(*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    cap = 0;
+    if(hipMalloc(buf, (size_t)fields * field_bytes) != hipSuccess) {
+        *buf = nullptr;
+        return false;
+    }
+    cap = fields;
+    return true;
+}
+
+static char *field_at(void *base, int64_t i, size_t field_bytes) { return (char *)base + (size_t)i * field_bytes; }
+
+// K steps from tape slot `from` into slots from+1 .. from+K
+static hipError_t tape_steps(ch_sys *s, int64_t from, int64_t count) {
+    const size_t fb = s->bins * s->elt;
+    const int Kt = chd::adjoint_steps_per_launch(s->d.dims, s->d.N);
+    while(count > 0) {
+        const int K = count >= Kt ? Kt : 1;
+        hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, K, field_at(s->tape, from, fb),
+                                        field_at(s->tape, from + 1, fb), s->ctx->stream);
+        if(e != hipSuccess) return e;
+        from += K;
+        count -= K;
+    }
+    return hipSuccess;
+}
+
+extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every) {
+    if(!s || nsteps < 1 || ckpt_every < 0) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->d.integrator != CH_EULER || s->d.mobility != CH_MOB_CONSTANT)
+        return fail(c, CH_EUNSUPPORTED, "the adjoint covers the euler integrator with constant mobility");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t fb = s->bins * s->elt;
+    s->rec_valid = false;
+    s->rec_full_tape = ckpt_every == 0;
+    s->rec_window = ckpt_every == 0 ? nsteps : std::min<int64_t>(ckpt_every, nsteps);
+    const int64_t nck = (nsteps + s->rec_window - 1) / s->rec_window;
+    if(!grow(&s->tape, s->tape_cap, s->rec_window, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)))
+        return fail(c, CH_ENOMEM, "no device memory for %lld checkpoints / a %lld-state tape", (long long)nck,
+                    (long long)s->rec_window);
+    if(s->rec_full_tape) {
+        // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
+        HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, tape_steps(s, 0, nsteps - 1));
+        HIPCHK(c, chd::tape_launch(s->p, s->d.precision, s->d.model, 1, field_at(s->tape, nsteps - 1, fb),
+                                   s->psi[s->cur], c->stream));
+    }
+    else {
+        for(int64_t k = 0; k < nck; k++) {
+            HIPCHK(c, hipMemcpyAsync(field_at(s->ckpt, k, fb), s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
+            const int64_t steps = std::min<int64_t>(s->rec_window, nsteps - k * s->rec_window);
+            ch_status st = ch_sys_evolve(s, steps);
+            if(st != CH_OK) return st;
+        }
+    }
+    s->rec_steps = nsteps;
+    s->rec_valid = true;
+    return CH_OK;
+}
+
+static ch_status adjoint_core(ch_sys *s, double *grads) {
+    ch_ctx *c = s->ctx;
+    const size_t fb = s->bins * s->elt;
+    const int ntiles = chd::adjoint_tiles(s->d.dims, s->d.N);
+    const int KA = chd::adjoint_steps_per_launch(s->d.dims, s->d.N);
+    const int64_t W = s->rec_window, T = s->rec_steps;
+    const int64_t nck = (T + W - 1) / W;
+    int li = 0;  // lambda lives in lam[li]
+    HIPCHK(c, chd::fill_zero(s->gacc, 3 * ntiles, c->stream));
+    for(int64_t k = nck - 1; k >= 0; k--) {
+        const int64_t start = k * W, len = std::min<int64_t>(W, T - start);
+        if(!s->rec_full_tape) {
+            HIPCHK(c, hipMemcpyAsync(s->tape, field_at(s->ckpt, k, fb), fb, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, tape_steps(s, 0, len - 1));
+        }
+        for(int64_t t = len - 1; t >= 0;) {
+            const int ka = (t + 1 >= KA) ? KA : 1;
+            HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, ka, s->lam[li], s->lam[li ^ 1], s->tape,
+                                          (int)t, s->gacc, c->stream));
+            li ^= 1;
+            t -= ka;
+        }
+    }
+    std::vector<double> g(3 * (size_t)ntiles);
+    HIPCHK(c, hipMemcpyAsync(g.data(), s->gacc, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double ge = 0, gk = 0, gm = 0;
+    for(int i = 0; i < ntiles; i++) {
+        ge += g[3 * i];
+        gk += g[3 * i + 1];
+        gm += g[3 * i + 2];
+    }
+    if(grads) {
+        grads[0] = s->d.model == CH_LANDAU ? -s->d.dt * s->d.M * ge : 0.0;
+        grads[1] = -2.0 * s->d.dt * s->d.M * gk;
+        grads[2] = s->d.dt * gm;
+    }
+    if(li == 1) HIPCHK(c, hipMemcpyAsync(s->lam[0], s->lam[1], fb, hipMemcpyDeviceToDevice, c->stream));
+    return CH_OK;
+}
+
+static ch_status adjoint_prepare(ch_sys *s) {
+    ch_ctx *c = s->ctx;
+    if(!s->rec_valid) return fail(c, CH_ESTATE, "ch_sys_adjoint needs a ch_sys_record first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t fb = s->bins * s->elt;
+    int64_t cap0 = s->lam[0] ? 1 : 0, cap1 = s->lam[1] ? 1 : 0, capg = s->gacc ? 1 : 0;
+    if(!grow(&s->lam[0], cap0, 1, fb) || !grow(&s->lam[1], cap1, 1, fb) ||
+       !grow((void **)&s->gacc, capg, 1, 3 * sizeof(double) * chd::adjoint_tiles(s->d.dims, s->d.N)))
+        return fail(c, CH_ENOMEM, "no device memory for the adjoint");
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_adjoint(ch_sys *s, const double *dL_dpsiT, double *dL_dpsi0, double *grads) {
+    if(!s || !dL_dpsiT) return CH_EINVAL;
+    ch_status st = adjoint_prepare(s);
+    if(st != CH_OK) return st;
+    ch_ctx *c = s->ctx;
+    if(s->d.precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(s->lam[0], dL_dpsiT, s->bins * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    else {
+        HIPCHK(c, hipMemcpyAsync(s->stage, dL_dpsiT, s->bins * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, chd::convert_d2f((const double *)s->stage, (float *)s->lam[0], s->bins, c->stream));
+    }
+    st = adjoint_core(s, grads);
+    if(st != CH_OK) return st;
+    if(dL_dpsi0) {
+        if(s->d.precision == CH_FP64) {
+            HIPCHK(c, hipMemcpyAsync(dL_dpsi0, s->lam[0], s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        else {
+            HIPCHK(c, chd::convert_f2d((const float *)s->lam[0], (double *)s->stage, s->bins, c->stream));
+            HIPCHK(c, hipMemcpyAsync(dL_dpsi0, s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_adjoint_device(ch_sys *s, const void *dL_dpsiT_dev, void *dL_dpsi0_dev, double *grads) {
+    if(!s || !dL_dpsiT_dev) return CH_EINVAL;
+    ch_status st = adjoint_prepare(s);
+    if(st != CH_OK) return st;
+    ch_ctx *c = s->ctx;
+    const size_t fb = s->bins * s->elt;
+    HIPCHK(c, hipMemcpyAsync(s->lam[0], dL_dpsiT_dev, fb, hipMemcpyDeviceToDevice, c->stream));
+    st = adjoint_core(s, grads);
+    if(st != CH_OK) return st;
+    if(dL_dpsi0_dev) HIPCHK(c, hipMemcpyAsync(dL_dpsi0_dev, s->lam[0], fb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return CH_OK;
 }
 

@@ -30,6 +30,15 @@ hipError_t observables(const Params &p, int precision, int model, const void *rh
                        double *result, hipStream_t s);
 constexpr int kObsParts = 1024;
 
+// adjoint (adjoint.hip)
+int adjoint_steps_per_launch(int dims, int N);
+int adjoint_tiles(int dims, int N);
+hipError_t adjoint_launch(const Params &p, int precision, int model, int KA, const void *lam_in, void *lam_out,
+                          const void *tape, int s_top, double *gacc, hipStream_t s);
+hipError_t tape_launch(const Params &p, int precision, int model, int K, const void *in, void *tape_out,
+                       hipStream_t s);
+hipError_t fill_zero(double *a, int n, hipStream_t s);
+
 hipError_t pointwise_fprime(const Params &p, int precision, int model, const void *in, void *out, int64_t n,
                             hipStream_t s);
 hipError_t pointwise_mobility(const Params &p, int precision, int mob, const void *in, void *out, int64_t n,

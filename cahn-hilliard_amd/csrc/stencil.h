@@ -29,7 +29,23 @@ struct TileGeom {
 };
 
 
-template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K, int NT>
+// store the tile interior of the LDS region A into the field `out`
+template <typename T, int DIMS, int TX, int TY, int TZ, int RX, int RXY, int H>
+__device__ __forceinline__ void store_tile(const T *A, T *__restrict__ out, int N, int tx, int ty, int tz, int NT) {
+    constexpr int TY_ = DIMS >= 2 ? TY : 1, TZ_ = DIMS == 3 ? TZ : 1;
+    for(int c = threadIdx.x; c < TX * TY_ * TZ_; c += NT) {
+        const int lx = c % TX, ly = (c / TX) % TY_, lz = c / (TX * TY_);
+        const int li = (H + lx) + (DIMS >= 2 ? (H + ly) * RX : 0) + (DIMS == 3 ? (H + lz) * RXY : 0);
+        const int gx = tx * TX + lx;
+        const int gy = DIMS >= 2 ? ty * TY + ly : 0;
+        const int gz = DIMS == 3 ? tz * TZ + lz : 0;
+        out[((size_t)gz * N + gy) * N + gx] = A[li];
+    }
+}
+
+// TAPE = true: `out` holds K consecutive fields and the state after every one
+// of the K steps is stored (the adjoint's recompute of a checkpoint window).
+template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K, int NT, bool TAPE = false>
 __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__restrict__ out, Params p,
                                                 int tiles_x, int tiles_y) {
     using G = TileGeom<DIMS, TX, TY, TZ, K>;
@@ -108,17 +124,14 @@ __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__r
             }
         }
         __syncthreads();
+        if constexpr(TAPE) {
+            if(s < K - 1) store_tile<T, DIMS, TX, TY, TZ, RX, RXY, H>(A, out + (size_t)s * ((size_t)1 << (p.log2N * DIMS)), N,
+                                                                     tx, ty, tz, NT);
+        }
     }
 
-    constexpr int TY_ = DIMS >= 2 ? TY : 1, TZ_ = DIMS == 3 ? TZ : 1;
-    for(int c = threadIdx.x; c < TX * TY_ * TZ_; c += NT) {
-        const int lx = c % TX, ly = (c / TX) % TY_, lz = c / (TX * TY_);
-        const int li = (H + lx) + (DIMS >= 2 ? (H + ly) * RX : 0) + (DIMS == 3 ? (H + lz) * RXY : 0);
-        const int gx = tx * TX + lx;
-        const int gy = DIMS >= 2 ? ty * TY + ly : 0;
-        const int gz = DIMS == 3 ? tz * TZ + lz : 0;
-        out[((size_t)gz * N + gy) * N + gx] = A[li];
-    }
+    T *dst = TAPE ? out + (size_t)(K - 1) * ((size_t)1 << (p.log2N * DIMS)) : out;
+    store_tile<T, DIMS, TX, TY, TZ, RX, RXY, H>(A, dst, N, tx, ty, tz, NT);
 }
 
 template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K>

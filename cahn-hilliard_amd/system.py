@@ -172,6 +172,25 @@ class System:
     def average_free_energy(self):
         return self.observables()[1]
 
+    def record(self, nsteps, ckpt_every=100):
+        """Advance nsteps keeping a checkpoint every ckpt_every steps (0: every state)."""
+        self._chk(L.lib().ch_sys_record(self.handle, int(nsteps), int(ckpt_every)))
+
+    def adjoint(self, dL_dpsiT):
+        """Reverse sweep of the last record: returns (dL/dpsi0, [dL/deps, dL/dk, dL/dM])."""
+        g = np.ascontiguousarray(dL_dpsiT, dtype=np.float64).ravel()
+        out = np.empty(self.bins, dtype=np.float64)
+        grads = np.zeros(3)
+        self._chk(L.lib().ch_sys_adjoint(self.handle, g.ctypes.data_as(ctypes.c_void_p),
+                                         out.ctypes.data_as(ctypes.c_void_p), grads.ctypes.data_as(ctypes.c_void_p)))
+        return out, grads
+
+    def adjoint_device(self, dL_dpsiT_ptr, dL_dpsi0_ptr):
+        grads = np.zeros(3)
+        self._chk(L.lib().ch_sys_adjoint_device(self.handle, ctypes.c_void_p(dL_dpsiT_ptr), ctypes.c_void_p(dL_dpsi0_ptr),
+                                                grads.ctypes.data_as(ctypes.c_void_p)))
+        return grads
+
     def close(self):
         if getattr(self, "handle", None):
             L.lib().ch_sys_destroy(self.handle)

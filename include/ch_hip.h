@@ -121,6 +121,21 @@ int32_t ch_sys_steps_per_launch(const ch_sys *sys);
  * Returns CH_ENAN when either sum is NaN. */
 ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 
+/* ---- reverse mode (no reference counterpart: the reference has no
+ * gradients).  Euler integrator with constant mobility, both models.
+ *
+ * ch_sys_record: advance the current state nsteps times (as ch_sys_evolve)
+ * keeping a checkpoint every `ckpt_every` steps; ckpt_every = 0 keeps every
+ * state (no recomputation).  The system ends at psi_T. */
+ch_status ch_sys_record(ch_sys *sys, int64_t nsteps, int32_t ckpt_every);
+/* Reverse sweep over the last record: given dL/dpsi_T (N^dims doubles on the
+ * host), writes dL/dpsi_0 and grads[3] = {dL/d epsilon (Landau; 0 for
+ * SimpleWertheim), dL/dk, dL/dM}.  Windows between checkpoints are
+ * recomputed on the device.  Synchronous. */
+ch_status ch_sys_adjoint(ch_sys *sys, const double *dL_dpsiT, double *dL_dpsi0, double *grads);
+/* Same with device buffers of the system's precision. */
+ch_status ch_sys_adjoint_device(ch_sys *sys, const void *dL_dpsiT_dev, void *dL_dpsi0_dev, double *grads);
+
 /* == FreeEnergyModel::der_bulk_free_energy(field_type*, float*, int) and
  * FreeEnergyModel::set_mobility(field_type*, double, field_type*, int)
  * (src/models/FreeEnergyModel.h:29,48; Landau.cu:5-19, SimpleWertheim.cu:10-42):

@@ -1,0 +1,137 @@
+"""Reverse mode of the Euler integrator on the GPU (ch_sys_record / ch_sys_adjoint).
+
+Tolerances: FP64 dL/dpsi0 and dL/d(eps, k, M) within 1e-10 relative of the
+oracle's FP64 adjoint (oracle/ch_oracle.c: och_euler_adjoint); central finite
+differences of the GPU forward map (h = 1e-6) within 1e-6; every checkpoint
+spacing gives the same dL/dpsi0 bits; FP32 fields within 2e-3 of FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(ch):
+    c = ch.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_adjoint(ch, ctx, kw, psi0, steps, w, ckpt=100, precision="fp64", eps=0.9):
+    s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=eps, precision=precision, **kw))
+    s.upload(psi0)
+    s.record(steps, ckpt)
+    psiT = s.download()
+    lam, g = s.adjoint(w)
+    s.close()
+    return psiT, lam, g
+
+
+@pytest.mark.parametrize("dims,N,steps,ckpt", [(2, 64, 60, 16), (2, 32, 40, 0), (1, 128, 80, 25), (3, 16, 30, 7)])
+def test_adjoint_against_oracle(ch, ctx, dims, N, steps, ckpt):
+    kw = dict(dims=dims, N=N, dt=0.01 if dims < 3 else 0.002, dx=1.0, k=1.2, M=0.9)
+    psi0 = O.init_density(dims, N, 0.0, 0.3, 0, 123456)
+    w = np.random.default_rng(dims).normal(size=N ** dims)
+    psiT, lam, g = gpu_adjoint(ch, ctx, kw, psi0, steps, w, ckpt)
+    ref_lam, ref_g = O.euler_adjoint(O.Params(**kw), O.Model(epsilon=0.9), psi0, steps, w)
+    ref_T = O.run(O.EULER, O.Params(**kw), O.Model(epsilon=0.9), psi0, steps)
+    assert rel_linf(psiT, ref_T) < 1e-12
+    assert rel_linf(lam, ref_lam) < 1e-10
+    np.testing.assert_allclose(g, ref_g, rtol=1e-10)
+
+
+def test_checkpoint_spacing_invariance(ch, ctx):
+    kw = dict(dims=2, N=64, dt=0.01, dx=1.0, k=1.0, M=1.0)
+    psi0 = O.init_density(2, 64, 0.0, 0.3, 0, 5)
+    w = np.random.default_rng(0).normal(size=64 * 64)
+    base = None
+    for ckpt in (0, 1, 3, 16, 37, 100):
+        _, lam, g = gpu_adjoint(ch, ctx, kw, psi0, 100, w, ckpt)
+        if base is None:
+            base = (lam, g)
+        else:
+            assert np.array_equal(lam, base[0]), ckpt
+            np.testing.assert_allclose(g, base[1], rtol=1e-12)
+
+
+def test_adjoint_finite_differences(ch, ctx):
+    """Central differences (h = 1e-6) of L = <w, psi_T> through the GPU forward."""
+    N, steps = 64, 200
+    kw = dict(dims=2, N=N, dt=0.01, dx=1.0)
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 123456)
+    w = np.random.default_rng(1).normal(size=N * N)
+    d = np.random.default_rng(2).normal(size=N * N)
+    params = dict(eps=0.9, k=1.2, M=1.0)
+
+    def loss(x, eps, k, M):
+        s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=eps, k=k, M=M, **kw))
+        s.upload(x)
+        s.evolve(steps)
+        v = float(np.dot(w, s.download()))
+        s.close()
+        return v
+
+    s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.9, k=1.2, M=1.0, **kw))
+    s.upload(psi0)
+    s.record(steps, 50)
+    lam, g = s.adjoint(w)
+    s.close()
+    h = 1e-6
+    fd = (loss(psi0 + h * d, **params) - loss(psi0 - h * d, **params)) / (2 * h)
+    assert np.dot(lam, d) == pytest.approx(fd, rel=1e-6)
+    for i, name in enumerate(("eps", "k", "M")):
+        up, dn = dict(params), dict(params)
+        up[name] += h
+        dn[name] -= h
+        fd = (loss(psi0, **up) - loss(psi0, **dn)) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-6), name
+
+
+def test_fp32_adjoint_against_fp64(ch, ctx):
+    kw = dict(dims=2, N=64, dt=0.01, dx=1.0, k=1.0, M=1.0)
+    psi0 = O.init_density(2, 64, 0.0, 0.3, 0, 9)
+    w = np.random.default_rng(3).normal(size=64 * 64)
+    _, lam64, g64 = gpu_adjoint(ch, ctx, kw, psi0, 100, w, 25)
+    _, lam32, g32 = gpu_adjoint(ch, ctx, kw, psi0, 100, w, 25, precision="fp32")
+    err = rel_linf(lam32, lam64)
+    print(f"fp32 adjoint rel L-inf vs fp64 = {err:.3e}, grads {np.abs(g32 / g64 - 1).max():.3e}")
+    assert err < 2e-3
+    np.testing.assert_allclose(g32, g64, rtol=2e-3)
+
+
+def test_wertheim_adjoint_against_oracle(ch, ctx):
+    """SimpleWertheim f'' in the adjoint (grads[0] is 0: no epsilon)."""
+    kw = dict(dims=2, N=32, dt=1e-4, dx=1.0, k=1.0, M=1.0)
+    mk = dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4)
+    psi0 = O.init_density(2, 32, 0.05, 0.05, 0, 7)
+    w = np.random.default_rng(4).normal(size=32 * 32)
+    s = ch.System(ctx, ch.make_desc(integrator="euler", **mk, **kw))
+    s.upload(psi0)
+    s.record(50, 10)
+    lam, g = s.adjoint(w)
+    s.close()
+    om = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4)
+    ref_lam, ref_g = O.euler_adjoint(O.Params(**kw), om, psi0, 50, w)
+    assert rel_linf(lam, ref_lam) < 1e-10
+    assert g[0] == 0.0
+    np.testing.assert_allclose(g[1:], ref_g[1:], rtol=1e-9)
+
+
+def test_autograd_gradcheck(ch):
+    import torch
+    from cahn_hilliard_amd.autograd import euler_evolve
+    N = 16
+    psi0 = torch.tensor(O.init_density(2, N, 0.0, 0.3, 0, 3), dtype=torch.float64, device="cuda", requires_grad=True)
+    eps = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    k = torch.tensor(1.1, dtype=torch.float64, device="cuda", requires_grad=True)
+    M = torch.tensor(0.8, dtype=torch.float64, device="cuda", requires_grad=True)
+    fn = lambda a, b, c, d: euler_evolve(a, b, c, d, 20, 5, dims=2, N=N, dt=0.01)
+    assert torch.autograd.gradcheck(fn, (psi0, eps, k, M), eps=1e-6, atol=1e-8, rtol=1e-6)
+    # composition with a torch loss
+    out = fn(psi0, eps, k, M)
+    (out ** 2).sum().backward()
+    assert psi0.grad is not None and torch.isfinite(psi0.grad).all()
