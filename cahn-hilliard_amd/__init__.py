@@ -8,8 +8,8 @@ lives in host/.
 from . import _lib
 from ._lib import (CHError, EULER, EULER_MOBILITY, FP32, FP64, LANDAU, PSEUDOSPECTRAL, SIMPLE_WERTHEIM,
                    header_functions)
-from .system import Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility
+from .system import Batch, Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility
 
-__all__ = ["CHError", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
+__all__ = ["CHError", "Batch", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
            "set_mobility", "header_functions", "EULER", "EULER_MOBILITY", "PSEUDOSPECTRAL", "LANDAU",
            "SIMPLE_WERTHEIM", "FP32", "FP64"]

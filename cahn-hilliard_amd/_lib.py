@@ -87,6 +87,13 @@ def lib():
         "ch_sys_evolve": (ctypes.c_int, [P, I64]),
         "ch_sys_steps_per_launch": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
+        "ch_batch_create": (ctypes.c_int, [P, I32, DP, PP]),
+        "ch_batch_destroy": (None, [P]),
+        "ch_batch_upload": (ctypes.c_int, [P, I32, P]),
+        "ch_batch_download": (ctypes.c_int, [P, I32, P]),
+        "ch_batch_evolve": (ctypes.c_int, [P, I64]),
+        "ch_batch_observables": (ctypes.c_int, [P, P, P]),
+        "ch_batch_steps_per_launch": (ctypes.c_int32, [P]),
         "ch_sys_record": (ctypes.c_int, [P, I64, I32]),
         "ch_sys_adjoint": (ctypes.c_int, [P, P, P, P]),
         "ch_sys_adjoint_device": (ctypes.c_int, [P, P, P, P]),

@@ -49,8 +49,9 @@ class EulerEvolve(torch.autograd.Function):
         g = grad_out.detach().contiguous()
         gin = torch.empty_like(g)
         torch.cuda.synchronize()
+        # the recorded system stays alive with the graph: backward may run more
+        # than once (retain_graph, gradcheck's row-by-row vector-Jacobian products)
         grads = fctx.sysm.adjoint_device(g.data_ptr(), gin.data_ptr())
-        fctx.sysm.close()
         pg = [torch.tensor(grads[i], dtype=fctx.param_dtype[i], device=fctx.param_device[i]) for i in range(3)]
         return gin, pg[0], pg[1], pg[2], None, None, None
 

@@ -17,8 +17,10 @@
 
 #include "ch_internal.h"
 
-// default temporal-blocking depth (time steps per stencil launch)
+// default temporal-blocking depth (time steps per stencil launch) and region
+// shape of the 2D wave kernel
 static constexpr int kDefaultTB = 4;
+static constexpr int kDefaultWaveCfg = 0;
 
 struct ch_ctx {
     int device = 0;
@@ -174,7 +176,7 @@ static chd::Params make_params(const ch_desc *d) {
     p.inv_dx2 = 1.0 / (d->dx * d->dx);
     p.dt = d->dt;
     p.Mdt = d->M * d->dt;
-    p.wave_rpw = 16;
+    p.wave_cfg = kDefaultWaveCfg;
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
@@ -245,7 +247,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     if(const char *tb = std::getenv("CH_TB_STEPS")) want = std::atoi(tb);
     s->tb = 1;
     while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
-    if(const char *r = std::getenv("CH_WAVE_RPW")) s->p.wave_rpw = std::atoi(r) == 32 ? 32 : 16;
+    if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(2, std::atoi(r)));
     // graph chunk: a multiple of 2 tb launches' worth of steps (parity restored)
     const char *g = std::getenv("CH_GRAPH_STEPS");
     s->graph_steps = g ? std::atoi(g) : 64;
@@ -439,6 +441,193 @@ extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_en
     if(free_energy) *free_energy = f;
     if(std::isnan(h[0])) return fail(c, CH_ENAN, "Encountered a nan while computing the total mass");
     if(std::isnan(h[1])) return fail(c, CH_ENAN, "Encountered a nan while computing the total free energy");
+    return CH_OK;
+}
+
+// ------------------------------------------------------------------ batches
+static constexpr int kDefaultBatchTB = 4;
+
+struct ch_batch {
+    ch_ctx *ctx = nullptr;
+    int n = 0, precision = CH_FP64, model = CH_LANDAU, tb = 1, nitems = 0, graph_steps = 0;
+    size_t elt = 8;
+    int64_t total = 0;                       // elements over all replicas
+    std::vector<chd::BatchRep> host_reps;
+    chd::BatchRep *reps = nullptr;
+    int2 *items = nullptr;
+    void *buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *stage = nullptr;                   // largest replica, doubles
+    double *result = nullptr;                // 2 n doubles
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+};
+
+static void free_batch(ch_batch *b) {
+    for(int i = 0; i < 2; i++) {
+        if(b->gexec[i]) (void)hipGraphExecDestroy(b->gexec[i]);
+        if(b->buf[i]) (void)hipFree(b->buf[i]);
+    }
+    void *bufs[] = {b->reps, b->items, b->stage, b->result};
+    for(void *x : bufs)
+        if(x) (void)hipFree(x);
+    delete b;
+}
+
+extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *descs, ch_batch **out) {
+    if(!ctx || !out || !descs || n < 1) return CH_EINVAL;
+    *out = nullptr;
+    for(int r = 0; r < n; r++) {
+        ch_status st = validate(ctx, &descs[r]);
+        if(st != CH_OK) return fail(ctx, st, "replica %d: %s", r, ctx->err.c_str());
+        const ch_desc &d = descs[r];
+        if(d.dims != 2 || d.integrator != CH_EULER || d.N < 64)
+            return fail(ctx, CH_EUNSUPPORTED, "replica %d: batches hold 2D euler systems with N >= 64", r);
+        if(d.precision != descs[0].precision || d.model != descs[0].model)
+            return fail(ctx, CH_EINVAL, "replica %d: all replicas share one precision and one model", r);
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ch_batch *b = new ch_batch();
+    b->ctx = ctx;
+    b->n = n;
+    b->precision = descs[0].precision;
+    b->model = descs[0].model;
+    b->elt = b->precision == CH_FP64 ? 8 : 4;
+    int want = kDefaultBatchTB;
+    if(const char *e = std::getenv("CH_BATCH_TB")) want = std::atoi(e);
+    b->tb = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
+    const int W = chd::batch_tile_w(b->tb), TH = chd::batch_tile_h(b->tb);
+    std::vector<int2> items;
+    int64_t maxbins = 0;
+    for(int r = 0; r < n; r++) {
+        chd::BatchRep rep{};
+        rep.p = make_params(&descs[r]);
+        rep.off = b->total;
+        const int N = descs[r].N;
+        rep.whole = N == 64 ? 1 : 0;
+        rep.tiles_x = rep.whole ? 1 : (N + W - 1) / W;
+        rep.tiles = rep.whole ? 1 : rep.tiles_x * ((N + TH - 1) / TH);
+        for(int t = 0; t < rep.tiles; t++) items.push_back(int2{r, t});
+        b->total += (int64_t)N * N;
+        maxbins = std::max<int64_t>(maxbins, (int64_t)N * N);
+        b->host_reps.push_back(rep);
+    }
+    b->nitems = (int)items.size();
+    const char *g = std::getenv("CH_GRAPH_STEPS");
+    b->graph_steps = g ? std::atoi(g) : 64;
+    if(b->graph_steps < 0) b->graph_steps = 0;
+    b->graph_steps -= b->graph_steps % (2 * b->tb);
+    bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
+              hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
+              hipMalloc((void **)&b->items, items.size() * sizeof(int2)) == hipSuccess &&
+              hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
+    if(ok) ok = hipMemcpy(b->reps, b->host_reps.data(), n * sizeof(chd::BatchRep), hipMemcpyHostToDevice) == hipSuccess &&
+                hipMemcpy(b->items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    if(!ok) {
+        free_batch(b);
+        return fail(ctx, CH_ENOMEM, "device allocation failed for a batch of %d replicas", n);
+    }
+    *out = b;
+    return CH_OK;
+}
+
+extern "C" void ch_batch_destroy(ch_batch *b) {
+    if(!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    free_batch(b);
+}
+
+extern "C" int32_t ch_batch_steps_per_launch(const ch_batch *b) { return b ? b->tb : 0; }
+
+extern "C" ch_status ch_batch_upload(ch_batch *b, int32_t r, const double *rho_host) {
+    if(!b || !rho_host || r < 0 || r >= b->n) return CH_EINVAL;
+    ch_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const chd::BatchRep &rep = b->host_reps[r];
+    const int64_t bins = (int64_t)rep.p.N * rep.p.N;
+    char *dst = (char *)b->buf[b->cur] + rep.off * b->elt;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if(b->precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(dst, rho_host, bins * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    else {
+        HIPCHK(c, hipMemcpyAsync(b->stage, rho_host, bins * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, chd::convert_d2f((const double *)b->stage, (float *)dst, bins, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host) {
+    if(!b || !rho_host || r < 0 || r >= b->n) return CH_EINVAL;
+    ch_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const chd::BatchRep &rep = b->host_reps[r];
+    const int64_t bins = (int64_t)rep.p.N * rep.p.N;
+    const char *src = (const char *)b->buf[b->cur] + rep.off * b->elt;
+    if(b->precision == CH_FP64) {
+        HIPCHK(c, hipMemcpyAsync(rho_host, src, bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    else {
+        HIPCHK(c, chd::convert_f2d((const float *)src, (double *)b->stage, bins, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rho_host, b->stage, bins * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+static hipError_t batch_step(ch_batch *b, int &cur, int K) {
+    hipError_t e = chd::batch_launch(b->precision, b->model, K, b->buf[cur], b->buf[cur ^ 1], b->reps, b->items,
+                                     b->nitems, b->ctx->stream);
+    cur ^= 1;
+    return e;
+}
+
+extern "C" ch_status ch_batch_evolve(ch_batch *b, int64_t nsteps) {
+    if(!b || nsteps < 0) return CH_EINVAL;
+    ch_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int G = b->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        const int par = b->cur;
+        if(!b->gexec[par]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int cur = b->cur;
+            hipError_t e = hipSuccess;
+            for(int i = 0; i < G && e == hipSuccess; i += b->tb) e = batch_step(b, cur, b->tb);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "batch launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&b->gexec[par], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(b->gexec[par], c->stream));
+        nsteps -= reps * G;
+    }
+    for(; nsteps >= b->tb; nsteps -= b->tb) HIPCHK(c, batch_step(b, b->cur, b->tb));
+    for(; nsteps > 0; nsteps--) HIPCHK(c, batch_step(b, b->cur, 1));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_batch_observables(ch_batch *b, double *mass, double *free_energy) {
+    if(!b) return CH_EINVAL;
+    ch_ctx *c = b->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, chd::batch_observables(b->precision, b->model, b->buf[b->cur], b->reps, b->n, b->result, c->stream));
+    std::vector<double> h(2 * (size_t)b->n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), b->result, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    bool nan = false;
+    for(int r = 0; r < b->n; r++) {
+        const double s = b->host_reps[r].p.vbin_over_bins;
+        if(mass) mass[r] = h[2 * r] * s;
+        if(free_energy) free_energy[r] = h[2 * r + 1] * s;
+        nan = nan || std::isnan(h[2 * r]) || std::isnan(h[2 * r + 1]);
+    }
+    if(nan) return fail(c, CH_ENAN, "Encountered a nan while computing the observables of a replica");
     return CH_OK;
 }
 

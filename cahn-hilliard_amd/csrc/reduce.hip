@@ -17,37 +17,41 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// adds cell i's density and free-energy density (bulk + k sum_d (grad_d+)^2,
+// forward differences as CahnHilliard.cpp:256-276, 300-319) to m and f
+template <typename T, int MODEL>
+__device__ __forceinline__ void obs_cell(const T *__restrict__ rho, const Params &p, int64_t i, double &m, double &f) {
+    const int mask = p.mask, lg = p.log2N;
+    const double r = (double)rho[i];
+    m += r;
+    const int x = (int)(i & mask);
+    double inter = 0.;
+    {
+        const int64_t ip = (i - x) + ((x + 1) & mask);
+        const double g = ((double)rho[ip] - r) * p.inv_dx;
+        inter += p.k * g * g;
+    }
+    if(p.dims >= 2) {
+        const int y = (int)((i >> lg) & mask);
+        const int64_t ip = i + ((int64_t)(((y + 1) & mask) - y) << lg);
+        const double g = ((double)rho[ip] - r) * p.inv_dx;
+        inter += p.k * g * g;
+    }
+    if(p.dims == 3) {
+        const int z = (int)(i >> (2 * lg));
+        const int64_t ip = i + ((int64_t)(((z + 1) & mask) - z) << (2 * lg));
+        const double g = ((double)rho[ip] - r) * p.inv_dx;
+        inter += p.k * g * g;
+    }
+    f += bulk<MODEL>(p, r) + inter;
+}
+
 template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, Params p, double *__restrict__ partials) {
     const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
-    const int N = p.N, mask = p.mask, lg = p.log2N;
     double m = 0, f = 0;
-    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256) {
-        const double r = (double)rho[i];
-        m += r;
-        // forward-difference gradient (CahnHilliard.cpp:256-276)
-        const int x = (int)(i & mask);
-        double inter = 0.;
-        {
-            const int64_t ip = (i - x) + ((x + 1) & mask);
-            const double g = ((double)rho[ip] - r) * p.inv_dx;
-            inter += p.k * g * g;
-        }
-        if(p.dims >= 2) {
-            const int y = (int)((i >> lg) & mask);
-            const int64_t ip = i + ((int64_t)(((y + 1) & mask) - y) << lg);
-            const double g = ((double)rho[ip] - r) * p.inv_dx;
-            inter += p.k * g * g;
-        }
-        if(p.dims == 3) {
-            const int z = (int)(i >> (2 * lg));
-            const int64_t ip = i + ((int64_t)(((z + 1) & mask) - z) << (2 * lg));
-            const double g = ((double)rho[ip] - r) * p.inv_dx;
-            inter += p.k * g * g;
-        }
-        (void)N;
-        f += bulk<MODEL>(p, r) + inter;
-    }
+    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256)
+        obs_cell<T, MODEL>(rho, p, i, m, f);
     __shared__ double sm[4], sf[4];
     m = wave_sum(m);
     f = wave_sum(f);
@@ -92,6 +96,40 @@ hipError_t observables(const Params &p, int precision, int model, const void *rh
         else hipLaunchKernelGGL((k_observables<float, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, result);
+    return hipGetLastError();
+}
+
+// one block per replica of a batch
+template <typename T, int MODEL>
+__global__ __launch_bounds__(256) void k_batch_obs(const T *__restrict__ buf, const BatchRep *__restrict__ reps,
+                                                   double *__restrict__ result) {
+    const Params p = reps[blockIdx.x].p;
+    const T *rho = buf + reps[blockIdx.x].off;
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    double m = 0, f = 0;
+    for(int64_t i = threadIdx.x; i < bins; i += 256) obs_cell<T, MODEL>(rho, p, i, m, f);
+    __shared__ double sm[4], sf[4];
+    m = wave_sum(m);
+    f = wave_sum(f);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if(lane == 0) { sm[w] = m; sf[w] = f; }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        result[2 * blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+        result[2 * blockIdx.x + 1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+    }
+}
+
+hipError_t batch_observables(int precision, int model, const void *buf, const BatchRep *reps, int n, double *result,
+                             hipStream_t s) {
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_batch_obs<double, CH_LANDAU>), dim3(n), dim3(256), 0, s, (const double *)buf, reps, result);
+        else hipLaunchKernelGGL((k_batch_obs<double, CH_SIMPLE_WERTHEIM>), dim3(n), dim3(256), 0, s, (const double *)buf, reps, result);
+    }
+    else {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_batch_obs<float, CH_LANDAU>), dim3(n), dim3(256), 0, s, (const float *)buf, reps, result);
+        else hipLaunchKernelGGL((k_batch_obs<float, CH_SIMPLE_WERTHEIM>), dim3(n), dim3(256), 0, s, (const float *)buf, reps, result);
+    }
     return hipGetLastError();
 }
 

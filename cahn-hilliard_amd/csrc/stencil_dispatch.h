@@ -28,15 +28,26 @@ hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t
     }
 }
 
-// 2D grids with N >= 64 use the register/DPP wave kernel (4 waves per block,
-// p.wave_rpw rows per wave); K in {1, 2, 4, 8} steps per launch.
-template <typename T, int MODEL, int MOB, bool EM, int RPW>
+// 2D grids with N >= 64 use the register/DPP wave kernel.  Region shapes
+// (p.wave_cfg): 0 = 1 column x 16 rows x 4 waves (64 x 64), 1 = 2 x 8 x 4
+// (128 x 32), 2 = 2 x 8 x 8 (128 x 64); K in {1, 2, 4, 8} steps per launch
+// where the region leaves an interior.
+using WaveCfg0 = WaveCfg<1, 16, 4>;
+using WaveCfg1 = WaveCfg<2, 8, 4>;
+using WaveCfg2 = WaveCfg<2, 8, 8>;
+
+template <class CFG, int K>
+constexpr bool wave_ok() { return CFG::RW - 4 * K > 0 && CFG::RH - 4 * K > 0; }
+
+template <typename T, int MODEL, int MOB, bool EM, class CFG>
 hipError_t wave_k(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     switch(K) {
-    case 1: return launch_wave_cfg<T, MODEL, MOB, EM, 1, RPW, 4>(p, in, out, s);
-    case 2: return launch_wave_cfg<T, MODEL, MOB, EM, 2, RPW, 4>(p, in, out, s);
-    case 4: return launch_wave_cfg<T, MODEL, MOB, EM, 4, RPW, 4>(p, in, out, s);
-    case 8: return launch_wave_cfg<T, MODEL, MOB, EM, 8, RPW, 4>(p, in, out, s);
+    case 1: return launch_wave_cfg<T, MODEL, MOB, EM, 1, CFG>(p, in, out, s);
+    case 2: return launch_wave_cfg<T, MODEL, MOB, EM, 2, CFG>(p, in, out, s);
+    case 4: return launch_wave_cfg<T, MODEL, MOB, EM, 4, CFG>(p, in, out, s);
+    case 8:
+        if constexpr(wave_ok<CFG, 8>()) return launch_wave_cfg<T, MODEL, MOB, EM, 8, CFG>(p, in, out, s);
+        return hipErrorInvalidValue;
     }
     return hipErrorInvalidValue;
 }
@@ -45,8 +56,9 @@ template <typename T, int DIMS, int MODEL, int MOB, bool EM>
 hipError_t stencil_k(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     if constexpr(DIMS == 2) {
         if(p.N >= 64) {
-            if(p.wave_rpw == 32) return wave_k<T, MODEL, MOB, EM, 32>(p, K, in, out, s);
-            return wave_k<T, MODEL, MOB, EM, 16>(p, K, in, out, s);
+            if(p.wave_cfg == 1 && p.N >= 128) return wave_k<T, MODEL, MOB, EM, WaveCfg1>(p, K, in, out, s);
+            if(p.wave_cfg == 2 && p.N >= 128) return wave_k<T, MODEL, MOB, EM, WaveCfg2>(p, K, in, out, s);
+            return wave_k<T, MODEL, MOB, EM, WaveCfg0>(p, K, in, out, s);
         }
     }
     if(K != 1) return hipErrorInvalidValue;

@@ -1,136 +1,224 @@
-// stencil_wave.h -- register/DPP 2D stencil kernel (Euler, EulerMobility).
+// stencil_wave.h -- register/DPP 2D stencil kernels (Euler, EulerMobility).
 //
-// A block is NW waves.  Lane l of every wave owns grid column x0 + l (a wave
-// spans 64 consecutive x), and wave w owns RPW consecutive rows of the block's
-// region, held in registers.  Per step:
-//   x neighbours  DPP wave shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1)
+// A block is NW waves.  Lane l of every wave owns C adjacent grid columns
+// x0 + C l + c (a wave spans 64 C consecutive x), and wave w owns RPW
+// consecutive rows of the block's region, held in registers.  Per step:
+//   x neighbours  the lane's own registers, and for its first / last column
+//                 the neighbouring lane's last / first column through DPP wave
+//                 shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1): 2 DPP moves
+//                 per row per field and phase for C cells
 //   y neighbours  the lane's own registers; the first / last row of each wave
-//                 goes through LDS (2 rows per field per phase)
-// K steps run per launch on a region 64 x (NW RPW) that carries a 2K halo on
-// every side; each phase invalidates one more ring, so after K steps the
-// (64 - 4K) x (NW RPW - 4K) interior is exact and is stored.  Tiles that do not
-// divide N are clipped at the store.  All per-cell arithmetic is the shared
-// code of ch_device.h, so results are bit-identical to k_stencil's.
+//                 goes through LDS
+// Halo mode: K steps run per launch on a region 64C x (NW RPW) that carries a
+// 2K halo on every side; each phase invalidates one more ring, so after K
+// steps the (64C - 4K) x (NW RPW - 4K) interior is exact and is stored.  Tiles
+// that do not divide N are clipped at the store.
+// Whole mode (N == 64 C == NW RPW): the region is the entire periodic grid; x
+// neighbours use DPP wave rotates and y neighbours wrap through LDS, so no
+// halo is recomputed and every cell is stored.
+// All per-cell arithmetic is the shared code of ch_device.h, so results are
+// bit-identical to k_stencil's whatever the mode, K, C or tile shape.
 #pragma once
 
 #include "ch_device.h"
 
 namespace chd {
 
-// value held by lane - 1 (the x - 1 neighbour); lane 0 reads 0
+// value held by lane - 1; lane 0 reads 0 (ROT: lane 63's value)
+template <bool ROT>
 __device__ __forceinline__ double from_left(double v) {
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false);
+    constexpr int ctl = ROT ? 0x13C : 0x138;  // wave_ror:1 / wave_shr:1
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctl, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
-// value held by lane + 1 (the x + 1 neighbour); lane 63 reads 0
+// value held by lane + 1; lane 63 reads 0 (ROT: lane 0's value)
+template <bool ROT>
 __device__ __forceinline__ double from_right(double v) {
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, false);
+    constexpr int ctl = ROT ? 0x134 : 0x130;  // wave_rol:1 / wave_shl:1
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctl, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
+template <bool ROT>
 __device__ __forceinline__ float from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    constexpr int ctl = ROT ? 0x13C : 0x138;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctl, 0xF, 0xF, true));
 }
+template <bool ROT>
 __device__ __forceinline__ float from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+    constexpr int ctl = ROT ? 0x134 : 0x130;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctl, 0xF, 0xF, true));
 }
 
-template <typename T, int MODEL, int MOB, bool EM, int K, int RPW, int NW>
-__global__ __launch_bounds__(64 * NW) void k_stencil2d_wave(const T *__restrict__ in, T *__restrict__ out, Params p,
-                                                          int tiles_x) {
-    constexpr int W = 64 - 4 * K;        // interior columns per tile
-    constexpr int RH = RPW * NW;         // region rows
-    constexpr int TH = RH - 4 * K;       // interior rows per tile
-    constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
-    constexpr int NF = HASMOB ? 3 : 2;   // exchanged fields: psi, mu (, M)
-    static_assert(W > 0 && TH > 0, "tile too small for K");
-    __shared__ T ex_first[NF][NW][64];   // first row of each wave
-    __shared__ T ex_last[NF][NW][64];    // last row of each wave
+// region shape: C columns per lane, RPW rows per wave, NW waves
+template <int C_, int RPW_, int NW_>
+struct WaveCfg {
+    static constexpr int C = C_, RPW = RPW_, NW = NW_;
+    static constexpr int RW = 64 * C, RH = RPW * NW;
+};
 
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % tiles_x, ty = tile / tiles_x;
-    const int gx0 = tx * W - 2 * K, gy0 = ty * TH - 2 * K;
-    const int N = p.N, mask = p.mask;
-    const int col = (gx0 + lane) & mask;
+template <typename T, int NF, int NW, int C>
+struct WaveExchange {
+    T first[NF][NW][C][64];  // first row of each wave
+    T last[NF][NW][C][64];   // last row of each wave
+};
 
-    T psi[RPW], mu[RPW], mob[HASMOB ? RPW : 1];
+template <int MOB, bool EM>
+constexpr int exchange_fields() { return (EM && MOB != CH_MOB_CONSTANT) ? 3 : 2; }
+
+// the 4 neighbours of every cell of row j of field v (C columns per lane)
+template <typename T, int C, int RPW, bool ROT>
+__device__ __forceinline__ void row_neighbours(const T (&v)[C][RPW], int j, const T (&above)[C], const T (&below)[C],
+                                               T (&m)[C][2], T (&q)[C][2]) {
+    const T xl = from_left<ROT>(v[C - 1][j]);
+    const T xr = from_right<ROT>(v[0][j]);
 #pragma unroll
-    for(int j = 0; j < RPW; j++) psi[j] = in[(size_t)((gy0 + w * RPW + j) & mask) * N + col];
+    for(int c = 0; c < C; c++) {
+        m[c][0] = c == 0 ? xl : v[c - 1][j];
+        q[c][0] = c == C - 1 ? xr : v[c + 1][j];
+        m[c][1] = j == 0 ? above[c] : v[c][j - 1];
+        q[c][1] = j == RPW - 1 ? below[c] : v[c][j + 1];
+    }
+}
+
+// K steps on one region whose top-left grid cell is (gx0, gy0) (halo mode) or
+// which is the whole grid (WHOLE).
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE>
+__device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restrict__ out, const Params &p, int gx0,
+                                            int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex) {
+    constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
+    constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
+    constexpr int H = WHOLE ? 0 : 2 * K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int N = p.N, mask = p.mask;
+    const int wa = WHOLE ? (w + NW - 1) % NW : w - 1;  // wave above
+    const int wb = WHOLE ? (w + 1) % NW : w + 1;       // wave below
+    const bool has_a = WHOLE || w > 0, has_b = WHOLE || w < NW - 1;
+
+    T psi[C][RPW], mu[C][RPW], mob[HASMOB ? C : 1][HASMOB ? RPW : 1];
+#pragma unroll
+    for(int j = 0; j < RPW; j++) {
+        const size_t row = (size_t)((gy0 + w * RPW + j) & mask) * N;
+#pragma unroll
+        for(int c = 0; c < C; c++) psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
+    }
 
     const T Mc = (T)p.M;
 #pragma unroll 1
     for(int s = 0; s < K; s++) {
         // ---- phase 1: mu (and M) from psi
-        ex_first[0][w][lane] = psi[0];
-        ex_last[0][w][lane] = psi[RPW - 1];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            ex.first[0][w][c][lane] = psi[c][0];
+            ex.last[0][w][c][lane] = psi[c][RPW - 1];
+        }
         __syncthreads();
-        const T above = (w > 0) ? ex_last[0][w - 1][lane] : psi[0];
-        const T below = (w < NW - 1) ? ex_first[0][w + 1][lane] : psi[RPW - 1];
+        T above[C], below[C];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            above[c] = has_a ? ex.last[0][wa][c][lane] : psi[c][0];
+            below[c] = has_b ? ex.first[0][wb][c][lane] : psi[c][RPW - 1];
+        }
 #pragma unroll
         for(int j = 0; j < RPW; j++) {
-            const T c = psi[j];
-            const T m[2] = {from_left(c), j == 0 ? above : psi[j - 1]};
-            const T q[2] = {from_right(c), j == RPW - 1 ? below : psi[j + 1]};
-            mu[j] = chem_pot<T, 2, MODEL>(p, c, m, q);
-            if constexpr(HASMOB) mob[j] = mobility<T, MOB>(p, c);
+            T m[C][2], q[C][2];
+            row_neighbours<T, C, RPW, WHOLE>(psi, j, above, below, m, q);
+#pragma unroll
+            for(int c = 0; c < C; c++) {
+                mu[c][j] = chem_pot<T, 2, MODEL>(p, psi[c][j], m[c], q[c]);
+                if constexpr(HASMOB) mob[c][j] = mobility<T, MOB>(p, psi[c][j]);
+            }
         }
         // ---- phase 2: psi update from mu (and M)
-        ex_first[1][w][lane] = mu[0];
-        ex_last[1][w][lane] = mu[RPW - 1];
-        if constexpr(HASMOB) {
-            ex_first[2][w][lane] = mob[0];
-            ex_last[2][w][lane] = mob[RPW - 1];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            ex.first[1][w][c][lane] = mu[c][0];
+            ex.last[1][w][c][lane] = mu[c][RPW - 1];
+            if constexpr(HASMOB) {
+                ex.first[2][w][c][lane] = mob[c][0];
+                ex.last[2][w][c][lane] = mob[c][RPW - 1];
+            }
         }
         __syncthreads();
-        const T mu_above = (w > 0) ? ex_last[1][w - 1][lane] : mu[0];
-        const T mu_below = (w < NW - 1) ? ex_first[1][w + 1][lane] : mu[RPW - 1];
-        T M_above = Mc, M_below = Mc;
-        if constexpr(HASMOB) {
-            M_above = (w > 0) ? ex_last[2][w - 1][lane] : mob[0];
-            M_below = (w < NW - 1) ? ex_first[2][w + 1][lane] : mob[RPW - 1];
+        T mu_a[C], mu_b[C], M_a[C], M_b[C];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            mu_a[c] = has_a ? ex.last[1][wa][c][lane] : mu[c][0];
+            mu_b[c] = has_b ? ex.first[1][wb][c][lane] : mu[c][RPW - 1];
+            if constexpr(HASMOB) {
+                M_a[c] = has_a ? ex.last[2][wa][c][lane] : mob[c][0];
+                M_b[c] = has_b ? ex.first[2][wb][c][lane] : mob[c][RPW - 1];
+            }
         }
 #pragma unroll
         for(int j = 0; j < RPW; j++) {
-            const T c = mu[j];
-            const T m[2] = {from_left(c), j == 0 ? mu_above : mu[j - 1]};
-            const T q[2] = {from_right(c), j == RPW - 1 ? mu_below : mu[j + 1]};
-            if constexpr(!EM) {
-                psi[j] = euler_update<T, 2>(p, psi[j], c, m, q);
-            }
-            else if constexpr(!HASMOB) {
-                const T Mm[2] = {Mc, Mc}, Mq[2] = {Mc, Mc};
-                psi[j] = mobility_update<T, 2>(p, psi[j], c, m, q, Mc, Mm, Mq);
+            T m[C][2], q[C][2];
+            row_neighbours<T, C, RPW, WHOLE>(mu, j, mu_a, mu_b, m, q);
+            if constexpr(HASMOB) {
+                T Mm[C][2], Mq[C][2];
+                row_neighbours<T, C, RPW, WHOLE>(mob, j, M_a, M_b, Mm, Mq);
+#pragma unroll
+                for(int c = 0; c < C; c++)
+                    psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], mob[c][j], Mm[c], Mq[c]);
             }
             else {
-                const T Mi = mob[j];
-                const T Mm[2] = {from_left(Mi), j == 0 ? M_above : mob[j - 1]};
-                const T Mq[2] = {from_right(Mi), j == RPW - 1 ? M_below : mob[j + 1]};
-                psi[j] = mobility_update<T, 2>(p, psi[j], c, m, q, Mi, Mm, Mq);
+#pragma unroll
+                for(int c = 0; c < C; c++) {
+                    if constexpr(!EM) {
+                        psi[c][j] = euler_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c]);
+                    }
+                    else {
+                        const T Mm[2] = {Mc, Mc}, Mq[2] = {Mc, Mc};
+                        psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], Mc, Mm, Mq);
+                    }
+                }
             }
         }
     }
 
-    const int gx = gx0 + lane;
-    const bool col_ok = lane >= 2 * K && lane < 64 - 2 * K && gx < N;
 #pragma unroll
     for(int j = 0; j < RPW; j++) {
         const int r = w * RPW + j;
         const int gy = gy0 + r;
-        if(col_ok && r >= 2 * K && r < RH - 2 * K && gy < N) out[(size_t)gy * N + gx] = psi[j];
+        if(!(WHOLE || (r >= H && r < RH - H && gy < N))) continue;
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            const int lx = C * lane + c;
+            const int gx = gx0 + lx;
+            if(WHOLE || (lx >= H && lx < RW - H && gx < N)) out[(size_t)gy * N + gx] = psi[c][j];
+        }
     }
 }
 
-template <typename T, int MODEL, int MOB, bool EM, int K, int RPW, int NW>
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE>
+__global__ __launch_bounds__(64 * CFG::NW) void k_stencil2d_wave(const T *__restrict__ in, T *__restrict__ out, Params p,
+                                                               int tiles_x) {
+    constexpr int W = CFG::RW - 4 * K, TH = CFG::RH - 4 * K;
+    static_assert(WHOLE || (W > 0 && TH > 0), "tile too small for K");
+    __shared__ WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> ex;
+    if constexpr(WHOLE) {
+        wave_region<T, MODEL, MOB, EM, K, CFG, true>(in, out, p, 0, 0, ex);
+    }
+    else {
+        const int tile = xcd_tile(blockIdx.x, gridDim.x);
+        const int tx = tile % tiles_x, ty = tile / tiles_x;
+        wave_region<T, MODEL, MOB, EM, K, CFG, false>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K, ex);
+    }
+}
+
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG>
 hipError_t launch_wave_cfg(const Params &p, const void *in, void *out, hipStream_t s) {
-    constexpr int W = 64 - 4 * K, TH = RPW * NW - 4 * K;
+    if(CFG::RW == CFG::RH && p.N == CFG::RW) {
+        hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, true>), dim3(1), dim3(64 * CFG::NW), 0, s,
+                           (const T *)in, (T *)out, p, 1);
+        return hipGetLastError();
+    }
+    constexpr int W = CFG::RW - 4 * K, TH = CFG::RH - 4 * K;
     const int tiles_x = (p.N + W - 1) / W;
     const int tiles_y = (p.N + TH - 1) / TH;
-    hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, RPW, NW>), dim3(tiles_x * tiles_y), dim3(64 * NW), 0, s,
-                       (const T *)in, (T *)out, p, tiles_x);
+    hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, false>), dim3(tiles_x * tiles_y), dim3(64 * CFG::NW),
+                       0, s, (const T *)in, (T *)out, p, tiles_x);
     return hipGetLastError();
 }
 

@@ -203,6 +203,56 @@ class System:
             pass
 
 
+class Batch:
+    """Independent 2D Euler replicas (mixed N and parameters) advanced together."""
+
+    def __init__(self, ctx, descs):
+        self.ctx = ctx
+        arr = (L.ChDesc * len(descs))(*descs)
+        h = ctypes.c_void_p()
+        _check(ctx.handle, L.lib().ch_batch_create(ctx.handle, len(descs), arr, ctypes.byref(h)))
+        self.handle, self.n = h, len(descs)
+        self.bins = [d.N ** d.dims for d in descs]
+
+    def _chk(self, st):
+        _check(self.ctx.handle, st)
+
+    def upload(self, r, rho):
+        a = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        if a.size != self.bins[r]:
+            raise ValueError(f"replica {r}: expected {self.bins[r]} values, got {a.size}")
+        self._chk(L.lib().ch_batch_upload(self.handle, r, a.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self, r):
+        out = np.empty(self.bins[r], dtype=np.float64)
+        self._chk(L.lib().ch_batch_download(self.handle, r, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def evolve(self, nsteps=1):
+        self._chk(L.lib().ch_batch_evolve(self.handle, int(nsteps)))
+
+    def observables(self):
+        m, f = np.empty(self.n), np.empty(self.n)
+        self._chk(L.lib().ch_batch_observables(self.handle, m.ctypes.data_as(ctypes.c_void_p),
+                                               f.ctypes.data_as(ctypes.c_void_p)))
+        return m, f
+
+    @property
+    def steps_per_launch(self):
+        return int(L.lib().ch_batch_steps_per_launch(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().ch_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def der_bulk_free_energy(ctx, desc, rho_ptr, out_ptr, n):
     """FreeEnergyModel::der_bulk_free_energy on device buffers (ints: device addresses)."""
     _check(ctx.handle, L.lib().ch_der_bulk_free_energy(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(rho_ptr),

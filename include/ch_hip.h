@@ -121,6 +121,21 @@ int32_t ch_sys_steps_per_launch(const ch_sys *sys);
  * Returns CH_ENAN when either sum is NaN. */
 ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 
+/* ---- batches of independent replicas (no reference counterpart: the
+ * reference runs one system per process, src/main.cpp:182-200).  Every
+ * replica is a 2D euler system with N >= 64 and its own N, epsilon (or
+ * Wertheim parameters), k, dt, dx and M; all share one model and precision.
+ * One launch advances every replica by ch_batch_steps_per_launch steps. */
+ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *descs, ch_batch **out);
+void ch_batch_destroy(ch_batch *batch);
+ch_status ch_batch_upload(ch_batch *batch, int32_t replica, const double *rho_host);
+ch_status ch_batch_download(ch_batch *batch, int32_t replica, double *rho_host);
+/* nsteps x CahnHilliard::evolve for every replica (asynchronous) */
+ch_status ch_batch_evolve(ch_batch *batch, int64_t nsteps);
+/* FP64 average mass and free energy of every replica (arrays of n); synchronous */
+ch_status ch_batch_observables(ch_batch *batch, double *mass, double *free_energy);
+int32_t ch_batch_steps_per_launch(const ch_batch *batch);
+
 /* ---- reverse mode (no reference counterpart: the reference has no
  * gradients).  Euler integrator with constant mobility, both models.
  *

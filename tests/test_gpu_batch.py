@@ -1,0 +1,87 @@
+"""Batched replicas (ch_batch_*): every replica of a mixed-size batch evolves
+exactly as the same system run on its own (bit-identical fields: the batch
+kernel shares the per-cell arithmetic), FP64 replicas match the oracle to
+1e-10, FP32 replicas stay within 1e-4 (relative L-inf, 200 steps) of FP64,
+and the per-replica FP64 observables match the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(ch):
+    c = ch.Context(0)
+    yield c
+    c.close()
+
+
+def make_replicas(ch, precision, Ns, seed=0):
+    rng = np.random.default_rng(seed)
+    descs, fields = [], []
+    for N in Ns:
+        eps = rng.uniform(0.5, 1.0)
+        c = rng.uniform(-0.3, 0.3)
+        descs.append(ch.make_desc(dims=2, N=N, integrator="euler", precision=precision, dt=0.01, k=1.0,
+                                  epsilon=float(eps)))
+        fields.append(c + rng.uniform(-0.01, 0.01, N * N))
+    return descs, fields
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
+    Ns = [64, 128, 64, 256, 128, 512, 64]
+    descs, fields = make_replicas(ch, precision, Ns)
+    for tb in ("1", "2", "4"):
+        monkeypatch.setenv("CH_BATCH_TB", tb)
+        b = ch.Batch(ctx, descs)
+        assert b.steps_per_launch == int(tb)
+        for r, f in enumerate(fields):
+            b.upload(r, f)
+        b.evolve(77)
+        outs = [b.download(r) for r in range(len(Ns))]
+        b.close()
+        for r, (d, f) in enumerate(zip(descs, fields)):
+            s = ch.System(ctx, d)
+            s.upload(f)
+            s.evolve(77)
+            ref = s.download()
+            s.close()
+            assert np.array_equal(outs[r], ref), (tb, r, Ns[r])
+
+
+def test_batch_against_oracle(ch, ctx):
+    Ns = [64, 128, 256]
+    descs64, fields = make_replicas(ch, "fp64", Ns, seed=1)
+    descs32, _ = make_replicas(ch, "fp32", Ns, seed=1)
+    res = {}
+    for name, descs in (("fp64", descs64), ("fp32", descs32)):
+        b = ch.Batch(ctx, descs)
+        for r, f in enumerate(fields):
+            b.upload(r, f)
+        b.evolve(200)
+        res[name] = ([b.download(r) for r in range(3)], b.observables())
+        b.close()
+    for r, (N, d) in enumerate(zip(Ns, descs64)):
+        p = O.Params(dims=2, N=N, dt=0.01, k=1.0)
+        m = O.Model(epsilon=d.epsilon)
+        ref = O.run(O.EULER, p, m, fields[r], 200)
+        assert rel_linf(res["fp64"][0][r], ref) < 1e-10
+        err32 = rel_linf(res["fp32"][0][r], ref)
+        print(f"replica N={N} fp32 rel L-inf vs fp64 oracle = {err32:.3e}")
+        assert err32 < 1e-4
+        mass, fe = res["fp64"][1]
+        assert mass[r] == pytest.approx(O.average_mass(p, ref), rel=1e-10, abs=1e-14)
+        assert fe[r] == pytest.approx(O.average_free_energy(p, m, ref), rel=1e-10)
+
+
+def test_batch_rejects_unsupported(ch, ctx):
+    with pytest.raises(ch.CHError):
+        ch.Batch(ctx, [ch.make_desc(dims=2, N=32)])
+    with pytest.raises(ch.CHError):
+        ch.Batch(ctx, [ch.make_desc(dims=2, N=64), ch.make_desc(dims=2, N=64, precision="fp32")])
+    with pytest.raises(ch.CHError):
+        ch.Batch(ctx, [ch.make_desc(dims=3, N=64)])
