@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: Landau 2D N=1024, explicit Euler, FP64 fields, on one MI355X.
+"""Benchmark of the cahn-hilliard time-stepping path on one MI355X.
 
-A bench "step" is one print interval of the reference's run loop
-(src/main.cpp:104-129): `--inner` (default 100) Euler time steps followed by
-the FP64 mass / free-energy reductions, as Manager::run does every
-print_every steps.  The field stays resident in HBM for the whole run.
+Headline (`value`): Landau 2D N=1024, explicit Euler, FP64 fields.  A bench
+"step" is one print interval of the reference's run loop (src/main.cpp:104-129):
+`--inner` (default 100) Euler time steps followed by the FP64 mass /
+free-energy reductions, as Manager::run does every print_every steps.  The
+field stays resident in HBM for the whole run.
 
 value = cells * inner * steps / wall seconds (Gcell-updates/s), timed with a
 device synchronisation on both sides.  The JSON line also carries
   roofline      the dominant kernel's algorithmic HBM rate (16 B per
                 cell-step: read psi + write psi, FP64) over its average launch
                 duration, timed with HIP events on the library's own stream;
-  cpu_baseline  the oracle's single-threaded FP64 Euler (the reference's
-                loop order, built -O3 -ffast-math) on a bounded sample of the
-                same workload on this host's cores;
-  extra         the other hot-path configurations at N=1024 (FP32 Euler,
-                pseudospectral FP64/FP32).
+  cpu_baseline  the oracle's single-threaded FP64 Euler (the reference's loop
+                order, built -O3 -ffast-math) on a bounded sample of the same
+                workload on this host;
+  extra         the other configurations of the path: FP32 Euler, the
+                pseudospectral integrator (FP64 / FP32) at N=1024, the batch of
+                256 mixed-size FP32 replicas, the forward + adjoint / forward
+                cost at N=256, and SimpleWertheim 3D N=128 euler_mobility.
+`--workload NAME` runs one of them alone (for profiling).
 """
 import argparse
 import json
@@ -40,19 +44,24 @@ def load_package():
     return mod
 
 
-def timed_run(ch, ctx, sysm, steps, warmup, inner):
+def sync(ctx):
     import torch
-    for _ in range(warmup):
-        sysm.evolve(inner)
-        sysm.observables()
     ctx.synchronize()
     torch.cuda.synchronize()
+
+
+def timed_run(ctx, sysm, steps, warmup, inner, observe=True):
+    for _ in range(warmup):
+        sysm.evolve(inner)
+        if observe:
+            sysm.observables()
+    sync(ctx)
     t0 = time.perf_counter()
     for _ in range(steps):
         sysm.evolve(inner)
-        sysm.observables()
-    ctx.synchronize()
-    torch.cuda.synchronize()
+        if observe:
+            sysm.observables()
+    sync(ctx)
     return time.perf_counter() - t0
 
 
@@ -61,20 +70,19 @@ def kernel_time_per_launch(ctx, sysm, nsteps, steps_per_launch):
     import torch
     ext = torch.cuda.ExternalStream(ctx.stream)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sysm.evolve(nsteps)  # warm graphs
     ctx.synchronize()
     e0.record(ext)
     sysm.evolve(nsteps)
     e1.record(ext)
     e1.synchronize()
-    launches = nsteps / steps_per_launch
-    return e0.elapsed_time(e1) * 1e-3 / launches
+    return e0.elapsed_time(e1) * 1e-3 / (nsteps / steps_per_launch)
 
 
 def cpu_baseline(N, budget_s=12.0):
     """Oracle FP64 Euler (liboracle_fast.so, 1 thread) on the N x N field."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    import numpy as np
     p = O.Params(dims=2, N=N, dt=0.01, dx=1.0, k=1.0)
     m = O.Model(epsilon=0.9)
     rho = O.init_density(2, N, 0.0, 0.01, 0, 123456)
@@ -85,9 +93,131 @@ def cpu_baseline(N, budget_s=12.0):
     t0 = time.perf_counter()
     O.run(O.EULER, p, m, rho, steps, fast=True)
     dt = time.perf_counter() - t0
-    return {"value": N * N * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+    return {"value": round(N * N * steps / dt / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{steps} FP64 Euler steps of the Landau 2D N={N} field (oracle, -O3 -ffast-math, "
                       f"1 thread), {dt:.1f} s"}
+
+
+# ------------------------------------------------------------------ workloads
+def w_euler(ch, ctx, args, precision="fp64"):
+    from cahn_hilliard_amd.init import initial_density
+    N = args.N
+    rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
+    desc = ch.make_desc(dims=2, N=N, integrator="euler", precision=precision, dt=0.01, dx=1.0, k=1.0, epsilon=0.9)
+    s = ch.System(ctx, desc)
+    s.upload(rho0)
+    steps = args.steps if precision == "fp64" else max(10, args.steps // 4)
+    wall = timed_run(ctx, s, steps, args.warmup, args.inner)
+    spl = s.steps_per_launch
+    t_launch = kernel_time_per_launch(ctx, s, max(1024, 10 * args.inner), spl)
+    mass, fe = s.observables()
+    s.close()
+    return dict(wall=wall, steps=steps, gcell=N * N * args.inner * steps / wall / 1e9, spl=spl, t_launch=t_launch,
+                mass=mass, fe=fe)
+
+
+def w_pseudospectral(ch, ctx, args, precision="fp64"):
+    from cahn_hilliard_amd.init import initial_density
+    N = args.N
+    rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
+    d = ch.make_desc(dims=2, N=N, integrator="pseudospectral", precision=precision, dt=0.1, dx=1.0, k=1.0,
+                     epsilon=0.9)
+    s = ch.System(ctx, d)
+    s.upload(rho0)
+    nst, inner = max(10, args.steps // 4), 20
+    w = timed_run(ctx, s, nst, 2, inner)
+    s.close()
+    return {"Gcell-updates/s": round(N * N * inner * nst / w / 1e9, 3),
+            "us_per_time_step": round(w / (nst * inner) * 1e6, 3)}
+
+
+def c3_replicas(ch):
+    """256 Landau 2D replicas, 64 each at N = 64, 128, 256, 512: per replica r,
+    drand48 seeded 123456 + r draws eps ~ U[0.5, 1], then c ~ U[-0.3, 0.3], then
+    the cells psi0 = c + U[-0.01, 0.01] (x fastest)."""
+    from cahn_hilliard_amd.init import drand48_stream
+    descs, fields = [], []
+    for i, N in enumerate([64] * 64 + [128] * 64 + [256] * 64 + [512] * 64):
+        r = drand48_stream(123456 + i, 2 + N * N)
+        eps = 0.5 + 0.5 * r[0]
+        c = -0.3 + 0.6 * r[1]
+        descs.append(ch.make_desc(dims=2, N=N, integrator="euler", precision="fp32", dt=0.01, dx=1.0, k=1.0,
+                                  epsilon=float(eps)))
+        fields.append(c + 0.02 * (r[2:] - 0.5))
+    return descs, fields
+
+
+def w_batch(ch, ctx, args):
+    descs, fields = c3_replicas(ch)
+    b = ch.Batch(ctx, descs)
+    for i, f in enumerate(fields):
+        b.upload(i, f)
+    cells = sum(d.N * d.N for d in descs)
+    b.evolve(64)
+    sync(ctx)
+    nst = 1000 if args.steps >= 50 else 200
+    t0 = time.perf_counter()
+    b.evolve(nst)
+    sync(ctx)
+    w = time.perf_counter() - t0
+    b.observables()
+    spl = b.steps_per_launch
+    b.close()
+    return {"Gcell-updates/s": round(cells * nst / w / 1e9, 3), "replicas": len(descs), "cells": cells,
+            "time_steps": nst, "steps_per_launch": spl, "us_per_time_step": round(w / nst * 1e6, 3)}
+
+
+def w_adjoint(ch, ctx, args):
+    """C4 shape: Landau 2D N=256, dt=0.01, eps=0.9, k=1.2; T steps forward, then
+    record (checkpoint every 100) + reverse sweep.  The loss here is
+    1/2 |psi_T|^2 (the adjoint's cost does not depend on the loss)."""
+    from cahn_hilliard_amd.init import initial_density
+    N, T = 256, 5000 if args.steps >= 50 else 1000
+    rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
+    d = ch.make_desc(dims=2, N=N, integrator="euler", precision="fp64", dt=0.01, dx=1.0, k=1.2, epsilon=0.9)
+    s = ch.System(ctx, d)
+    s.upload(rho0)
+    s.evolve(T)
+    sync(ctx)
+    t0 = time.perf_counter()
+    s.evolve(T)
+    sync(ctx)
+    t_fwd = time.perf_counter() - t0
+    out = {}
+    for ckpt in (100, 0):
+        s.upload(rho0)
+        sync(ctx)
+        t0 = time.perf_counter()
+        s.record(T, ckpt)
+        psiT = s.download()
+        s.adjoint(psiT)
+        t_fa = time.perf_counter() - t0
+        out[f"ckpt_{ckpt}"] = round(t_fa / t_fwd, 3)
+    s.close()
+    return {"fwd_plus_adjoint_over_fwd": out, "N": N, "time_steps": T, "fwd_ms": round(t_fwd * 1e3, 3)}
+
+
+def w_wertheim(ch, ctx, args):
+    """C5 shape: SimpleWertheim 3D N=128 euler_mobility, valence 4, B2 = 1,
+    delta = 1e4, k = 1, dt = 1e-4, M = rho / 0.05, FP64; the 8 replicas run one
+    after the other (seeds 123456..123463)."""
+    from cahn_hilliard_amd.init import initial_density
+    N, nst = 128, 200 if args.steps >= 50 else 50
+    tot = 0.0
+    for r in range(8):
+        rho0 = initial_density(3, N, 0.05, 0.05, 0, 123456 + r)
+        d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
+                         valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
+        s = ch.System(ctx, d)
+        s.upload(rho0)
+        s.evolve(4)
+        sync(ctx)
+        t0 = time.perf_counter()
+        s.evolve(nst)
+        sync(ctx)
+        tot += time.perf_counter() - t0
+        s.close()
+    return {"Gcell-updates/s": round(8 * N ** 3 * nst / tot / 1e9, 3), "replicas": 8, "time_steps": nst}
 
 
 def main():
@@ -99,6 +229,9 @@ def main():
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--workload", default=None,
+                    choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
+                             "wertheim"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -107,28 +240,24 @@ def main():
 
     import torch  # device plumbing only (events on the library stream)
     ch = load_package()
-    from cahn_hilliard_amd.init import initial_density
-
-    N = args.N
-    cells = N * N
     ctx = ch.Context(0)
-    rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
-    desc = ch.make_desc(dims=2, N=N, integrator="euler", precision="fp64", dt=0.01, dx=1.0, k=1.0, epsilon=0.9)
-    s = ch.System(ctx, desc)
-    s.upload(rho0)
-    wall = timed_run(ch, ctx, s, args.steps, args.warmup, args.inner)
-    value = cells * args.inner * args.steps / wall / 1e9
-    spl = s.steps_per_launch
-    t_launch = kernel_time_per_launch(ctx, s, max(1000, 10 * args.inner), spl)
-    bytes_per_launch = 16.0 * cells * spl
-    achieved = bytes_per_launch / t_launch / 1e9
-    mass, fe = s.observables()
-    s.close()
+    table = {"euler_fp32": lambda: w_euler(ch, ctx, args, "fp32"),
+             "pseudospectral": lambda: w_pseudospectral(ch, ctx, args, "fp64"),
+             "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
+             "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
+             "wertheim": lambda: w_wertheim(ch, ctx, args)}
+    if args.workload and args.workload != "euler":
+        r = table[args.workload]()
+        print(json.dumps({"workload": args.workload, "result": {k: v for k, v in r.items() if k != "wall"}}), flush=True)
+        return
 
+    r = w_euler(ch, ctx, args)
+    N = args.N
+    achieved = 16.0 * N * N * r["spl"] / r["t_launch"] / 1e9
     out = {
         "metric": "Gcell-updates/s & %HBM roofline, Landau 2D N=1024; fwd+adjoint/fwd cost",
-        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+        "value": round(r["gcell"], 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["wall"] / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"Landau 2D N={N} periodic, explicit Euler, eps=0.9 k=1 dt=0.01 dx=1, "
                                f"psi0 = U[-0.01,0.01] (drand48 seed 123456); one step = {args.inner} time steps "
@@ -136,23 +265,18 @@ def main():
                    "N": N, "dims": 2, "integrator": "euler", "time_steps_per_step": args.inner},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_stencil (fused Euler step)", "time_steps_per_launch": spl,
-                     "avg_launch_us": round(t_launch * 1e6, 3)},
-        "parity": {"mass": mass, "free_energy": fe},
+                     "kernel": "k_stencil2d_wave (fused Euler steps)", "time_steps_per_launch": r["spl"],
+                     "avg_launch_us": round(r["t_launch"] * 1e6, 3)},
+        "parity": {"mass": r["mass"], "free_energy": r["fe"]},
     }
     if not args.no_extra:
         extra = {}
-        for name, integ, prec, dt, inner in [("euler_fp32", "euler", "fp32", 0.01, args.inner),
-                                            ("pseudospectral_fp64", "pseudospectral", "fp64", 0.1, 20),
-                                            ("pseudospectral_fp32", "pseudospectral", "fp32", 0.1, 20)]:
-            d = ch.make_desc(dims=2, N=N, integrator=integ, precision=prec, dt=dt, dx=1.0, k=1.0, epsilon=0.9)
-            sx = ch.System(ctx, d)
-            sx.upload(rho0)
-            nst = max(10, args.steps // 4)
-            w = timed_run(ch, ctx, sx, nst, 2, inner)
-            extra[name] = {"Gcell-updates/s": round(cells * inner * nst / w / 1e9, 3),
-                           "us_per_time_step": round(w / (nst * inner) * 1e6, 3)}
-            sx.close()
+        e32 = w_euler(ch, ctx, args, "fp32")
+        extra["euler_fp32"] = {"Gcell-updates/s": round(e32["gcell"], 3),
+                               "us_per_time_step": round(e32["wall"] / (e32["steps"] * args.inner) * 1e6, 3)}
+        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim"):
+            extra[name] = table[name]()
+        extra["batch"]["vs_fp64_single"] = round(extra["batch"]["Gcell-updates/s"] / r["gcell"], 3)
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N)

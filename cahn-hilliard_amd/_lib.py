@@ -87,6 +87,9 @@ def lib():
         "ch_sys_evolve": (ctypes.c_int, [P, I64]),
         "ch_sys_steps_per_launch": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
+        "ch_sq_nshells": (I32, [I32]),
+        "ch_sys_structure_factor": (ctypes.c_int, [P, P, I32]),
+        "ch_sys_sq_loss": (ctypes.c_int, [P, P, I32, DD, P]),
         "ch_batch_create": (ctypes.c_int, [P, I32, DP, PP]),
         "ch_batch_destroy": (None, [P]),
         "ch_batch_upload": (ctypes.c_int, [P, I32, P]),

@@ -444,6 +444,105 @@ extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_en
     return CH_OK;
 }
 
+// ------------------------------------------------------------------ S(q)
+extern "C" int32_t ch_sq_nshells(int32_t N) { return N < 2 ? 0 : (int32_t)std::lrint(std::sqrt(2.0) * (N / 2)) + 1; }
+
+// spectral scratch + twiddles for systems created without them (stencil paths)
+static ch_status ensure_spectral(ch_sys *s) {
+    ch_ctx *c = s->ctx;
+    const int N = s->d.N;
+    if(!s->work && hipMalloc(&s->work, s->hat * 2 * s->elt) != hipSuccess) {
+        s->work = nullptr;
+        return fail(c, CH_ENOMEM, "no device memory for the spectrum");
+    }
+    if(!s->twiddles) {
+        if(hipMalloc(&s->twiddles, (size_t)N * 2 * s->elt) != hipSuccess) {
+            s->twiddles = nullptr;
+            return fail(c, CH_ENOMEM, "no device memory for the twiddles");
+        }
+        std::vector<double> twd(2 * N);
+        std::vector<float> twf(2 * N);
+        for(int m = 0; m < N; m++) {
+            long double cc, sn;
+            twiddle(m, N, cc, sn);
+            twd[2 * m] = (double)cc; twd[2 * m + 1] = (double)sn;
+            twf[2 * m] = (float)twd[2 * m]; twf[2 * m + 1] = (float)twd[2 * m + 1];
+        }
+        const void *src = s->d.precision == CH_FP64 ? (const void *)twd.data() : (const void *)twf.data();
+        HIPCHK(c, hipMemcpy(s->twiddles, src, (size_t)N * 2 * s->elt, hipMemcpyHostToDevice));
+    }
+    return CH_OK;
+}
+
+// S(q) of the current field into sq_dev[0..ns) (and shell counts into cnt_dev)
+static ch_status sq_compute(ch_sys *s, int ns, double *sq_dev, double *cnt_dev) {
+    ch_ctx *c = s->ctx;
+    chd::SpectralArgs a = spectral_args(s);
+    a.rho = s->psi[s->cur];
+    HIPCHK(c, chd::fft2_forward(a, c->stream));
+    HIPCHK(c, chd::sq_shells(s->d.precision, s->work, s->d.N, ns, sq_dev, cnt_dev, c->stream));
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_structure_factor(ch_sys *s, double *Sq, int32_t nshells) {
+    if(!s || !Sq) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->d.dims != 2) return fail(c, CH_EUNSUPPORTED, "the structure factor is defined for 2D fields");
+    const int ns = ch_sq_nshells(s->d.N);
+    if(nshells != ns) return fail(c, CH_EINVAL, "N = %d has %d shells (got %d)", s->d.N, ns, nshells);
+    HIPCHK(c, hipSetDevice(c->device));
+    ch_status st = ensure_spectral(s);
+    if(st != CH_OK) return st;
+    double *buf = nullptr;
+    HIPCHK(c, hipMalloc((void **)&buf, 2 * ns * sizeof(double)));
+    st = sq_compute(s, ns, buf, buf + ns);
+    if(st == CH_OK) {
+        hipError_t e = hipMemcpyAsync(Sq, buf, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if(e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if(e != hipSuccess) st = fail(c, CH_EHIP, "structure factor copy failed: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(buf);
+    return st;
+}
+
+extern "C" ch_status ch_sys_sq_loss(ch_sys *s, const double *target, int32_t nshells, double *loss, double *grad) {
+    if(!s || !target) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->d.dims != 2) return fail(c, CH_EUNSUPPORTED, "the structure factor is defined for 2D fields");
+    const int ns = ch_sq_nshells(s->d.N);
+    if(nshells != ns) return fail(c, CH_EINVAL, "N = %d has %d shells (got %d)", s->d.N, ns, nshells);
+    HIPCHK(c, hipSetDevice(c->device));
+    ch_status st = ensure_spectral(s);
+    if(st != CH_OK) return st;
+    double *buf = nullptr;
+    HIPCHK(c, hipMalloc((void **)&buf, 3 * ns * sizeof(double)));
+    double *sq = buf, *cnt = buf + ns, *tg = buf + 2 * ns;
+    std::vector<double> h(ns);
+    hipError_t e = hipMemcpyAsync(tg, target, ns * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if(e == hipSuccess) {
+        st = sq_compute(s, ns, sq, cnt);
+        if(st == CH_OK) e = hipMemcpyAsync(h.data(), sq, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if(st == CH_OK && e == hipSuccess && grad) {
+            e = chd::sq_scale(s->d.precision, s->work, s->d.N, sq, tg, cnt, c->stream);
+            chd::SpectralArgs a = spectral_args(s);
+            if(e == hipSuccess) e = chd::fft2_inverse(a, s->stage2, c->stream);
+            if(e == hipSuccess && s->d.precision == CH_FP32) e = chd::convert_f2d((const float *)s->stage2, (double *)s->stage, s->bins, c->stream);
+            if(e == hipSuccess)
+                e = hipMemcpyAsync(grad, s->d.precision == CH_FP64 ? s->stage2 : s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream);
+        }
+        if(e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(buf);
+    if(st != CH_OK) return st;
+    if(e != hipSuccess) return fail(c, CH_EHIP, "structure-factor loss failed: %s", hipGetErrorString(e));
+    if(loss) {
+        double L = 0;
+        for(int i = 0; i < ns; i++) L += (h[i] - target[i]) * (h[i] - target[i]);
+        *loss = L;
+    }
+    return CH_OK;
+}
+
 // ------------------------------------------------------------------ batches
 static constexpr int kDefaultBatchTB = 4;
 

@@ -24,6 +24,13 @@ struct SpectralArgs {
 };
 // init = true: rho_hat = r2c(rho); otherwise one time step
 hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init);
+// 2D transforms on a.work: forward r2c of a.rho, inverse (c2r) into out
+hipError_t fft2_forward(const SpectralArgs &a, hipStream_t s);
+hipError_t fft2_inverse(const SpectralArgs &a, void *out, hipStream_t s);
+// structure factor shells of a.work and the loss-gradient scaling
+hipError_t sq_shells(int precision, const void *W, int N, int nshells, double *sq, double *cnt, hipStream_t s);
+hipError_t sq_scale(int precision, void *W, int N, const double *sq, const double *target, const double *cnt,
+                    hipStream_t s);
 
 // FP64 observables: partials -> result[0] = sum rho, result[1] = sum f-density
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,

@@ -409,4 +409,125 @@ hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init) {
     return ps_run_p<float>(a, s, init);
 }
 
+// ---- 2D inverse transform (column IFFT + c2r) of a.work into `out`
+template <typename T, int LOG2N>
+static hipError_t fft2_inverse_t(const SpectralArgs &a, void *out, hipStream_t s) {
+    using T2 = typename Complex<T>::type;
+    constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
+    using GX = FftGeom<LX, PackedLines<LX>::value>;
+    constexpr int NLC = ColLines<LOG2N>::value;
+    using GC = FftGeom<LOG2N, NLC>;
+    const T2 *tw2 = (const T2 *)a.twiddles;
+    T2 *W = (T2 *)a.work;
+    hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3((Nh + NLC - 1) / NLC), dim3(GC::NT), 0, s, W, W, W, tw2, a.p, Nh,
+                       (long)N * Nh, (long)Nh, 1);
+    hipLaunchKernelGGL((k_xinv<T, LX>), dim3((N + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, (T *)out, tw2, a.p, N);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t fft2_inverse_p(const SpectralArgs &a, void *out, hipStream_t s) {
+    switch(a.p.log2N) {
+    case 2: return fft2_inverse_t<T, 2>(a, out, s);
+    case 3: return fft2_inverse_t<T, 3>(a, out, s);
+    case 4: return fft2_inverse_t<T, 4>(a, out, s);
+    case 5: return fft2_inverse_t<T, 5>(a, out, s);
+    case 6: return fft2_inverse_t<T, 6>(a, out, s);
+    case 7: return fft2_inverse_t<T, 7>(a, out, s);
+    case 8: return fft2_inverse_t<T, 8>(a, out, s);
+    case 9: return fft2_inverse_t<T, 9>(a, out, s);
+    case 10: return fft2_inverse_t<T, 10>(a, out, s);
+    case 11: return fft2_inverse_t<T, 11>(a, out, s);
+    case 12: return fft2_inverse_t<T, 12>(a, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t fft2_forward(const SpectralArgs &a, hipStream_t s) {
+    SpectralArgs b = a;
+    b.rho_hat = a.work;  // the forward r2c of ps_run's set-up, written into work
+    return ps_run(b, s, true);
+}
+
+hipError_t fft2_inverse(const SpectralArgs &a, void *out, hipStream_t s) {
+    if(a.precision == CH_FP64) return fft2_inverse_p<double>(a, out, s);
+    return fft2_inverse_p<float>(a, out, s);
+}
+
+// ---- circularly averaged structure factor (utils/sq_from_traj.py:72-103)
+// Shell of the mode (x-half index kx, row y): rint(sqrt(kx^2 + iy^2)) with iy the
+// signed frequency of y; every half-spectrum entry with 0 < kx < N/2 stands for
+// two modes of the full spectrum.  One block per shell visits the candidate
+// modes of its annulus in a fixed order (deterministic sums).
+__device__ __forceinline__ int sq_shell(int kx, int iy) { return (int)rint(sqrt((double)(kx * kx + iy * iy))); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sq_shells(const typename Complex<T>::type *__restrict__ W, int N,
+                                                   double *__restrict__ sq, double *__restrict__ cnt) {
+    const int s = blockIdx.x, Nh = N / 2 + 1;
+    const int r = min(s + 1, N / 2);
+    const int nkx = r + 1, niy = 2 * r + 1;  // kx in [0, r], iy in [-r, r]
+    double acc = 0, c = 0;
+    const double inv_n = 1.0 / ((double)N * N);
+    for(int q = threadIdx.x; q < nkx * niy; q += 256) {
+        const int kx = q / niy, iy = q % niy - r;
+        if(iy >= N / 2 || iy < -N / 2 || sq_shell(kx, iy) != s) continue;
+        const int y = iy < 0 ? iy + N : iy;
+        const auto f = W[(size_t)y * Nh + kx];
+        const double w = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+        const double a2 = (kx == 0 && iy == 0) ? 0.0 : ((double)f.x * f.x + (double)f.y * f.y);  // mean removed
+        acc += w * a2 * inv_n;
+        c += w;
+    }
+    __shared__ double sa[256], sc[256];
+    sa[threadIdx.x] = acc;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for(int h = 128; h > 0; h >>= 1) {
+        if(threadIdx.x < h) {
+            sa[threadIdx.x] += sa[threadIdx.x + h];
+            sc[threadIdx.x] += sc[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) {
+        sq[s] = sc[0] > 0 ? sa[0] / sc[0] : 0.0;
+        cnt[s] = sc[0];
+    }
+}
+
+// W_k <- 2 G_shell W_k with G_s = 2 (S_s - T_s) / (count_s N^2), DC set to 0:
+// the c2r of the result is dLoss/dpsi for Loss = sum_s (S_s - T_s)^2
+template <typename T>
+__global__ void k_sq_scale(typename Complex<T>::type *__restrict__ W, int N, const double *__restrict__ sq,
+                           const double *__restrict__ target, const double *__restrict__ cnt) {
+    const int Nh = N / 2 + 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= (int64_t)N * Nh) return;
+    const int y = (int)(i / Nh), kx = (int)(i % Nh);
+    const int iy = y < N / 2 ? y : y - N;
+    const int s = sq_shell(kx, iy);
+    double g = 0.0;
+    if(!(kx == 0 && iy == 0)) g = 2.0 * 2.0 * (sq[s] - target[s]) / (cnt[s] * (double)N * N);
+    auto f = W[i];
+    f.x = (T)(g * (double)f.x);
+    f.y = (T)(g * (double)f.y);
+    W[i] = f;
+}
+
+hipError_t sq_shells(int precision, const void *W, int N, int nshells, double *sq, double *cnt, hipStream_t s) {
+    if(precision == CH_FP64) hipLaunchKernelGGL((k_sq_shells<double>), dim3(nshells), dim3(256), 0, s, (const double2 *)W, N, sq, cnt);
+    else hipLaunchKernelGGL((k_sq_shells<float>), dim3(nshells), dim3(256), 0, s, (const float2 *)W, N, sq, cnt);
+    return hipGetLastError();
+}
+
+hipError_t sq_scale(int precision, void *W, int N, const double *sq, const double *target, const double *cnt,
+                    hipStream_t s) {
+    const int64_t n = (int64_t)N * (N / 2 + 1);
+    const int g = (int)((n + 255) / 256);
+    if(precision == CH_FP64) hipLaunchKernelGGL((k_sq_scale<double>), dim3(g), dim3(256), 0, s, (double2 *)W, N, sq, target, cnt);
+    else hipLaunchKernelGGL((k_sq_scale<float>), dim3(g), dim3(256), 0, s, (float2 *)W, N, sq, target, cnt);
+    return hipGetLastError();
+}
+
 }  // namespace chd

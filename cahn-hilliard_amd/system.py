@@ -172,6 +172,22 @@ class System:
     def average_free_energy(self):
         return self.observables()[1]
 
+    def structure_factor(self):
+        """Circularly averaged S(q) of the current 2D field (utils/sq_from_traj.py:72-103)."""
+        ns = L.lib().ch_sq_nshells(self.desc.N)
+        out = np.empty(ns)
+        self._chk(L.lib().ch_sys_structure_factor(self.handle, out.ctypes.data_as(ctypes.c_void_p), ns))
+        return out
+
+    def sq_loss(self, target, grad=True):
+        """sum_s (S_s - target_s)^2 and (optionally) its gradient with respect to the field."""
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        loss = ctypes.c_double()
+        g = np.empty(self.bins) if grad else None
+        self._chk(L.lib().ch_sys_sq_loss(self.handle, t.ctypes.data_as(ctypes.c_void_p), t.size, ctypes.byref(loss),
+                                         g.ctypes.data_as(ctypes.c_void_p) if grad else None))
+        return (loss.value, g) if grad else loss.value
+
     def record(self, nsteps, ckpt_every=100):
         """Advance nsteps keeping a checkpoint every ckpt_every steps (0: every state)."""
         self._chk(L.lib().ch_sys_record(self.handle, int(nsteps), int(ckpt_every)))

@@ -121,6 +121,16 @@ int32_t ch_sys_steps_per_launch(const ch_sys *sys);
  * Returns CH_ENAN when either sum is NaN. */
 ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 
+/* ---- structure factor (2D): the circular average of utils/sq_from_traj.py:72-103
+ * (mean removed, |FFT|^2 / N^2, shell = rint(|k| / dq)), computed with the
+ * library's own FFT.  nshells must equal ch_sq_nshells(N) = rint(N sqrt(2) / 2) + 1. */
+int32_t ch_sq_nshells(int32_t N);
+ch_status ch_sys_structure_factor(ch_sys *sys, double *Sq, int32_t nshells);
+/* loss = sum_s (S_s - target_s)^2 of the current field and, when grad is not
+ * NULL, dloss/dpsi (N^2 doubles): the seed of ch_sys_adjoint for the
+ * structure-factor loss.  Synchronous. */
+ch_status ch_sys_sq_loss(ch_sys *sys, const double *target, int32_t nshells, double *loss, double *grad);
+
 /* ---- batches of independent replicas (no reference counterpart: the
  * reference runs one system per process, src/main.cpp:182-200).  Every
  * replica is a 2D euler system with N >= 64 and its own N, epsilon (or

@@ -135,3 +135,57 @@ def test_autograd_gradcheck(ch):
     out = fn(psi0, eps, k, M)
     (out ** 2).sum().backward()
     assert psi0.grad is not None and torch.isfinite(psi0.grad).all()
+
+
+@pytest.mark.parametrize("integrator,precision,tol", [("euler", "fp64", 1e-10), ("pseudospectral", "fp64", 1e-10),
+                                                      ("euler", "fp32", 2e-4)])
+def test_structure_factor_and_loss_against_oracle(ch, ctx, integrator, precision, tol):
+    """Device S(q) and its loss gradient (utils/sq_from_traj.py:72-103) against the
+    oracle's restatement, which is itself pinned to numpy.fft."""
+    N = 64
+    psi = O.init_density(2, N, 0.0, 0.3, 0, 11)
+    s = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator=integrator, precision=precision, dt=0.01))
+    s.upload(psi)
+    sq = s.structure_factor()
+    ref = O.radial_sq(psi.reshape(N, N))
+    assert len(sq) == len(ref) == 46
+    assert np.abs(sq - ref).max() <= tol * ref.max()
+    target = O.radial_sq(O.init_density(2, N, 0.0, 0.3, 0, 12).reshape(N, N))
+    loss, g = s.sq_loss(target)
+    ref_loss, ref_g = O.sq_loss(psi.reshape(N, N), target)
+    s.close()
+    assert loss == pytest.approx(ref_loss, rel=10 * tol)
+    assert np.abs(g - ref_g.ravel()).max() <= 10 * tol * np.abs(ref_g).max()
+
+
+def test_c4_loss_gradient_chain(ch, ctx):
+    """C4 shape at small size: S(q) loss of the final field, its gradient seeded into
+    the reverse sweep, gradients wrt eps / k checked by central differences."""
+    N, T = 32, 100
+    kw = dict(dims=2, N=N, dt=0.01, dx=1.0, M=1.0)
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 123456)
+    tgt_sys = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.8, k=1.0, **kw))
+    tgt_sys.upload(psi0)
+    tgt_sys.evolve(T)
+    target = tgt_sys.structure_factor()
+    tgt_sys.close()
+
+    def loss_at(eps, k):
+        s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=eps, k=k, **kw))
+        s.upload(psi0)
+        s.evolve(T)
+        v = s.sq_loss(target, grad=False)
+        s.close()
+        return v
+
+    s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.9, k=1.2, **kw))
+    s.upload(psi0)
+    s.record(T, 25)
+    loss, g = s.sq_loss(target)
+    lam, grads = s.adjoint(g)
+    s.close()
+    h = 1e-6
+    fd_eps = (loss_at(0.9 + h, 1.2) - loss_at(0.9 - h, 1.2)) / (2 * h)
+    fd_k = (loss_at(0.9, 1.2 + h) - loss_at(0.9, 1.2 - h)) / (2 * h)
+    assert grads[0] == pytest.approx(fd_eps, rel=1e-5)
+    assert grads[1] == pytest.approx(fd_k, rel=1e-5)
