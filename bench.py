@@ -200,24 +200,38 @@ def w_adjoint(ch, ctx, args):
 def w_wertheim(ch, ctx, args):
     """C5 shape: SimpleWertheim 3D N=128 euler_mobility, valence 4, B2 = 1,
     delta = 1e4, k = 1, dt = 1e-4, M = rho / 0.05, FP64; the 8 replicas run one
-    after the other (seeds 123456..123463)."""
+    after the other (seeds 123456..123463).  Then, per replica, dF(psi_T)/d delta:
+    record (checkpoint every 50 steps), the free-energy gradient as the seed, the
+    reverse sweep, plus the explicit dF/d delta."""
     from cahn_hilliard_amd.init import initial_density
     N, nst = 128, 200 if args.steps >= 50 else 50
-    tot = 0.0
+    d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
+                     valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
+    tot = tot_grad = 0.0
+    dFdd = []
     for r in range(8):
         rho0 = initial_density(3, N, 0.05, 0.05, 0, 123456 + r)
-        d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
-                         valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
         s = ch.System(ctx, d)
         s.upload(rho0)
         s.evolve(4)
+        s.upload(rho0)
         sync(ctx)
         t0 = time.perf_counter()
         s.evolve(nst)
         sync(ctx)
         tot += time.perf_counter() - t0
+        s.upload(rho0)
+        sync(ctx)
+        t0 = time.perf_counter()
+        s.record(nst, 50)
+        _, g, dp = s.free_energy_grad()
+        _, grads = s.adjoint(g)
+        tot_grad += time.perf_counter() - t0
+        dFdd.append(float(dp[ch.GRAD_DELTA] + grads[ch.GRAD_DELTA]))
         s.close()
-    return {"Gcell-updates/s": round(8 * N ** 3 * nst / tot / 1e9, 3), "replicas": 8, "time_steps": nst}
+    return {"Gcell-updates/s": round(8 * N ** 3 * nst / tot / 1e9, 3), "replicas": 8, "time_steps": nst,
+            "dF_ddelta": [float(f"{v:.6e}") for v in dFdd],
+            "fwd_plus_adjoint_over_fwd": round(tot_grad / tot, 3)}
 
 
 def main():

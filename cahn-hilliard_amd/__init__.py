@@ -6,10 +6,10 @@ the tests, the benchmark and the autograd wrapper; the drop-in C++ host code
 lives in host/.
 """
 from . import _lib
-from ._lib import (CHError, EULER, EULER_MOBILITY, FP32, FP64, LANDAU, PSEUDOSPECTRAL, SIMPLE_WERTHEIM,
-                   header_functions)
+from ._lib import (CHError, EULER, EULER_MOBILITY, FP32, FP64, GRAD_B2, GRAD_DELTA, GRAD_EPSILON, GRAD_K, GRAD_M,
+                   LANDAU, NGRADS, PSEUDOSPECTRAL, SIMPLE_WERTHEIM, header_functions)
 from .system import Batch, Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility
 
 __all__ = ["CHError", "Batch", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
            "set_mobility", "header_functions", "EULER", "EULER_MOBILITY", "PSEUDOSPECTRAL", "LANDAU",
-           "SIMPLE_WERTHEIM", "FP32", "FP64"]
+           "SIMPLE_WERTHEIM", "FP32", "FP64", "GRAD_EPSILON", "GRAD_K", "GRAD_M", "GRAD_DELTA", "GRAD_B2", "NGRADS"]

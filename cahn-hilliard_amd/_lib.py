@@ -16,6 +16,8 @@ FP64, FP32 = 0, 1
 EULER, EULER_MOBILITY, PSEUDOSPECTRAL = 0, 1, 2
 LANDAU, SIMPLE_WERTHEIM = 0, 1
 MOB_CONSTANT, MOB_FREE_ENERGY, MOB_REGULARISED, MOB_LINEAR = 0, 1, 2, 3
+# parameter-gradient slots of ch_sys_adjoint / ch_sys_free_energy_grad
+GRAD_EPSILON, GRAD_K, GRAD_M, GRAD_DELTA, GRAD_B2, NGRADS = 0, 1, 2, 3, 4, 5
 
 INTEGRATORS = {"euler": EULER, "euler_mobility": EULER_MOBILITY, "pseudospectral": PSEUDOSPECTRAL}
 MODELS = {"landau": LANDAU, "simple_wertheim": SIMPLE_WERTHEIM}
@@ -100,6 +102,7 @@ def lib():
         "ch_sys_record": (ctypes.c_int, [P, I64, I32]),
         "ch_sys_adjoint": (ctypes.c_int, [P, P, P, P]),
         "ch_sys_adjoint_device": (ctypes.c_int, [P, P, P, P]),
+        "ch_sys_free_energy_grad": (ctypes.c_int, [P, DD, P, P]),
         "ch_der_bulk_free_energy": (ctypes.c_int, [P, DP, P, P, I64]),
         "ch_set_mobility": (ctypes.c_int, [P, DP, P, P, I64]),
     }

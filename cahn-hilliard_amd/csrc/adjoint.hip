@@ -1,18 +1,33 @@
-// adjoint.hip -- reverse-mode sweep of the Euler integrator (constant mobility).
+// adjoint.hip -- reverse-mode sweep of the euler and euler_mobility integrators.
 //
-// Forward step (src/integrators/EulerCPU.cpp:18-36):
+// euler (src/integrators/EulerCPU.cpp:18-36), constant mobility:
 //     psi_{n+1} = psi_n + dt M L(mu_n),  mu_n = f'(psi_n) - 2k L psi_n
 // with L the periodic (2d+1)-point Laplacian (symmetric).  Its transpose gives
 //     lambda_n = lambda_{n+1} + dt M ( f''(psi_n) w - 2k L w ),  w = L lambda_{n+1}
-// and the parameter gradients of the loss, summed over steps,
-//     dL/deps = -dt M <w, psi_n>,  dL/dk = -2 dt M <w, L psi_n>,  dL/dM = dt <w, mu_n>
-// (Landau: d mu/d eps = -psi).  One launch runs KA backward steps on a tile
-// carrying a 2 KA halo of lambda; the states psi_n come from the tape of the
-// current checkpoint window.  Each block keeps FP64 partial sums of the three
-// inner products for its own tile (no atomics: fixed tile -> slot mapping, so
-// the result is deterministic).  This reverse sweep has no reference
-// counterpart; it is pinned by the oracle's FP64 adjoint and by finite
-// differences of the forward map.
+// euler_mobility (src/integrators/EulerMobilityCPU.cpp:31-83), any mobility rule:
+//     psi_{n+1} = psi_n + dt D(Mob_n, mu_n),  Mob_n = mobility(psi_n),
+//     D(a, u)_i = sum_d [A(i,i+e)(u_{i+e} - u_i) - A(i-e,i)(u_i - u_{i-e})] / dx^2,
+//     A(i,j) = (a_i + a_j) / 2 (the face mobility of the flux form).
+// <lambda, D(Mob, mu)> is bilinear in (Mob, mu) and D(a, .) is symmetric, so
+//     nu = D(Mob, lambda_{n+1}),
+//     eta_j = -sum_d [(mu_{j+e}-mu_j)(l_{j+e}-l_j) + (mu_j-mu_{j-e})(l_j-l_{j-e})] / (2 dx^2),
+//     lambda_n = lambda_{n+1} + dt ( f''(psi_n) nu - 2k L nu + Mob'(psi_n) eta ).
+// Parameter gradients of the loss, summed over steps (nu = M w for euler):
+//     dL/deps = -dt <nu, psi_n>        (Landau: d mu/d eps = -psi)
+//     dL/dk   = -2 dt <nu, L psi_n>
+//     dL/dM   = dt <eta, Mob_n / M>    (euler: dt <w, mu_n>)
+//     dL/ddelta = dt (<nu, d f'/d delta> + <eta, d Mob/d delta>)   (SimpleWertheim)
+//     dL/dB2  = 2 dt <nu, psi_n>       (SimpleWertheim: d f'/d B2 = 2 rho)
+// One euler launch runs KA backward steps on a tile carrying a 2 KA halo of
+// lambda; euler_mobility runs one step per launch.  The states psi_n come from
+// the tape of the current checkpoint window.  Each block keeps FP64 partial
+// sums of the inner products for its own tile (fixed tile -> slot mapping, no
+// atomics: deterministic).  This reverse sweep has no reference counterpart;
+// it is pinned by the oracle's FP64 adjoint and by finite differences of the
+// forward map.
+#include <algorithm>
+#include <type_traits>
+
 #include "ch_internal.h"
 #include "stencil.h"
 
@@ -35,12 +50,88 @@ __device__ __forceinline__ double fsecond(const Params &p, double r) {
     }
 }
 
+// d f'/d delta (SimpleWertheim, rho > 0): valence X_delta / X with
+// X_delta = -valence rho X^2 / (2 c rho X + 1), c = valence delta
+template <int MODEL>
+__device__ __forceinline__ double dfprime_ddelta(const Params &p, double r) {
+    if constexpr(MODEL == CH_LANDAU) {
+        return 0.0;
+    }
+    else {
+        if(!(r > 0.)) return 0.0;
+        const double c = 0.5 * p.tvd, X = wertheim_X(p, r);
+        return -(double)p.valence * p.valence * r * X / (2.0 * c * r * X + 1.0);
+    }
+}
+
+// d f/d delta (SimpleWertheim, rho > 0): valence rho X_delta (1/X - 1/2)
+template <int MODEL>
+__device__ __forceinline__ double dbulk_ddelta(const Params &p, double r) {
+    if constexpr(MODEL == CH_LANDAU) {
+        return 0.0;
+    }
+    else {
+        if(!(r > 0.)) return 0.0;
+        const double c = 0.5 * p.tvd, X = wertheim_X(p, r);
+        const double Xd = -(double)p.valence * r * X * X / (2.0 * c * r * X + 1.0);
+        return p.valence * r * Xd * (1.0 / X - 0.5);
+    }
+}
+
+// mobility(rho) / M, d mobility / d rho and d mobility / d delta (FP64)
+template <int MOB>
+struct MobDeriv {
+    double hat, drho, ddelta;
+};
+
+template <int MOB>
+__device__ __forceinline__ MobDeriv<MOB> mob_deriv(const Params &p, double r) {
+    MobDeriv<MOB> m{1.0, 0.0, 0.0};
+    if constexpr(MOB == CH_MOB_FREE_ENERGY) {
+        const double c = 0.5 * p.tvd, X = wertheim_X(p, r), v = p.valence;
+        const double den = 2.0 * c * r * X + 1.0;
+        const double Xv1 = pow(X, v - 1.0);
+        m.hat = r * Xv1 * X;
+        m.drho = p.M * (Xv1 * X + r * v * Xv1 * (-c * X * X / den));
+        m.ddelta = p.M * r * v * Xv1 * (-v * r * X * X / den);
+    }
+    else if constexpr(MOB == CH_MOB_REGULARISED) {
+        m.hat = r / (r + p.rho_min);
+        m.drho = p.M * p.rho_min / ((r + p.rho_min) * (r + p.rho_min));
+    }
+    else if constexpr(MOB == CH_MOB_LINEAR) {
+        m.hat = r / p.rho_ref;
+        m.drho = p.M / p.rho_ref;
+    }
+    return m;
+}
+
 __device__ __forceinline__ double wave_sum64(double v) {
 #pragma unroll
     for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
+// adds the block's sums of the kGradSlots per-thread values to its own slots
+template <int NT>
+__device__ __forceinline__ void block_accumulate(const double (&g)[kGradSlots], double (&red)[kGradSlots][NT / 64],
+                                                 double *__restrict__ slots) {
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for(int i = 0; i < kGradSlots; i++) {
+        const double v = wave_sum64(g[i]);
+        if((threadIdx.x & 63) == 0) red[i][w] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < kGradSlots) {
+        double a = 0;
+        for(int i = 0; i < NT / 64; i++) a += red[threadIdx.x][i];
+        slots[threadIdx.x] += a;
+    }
+}
+
+// slots: <nu, psi>, <nu, L psi>, the M term, the delta term (nu = w for euler;
+// the host applies the factors dt, M)
 template <typename T, int DIMS, int MODEL, int TX, int TY, int TZ, int KA, int NT>
 __global__ __launch_bounds__(NT) void k_adjoint(const T *__restrict__ lam_in, T *__restrict__ lam_out,
                                                 const T *__restrict__ tape, int s_top, Params p, int tiles_x,
@@ -51,7 +142,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(const T *__restrict__ lam_in, T 
     __shared__ T Lm[R];   // lambda
     __shared__ T Wb[R];   // w = L lambda
     __shared__ T P[R];    // psi_n
-    __shared__ double red[3][NT / 64];
+    __shared__ double red[kGradSlots][NT / 64];
 
     const int tile = blockIdx.x;
     const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
@@ -69,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(const T *__restrict__ lam_in, T 
     for(int c = threadIdx.x; c < R; c += NT) Lm[c] = lam_in[gidx(c)];
 
     const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
-    double ge = 0, gk = 0, gm = 0;
+    double ge = 0, gk = 0, gm = 0, gd = 0;
 #pragma unroll 1
     for(int j = 0; j < KA; j++) {
         const T *psi = tape + (size_t)(s_top - j) * bins;
@@ -102,6 +193,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(const T *__restrict__ lam_in, T 
                 ge += w * ps;
                 gk += w * lps;
                 gm += w * mu;
+                if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += w * dfprime_ddelta<MODEL>(p, ps);
             }
         }
         {   // lambda += dt M (f''(psi) w - 2k L w) on [2j+2, R-2j-2)
@@ -130,24 +222,165 @@ __global__ __launch_bounds__(NT) void k_adjoint(const T *__restrict__ lam_in, T 
         lam_out[((size_t)gz * N + gy) * N + gx] = Lm[li];
     }
 
-    ge = wave_sum64(ge);
-    gk = wave_sum64(gk);
-    gm = wave_sum64(gm);
-    const int w = threadIdx.x >> 6;
-    if((threadIdx.x & 63) == 0) { red[0][w] = ge; red[1][w] = gk; red[2][w] = gm; }
-    __syncthreads();
-    if(threadIdx.x == 0) {
-        double a = 0, b = 0, c = 0;
-        for(int i = 0; i < NT / 64; i++) { a += red[0][i]; b += red[1][i]; c += red[2][i]; }
-        gacc[3 * tile] += a;
-        gacc[3 * tile + 1] += b;
-        gacc[3 * tile + 2] += c;
+    const double g[kGradSlots] = {ge, gk, gm, gd};
+    block_accumulate<NT>(g, red, gacc + kGradSlots * tile);
+}
+
+// one euler_mobility backward step: lambda_in = lambda_{n+1} -> lambda_out =
+// lambda_n, psi = psi_n; the tile carries a 2-cell halo
+template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_adjoint_em(const T *__restrict__ lam_in, T *__restrict__ lam_out,
+                                                   const T *__restrict__ psi, Params p, int tiles_x, int tiles_y,
+                                                   double *__restrict__ gacc) {
+    using G = TileGeom<DIMS, TX, TY, TZ, 1>;
+    constexpr int RX = G::RX, RY = G::RY, RZ = G::RZ, RXY = G::RXY, R = G::R, H = G::H;
+    constexpr int off[3] = {1, RX, RXY};
+    constexpr bool VARM = MOB != CH_MOB_CONSTANT;
+    __shared__ T Lm[R];            // lambda_{n+1}
+    __shared__ T P[R];             // psi_n
+    __shared__ T Mu[R];            // mu_n
+    __shared__ T Nu[R];            // nu = D(Mob, lambda)
+    __shared__ T Mb[VARM ? R : 1];  // Mob_n
+    __shared__ double red[kGradSlots][NT / 64];
+
+    const int tile = blockIdx.x;
+    const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
+    const int x0 = tx * TX - H, y0 = ty * TY - H, z0 = tz * TZ - H;
+    const int N = p.N, mask = p.mask;
+    for(int c = threadIdx.x; c < R; c += NT) {
+        const int lx = c % RX, ly = (c / RX) % RY, lz = c / RXY;
+        const int gx = (x0 + lx) & mask;
+        const int gy = DIMS >= 2 ? (y0 + ly) & mask : 0;
+        const int gz = DIMS == 3 ? (z0 + lz) & mask : 0;
+        const size_t g = ((size_t)gz * N + gy) * N + gx;
+        Lm[c] = lam_in[g];
+        const T ps = psi[g];
+        P[c] = ps;
+        if constexpr(VARM) Mb[c] = mobility<T, MOB>(p, ps);
     }
+    __syncthreads();
+    const T inv_dx2 = (T)p.inv_dx2, half = (T)0.5, Mc = (T)p.M;
+    auto cell = [&](int lo, int c) {
+        const int EX = RX - 2 * lo, EY = DIMS >= 2 ? RY - 2 * lo : 1;
+        const int ex = c % EX, ey = (c / EX) % EY, ez = c / (EX * EY);
+        return (lo + ex) + (DIMS >= 2 ? (lo + ey) * RX : 0) + (DIMS == 3 ? (lo + ez) * RXY : 0);
+    };
+    auto box = [&](int lo) {
+        return (RX - 2 * lo) * (DIMS >= 2 ? RY - 2 * lo : 1) * (DIMS == 3 ? RZ - 2 * lo : 1);
+    };
+    for(int c = threadIdx.x; c < box(1); c += NT) {  // mu and nu on [1, R-1)
+        const int li = cell(1, c);
+        T m[DIMS], q[DIMS];
+#pragma unroll
+        for(int d = 0; d < DIMS; d++) { m[d] = P[li - off[d]]; q[d] = P[li + off[d]]; }
+        Mu[li] = chem_pot<T, DIMS, MODEL>(p, P[li], m, q);
+        const T l = Lm[li];
+        const T Mi = VARM ? Mb[li] : Mc;
+        T s = 0;
+#pragma unroll
+        for(int d = 0; d < DIMS; d++) {
+            const T Mm = VARM ? Mb[li - off[d]] : Mc, Mq = VARM ? Mb[li + off[d]] : Mc;
+            s = s + half * (Mi + Mq) * (Lm[li + off[d]] - l) - half * (Mm + Mi) * (l - Lm[li - off[d]]);
+        }
+        Nu[li] = s * inv_dx2;
+    }
+    __syncthreads();
+    double ge = 0, gk = 0, gm = 0, gd = 0;
+    for(int c = threadIdx.x; c < box(2); c += NT) {  // the tile interior [2, R-2)
+        const int li = cell(2, c);
+        T nm[DIMS], nq[DIMS], pm[DIMS], pq[DIMS];
+#pragma unroll
+        for(int d = 0; d < DIMS; d++) {
+            nm[d] = Nu[li - off[d]]; nq[d] = Nu[li + off[d]];
+            pm[d] = P[li - off[d]]; pq[d] = P[li + off[d]];
+        }
+        const T nu = Nu[li], l = Lm[li], mu = Mu[li];
+        T se = 0;
+#pragma unroll
+        for(int d = 0; d < DIMS; d++)
+            se = se + (Mu[li + off[d]] - mu) * (Lm[li + off[d]] - l) + (mu - Mu[li - off[d]]) * (l - Lm[li - off[d]]);
+        const T eta = -half * se * inv_dx2;
+        const T lnu = laplacian<T, DIMS>(nu, nm, nq, inv_dx2);
+        const double ps = (double)P[li];
+        const MobDeriv<MOB> md = mob_deriv<MOB>(p, ps);
+        const T upd = (T)fsecond<MODEL>(p, ps) * nu - (T)p.two_k * lnu + (T)md.drho * eta;
+        const int lx = li % RX, ly = (li / RX) % RY, lz = li / RXY;
+        const int gx = tx * TX + lx - H, gy = DIMS >= 2 ? ty * TY + ly - H : 0, gz = DIMS == 3 ? tz * TZ + lz - H : 0;
+        lam_out[((size_t)gz * N + gy) * N + gx] = l + (T)p.dt * upd;
+        const double lps = (double)laplacian<T, DIMS>(P[li], pm, pq, inv_dx2);
+        ge += (double)nu * ps;
+        gk += (double)nu * lps;
+        gm += (double)eta * md.hat;
+        if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += (double)nu * dfprime_ddelta<MODEL>(p, ps) + (double)eta * md.ddelta;
+    }
+    const double g[kGradSlots] = {ge, gk, gm, gd};
+    block_accumulate<NT>(g, red, gacc + kGradSlots * tile);
 }
 
 __global__ void k_fill_zero(double *__restrict__ a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i < n) a[i] = 0.0;
+}
+
+// F = V_bin / bins sum_i [f(rho_i) + k sum_d ((rho_{i+e} - rho_i) / dx)^2]
+// (src/CahnHilliard.cpp:300-319): dF/drho_i = V_bin / bins (f'(rho_i) - 2k L rho_i),
+// plus the sums behind the explicit parameter derivatives.  FP64 throughout.
+template <typename T, int MODEL>
+__global__ __launch_bounds__(256) void k_fe_grad(const T *__restrict__ rho, Params p, double *__restrict__ grad,
+                                                 double *__restrict__ partials) {
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    const int mask = p.mask, lg = p.log2N;
+    double acc[kFeSlots] = {0, 0, 0, 0, 0};
+    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256) {
+        const double r = (double)rho[i];
+        double nsum = 0, g2 = 0;
+        for(int d = 0; d < p.dims; d++) {
+            const int sh = d * lg;
+            const int c = (int)((i >> sh) & mask);
+            const int64_t base = i - ((int64_t)c << sh);
+            const double rm = (double)rho[base + ((int64_t)((c - 1) & mask) << sh)];
+            const double rp = (double)rho[base + ((int64_t)((c + 1) & mask) << sh)];
+            nsum += rm + rp;
+            const double g = (rp - r) * p.inv_dx;
+            g2 += g * g;
+        }
+        const double lap = (nsum - 2.0 * p.dims * r) * p.inv_dx2;
+        grad[i] = p.vbin_over_bins * ((double)fprime<double, MODEL>(p, r) - p.two_k * lap);
+        acc[0] += bulk<MODEL>(p, r);
+        acc[1] += g2;
+        acc[2] += -0.5 * r * r;
+        acc[3] += dbulk_ddelta<MODEL>(p, r);
+        acc[4] += r * r;
+    }
+    __shared__ double red[kFeSlots][4];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for(int j = 0; j < kFeSlots; j++) {
+        const double v = wave_sum64(acc[j]);
+        if((threadIdx.x & 63) == 0) red[j][w] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < kFeSlots)
+        partials[kFeSlots * blockIdx.x + threadIdx.x] =
+            (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
+}
+
+hipError_t free_energy_grad(const Params &p, int precision, int model, const void *rho, double *grad,
+                            double *partials, int max_parts, int *nb, hipStream_t s) {
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    const int n = (int)std::min<int64_t>(max_parts, (bins + 255) / 256);
+    *nb = n;
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        if(model == CH_LANDAU)
+            hipLaunchKernelGGL((k_fe_grad<T, CH_LANDAU>), dim3(n), dim3(256), 0, s, (const T *)rho, p, grad, partials);
+        else
+            hipLaunchKernelGGL((k_fe_grad<T, CH_SIMPLE_WERTHEIM>), dim3(n), dim3(256), 0, s, (const T *)rho, p, grad,
+                               partials);
+    };
+    if(precision == CH_FP64) go(double{});
+    else go(float{});
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- host side
@@ -183,77 +416,136 @@ static hipError_t adj_tiles(const Params &p, int KA, const void *li, void *lo, c
     }
 }
 
-int adjoint_steps_per_launch(int dims, int N) { return (dims <= 2 && N >= 64) ? 4 : 1; }
+template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ>
+static hipError_t launch_adj_em(const Params &p, const void *lam_in, void *lam_out, const void *psi, double *gacc,
+                                hipStream_t s) {
+    constexpr int NT = 256;
+    const int tiles_x = p.N / TX, tiles_y = DIMS >= 2 ? p.N / TY : 1, tiles_z = DIMS == 3 ? p.N / TZ : 1;
+    hipLaunchKernelGGL((k_adjoint_em<T, DIMS, MODEL, MOB, TX, TY, TZ, NT>), dim3(tiles_x * tiles_y * tiles_z), dim3(NT),
+                       0, s, (const T *)lam_in, (T *)lam_out, (const T *)psi, p, tiles_x, tiles_y, gacc);
+    return hipGetLastError();
+}
+
+// euler_mobility adjoint tiles: one step per launch, LDS of five tile fields
+template <typename T, int DIMS, int MODEL, int MOB>
+static hipError_t adj_em_tiles(const Params &p, const void *li, void *lo, const void *psi, double *gacc,
+                               hipStream_t s) {
+    const int N = p.N;
+    if constexpr(DIMS == 1) {
+        if(N >= 64) return launch_adj_em<T, 1, MODEL, MOB, 64, 1, 1>(p, li, lo, psi, gacc, s);
+        if(N >= 16) return launch_adj_em<T, 1, MODEL, MOB, 16, 1, 1>(p, li, lo, psi, gacc, s);
+        return launch_adj_em<T, 1, MODEL, MOB, 4, 1, 1>(p, li, lo, psi, gacc, s);
+    }
+    else if constexpr(DIMS == 2) {
+        if(N >= 64) return launch_adj_em<T, 2, MODEL, MOB, 64, 16, 1>(p, li, lo, psi, gacc, s);
+        if(N >= 16) return launch_adj_em<T, 2, MODEL, MOB, 16, 16, 1>(p, li, lo, psi, gacc, s);
+        return launch_adj_em<T, 2, MODEL, MOB, 4, 4, 1>(p, li, lo, psi, gacc, s);
+    }
+    else {
+        if(N >= 16) return launch_adj_em<T, 3, MODEL, MOB, 16, 4, 4>(p, li, lo, psi, gacc, s);
+        return launch_adj_em<T, 3, MODEL, MOB, 4, 4, 4>(p, li, lo, psi, gacc, s);
+    }
+}
+
+int adjoint_steps_per_launch(int dims, int N, bool em) { return (!em && dims <= 2 && N >= 64) ? 4 : 1; }
 
 // number of tiles (gradient accumulator slots) of the adjoint launches
-int adjoint_tiles(int dims, int N) {
+int adjoint_tiles(int dims, int N, bool em) {
     if(dims == 1) return N >= 64 ? N / 64 : (N >= 16 ? N / 16 : N / 4);
     if(dims == 2) return N >= 64 ? (N / 64) * (N / 16) : (N >= 16 ? (N / 16) * (N / 16) : (N / 4) * (N / 4));
+    if(em) return N >= 16 ? (N / 16) * (N / 4) * (N / 4) : (N / 4) * (N / 4) * (N / 4);
     return N >= 16 ? (N / 16) * (N / 8) * (N / 8) : (N / 4) * (N / 4) * (N / 4);
 }
 
-hipError_t adjoint_launch(const Params &p, int precision, int model, int KA, const void *lam_in, void *lam_out,
-                          const void *tape, int s_top, double *gacc, hipStream_t s) {
-    const int d = p.dims;
+// dims x model (x mobility rule) dispatch; free_energy mobility only exists for
+// SimpleWertheim (the descriptor check rejects it for Landau)
+template <typename T, typename F>
+static hipError_t by_dims_model_mob(int dims, int model, int mob, F &&f) {
+    auto with_model = [&](auto dtag, auto mtag) -> hipError_t {
+        constexpr int D = decltype(dtag)::value, MO = decltype(mtag)::value;
+        switch(mob) {
+        case CH_MOB_CONSTANT: return f(dtag, mtag, std::integral_constant<int, CH_MOB_CONSTANT>{});
+        case CH_MOB_REGULARISED: return f(dtag, mtag, std::integral_constant<int, CH_MOB_REGULARISED>{});
+        case CH_MOB_LINEAR: return f(dtag, mtag, std::integral_constant<int, CH_MOB_LINEAR>{});
+        default:
+            if constexpr(MO == CH_SIMPLE_WERTHEIM) return f(dtag, mtag, std::integral_constant<int, CH_MOB_FREE_ENERGY>{});
+            (void)D;
+            return hipErrorInvalidValue;
+        }
+    };
+    auto with_dims = [&](auto mtag) -> hipError_t {
+        if(dims == 1) return with_model(std::integral_constant<int, 1>{}, mtag);
+        if(dims == 2) return with_model(std::integral_constant<int, 2>{}, mtag);
+        return with_model(std::integral_constant<int, 3>{}, mtag);
+    };
+    if(model == CH_LANDAU) return with_dims(std::integral_constant<int, CH_LANDAU>{});
+    return with_dims(std::integral_constant<int, CH_SIMPLE_WERTHEIM>{});
+}
+
+hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bool em, int KA, const void *lam_in,
+                          void *lam_out, const void *tape, int s_top, double *gacc, hipStream_t s) {
+    const size_t bins = (size_t)1 << (p.log2N * p.dims);
     auto go = [&](auto tag) -> hipError_t {
         using T = decltype(tag);
-        if(model == CH_LANDAU) {
-            if(d == 1) return adj_tiles<T, 1, CH_LANDAU>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
-            if(d == 2) return adj_tiles<T, 2, CH_LANDAU>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
-            return adj_tiles<T, 3, CH_LANDAU>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
+        if(em) {
+            const T *psi = (const T *)tape + (size_t)s_top * bins;
+            return by_dims_model_mob<T>(p.dims, model, mob, [&](auto dt, auto mt, auto bt) {
+                return adj_em_tiles<T, decltype(dt)::value, decltype(mt)::value, decltype(bt)::value>(
+                    p, lam_in, lam_out, psi, gacc, s);
+            });
         }
-        if(d == 1) return adj_tiles<T, 1, CH_SIMPLE_WERTHEIM>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
-        if(d == 2) return adj_tiles<T, 2, CH_SIMPLE_WERTHEIM>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
-        return adj_tiles<T, 3, CH_SIMPLE_WERTHEIM>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
+        return by_dims_model_mob<T>(p.dims, model, CH_MOB_CONSTANT, [&](auto dt, auto mt, auto) {
+            return adj_tiles<T, decltype(dt)::value, decltype(mt)::value>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
+        });
     };
     if(precision == CH_FP64) return go(double{});
     return go(float{});
 }
 
-// recompute of a checkpoint window: K Euler steps from `in`, all K states to tape_out
-template <typename T, int DIMS, int MODEL, int TX, int TY, int TZ, int K>
+// recompute of a checkpoint window: K steps from `in`, all K states to tape_out
+template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K>
 static hipError_t launch_tape(const Params &p, const void *in, void *tape_out, hipStream_t s) {
     constexpr int NT = 256;
     const int tiles_x = p.N / TX, tiles_y = DIMS >= 2 ? p.N / TY : 1, tiles_z = DIMS == 3 ? p.N / TZ : 1;
-    hipLaunchKernelGGL((k_stencil<T, DIMS, MODEL, CH_MOB_CONSTANT, false, TX, TY, TZ, K, NT, true>),
-                       dim3(tiles_x * tiles_y * tiles_z), dim3(NT), 0, s, (const T *)in, (T *)tape_out, p, tiles_x, tiles_y);
+    hipLaunchKernelGGL((k_stencil<T, DIMS, MODEL, MOB, EM, TX, TY, TZ, K, NT, true>), dim3(tiles_x * tiles_y * tiles_z),
+                       dim3(NT), 0, s, (const T *)in, (T *)tape_out, p, tiles_x, tiles_y);
     return hipGetLastError();
 }
 
-template <typename T, int DIMS, int MODEL>
+template <typename T, int DIMS, int MODEL, int MOB, bool EM>
 static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     const int N = p.N;
     if constexpr(DIMS == 1) {
-        if(K == 4 && N >= 64) return launch_tape<T, 1, MODEL, 64, 1, 1, 4>(p, in, out, s);
-        if(N >= 64) return launch_tape<T, 1, MODEL, 64, 1, 1, 1>(p, in, out, s);
-        if(N >= 16) return launch_tape<T, 1, MODEL, 16, 1, 1, 1>(p, in, out, s);
-        return launch_tape<T, 1, MODEL, 4, 1, 1, 1>(p, in, out, s);
+        if constexpr(!EM) if(K == 4 && N >= 64) return launch_tape<T, 1, MODEL, MOB, EM, 64, 1, 1, 4>(p, in, out, s);
+        if(N >= 64) return launch_tape<T, 1, MODEL, MOB, EM, 64, 1, 1, 1>(p, in, out, s);
+        if(N >= 16) return launch_tape<T, 1, MODEL, MOB, EM, 16, 1, 1, 1>(p, in, out, s);
+        return launch_tape<T, 1, MODEL, MOB, EM, 4, 1, 1, 1>(p, in, out, s);
     }
     else if constexpr(DIMS == 2) {
-        if(K == 4 && N >= 64) return launch_tape<T, 2, MODEL, 64, 16, 1, 4>(p, in, out, s);
-        if(N >= 64) return launch_tape<T, 2, MODEL, 64, 16, 1, 1>(p, in, out, s);
-        if(N >= 16) return launch_tape<T, 2, MODEL, 16, 16, 1, 1>(p, in, out, s);
-        return launch_tape<T, 2, MODEL, 4, 4, 1, 1>(p, in, out, s);
+        if constexpr(!EM) if(K == 4 && N >= 64) return launch_tape<T, 2, MODEL, MOB, EM, 64, 16, 1, 4>(p, in, out, s);
+        if(N >= 64) return launch_tape<T, 2, MODEL, MOB, EM, 64, 16, 1, 1>(p, in, out, s);
+        if(N >= 16) return launch_tape<T, 2, MODEL, MOB, EM, 16, 16, 1, 1>(p, in, out, s);
+        return launch_tape<T, 2, MODEL, MOB, EM, 4, 4, 1, 1>(p, in, out, s);
     }
     else {
-        if(N >= 16) return launch_tape<T, 3, MODEL, 16, 8, 8, 1>(p, in, out, s);
-        return launch_tape<T, 3, MODEL, 4, 4, 4, 1>(p, in, out, s);
+        if(N >= 16) return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1>(p, in, out, s);
+        return launch_tape<T, 3, MODEL, MOB, EM, 4, 4, 4, 1>(p, in, out, s);
     }
 }
 
-hipError_t tape_launch(const Params &p, int precision, int model, int K, const void *in, void *tape_out,
-                       hipStream_t s) {
-    const int d = p.dims;
+hipError_t tape_launch(const Params &p, int precision, int model, int mob, bool em, int K, const void *in,
+                       void *tape_out, hipStream_t s) {
+    if(em && K != 1) return hipErrorInvalidValue;
     auto go = [&](auto tag) -> hipError_t {
         using T = decltype(tag);
-        if(model == CH_LANDAU) {
-            if(d == 1) return tape_tiles<T, 1, CH_LANDAU>(p, K, in, tape_out, s);
-            if(d == 2) return tape_tiles<T, 2, CH_LANDAU>(p, K, in, tape_out, s);
-            return tape_tiles<T, 3, CH_LANDAU>(p, K, in, tape_out, s);
-        }
-        if(d == 1) return tape_tiles<T, 1, CH_SIMPLE_WERTHEIM>(p, K, in, tape_out, s);
-        if(d == 2) return tape_tiles<T, 2, CH_SIMPLE_WERTHEIM>(p, K, in, tape_out, s);
-        return tape_tiles<T, 3, CH_SIMPLE_WERTHEIM>(p, K, in, tape_out, s);
+        if(em)
+            return by_dims_model_mob<T>(p.dims, model, mob, [&](auto dt, auto mt, auto bt) {
+                return tape_tiles<T, decltype(dt)::value, decltype(mt)::value, decltype(bt)::value, true>(p, K, in,
+                                                                                                         tape_out, s);
+            });
+        return by_dims_model_mob<T>(p.dims, model, CH_MOB_CONSTANT, [&](auto dt, auto mt, auto) {
+            return tape_tiles<T, decltype(dt)::value, decltype(mt)::value, CH_MOB_CONSTANT, false>(p, K, in, tape_out, s);
+        });
     };
     if(precision == CH_FP64) return go(double{});
     return go(float{});

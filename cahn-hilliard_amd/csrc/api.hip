@@ -20,7 +20,7 @@
 // default temporal-blocking depth (time steps per stencil launch) and region
 // shape of the 2D wave kernel
 static constexpr int kDefaultTB = 4;
-static constexpr int kDefaultWaveCfg = 0;
+static constexpr int kDefaultWaveCfg = 2;
 
 struct ch_ctx {
     int device = 0;
@@ -543,6 +543,37 @@ extern "C" ch_status ch_sys_sq_loss(ch_sys *s, const double *target, int32_t nsh
     return CH_OK;
 }
 
+extern "C" ch_status ch_sys_free_energy_grad(ch_sys *s, double *F, double *dF_dpsi, double *dF_dparams) {
+    if(!s) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    constexpr int NS = chd::kFeSlots;
+    const int max_parts = chd::kObsParts * 2 / NS;
+    int nb = 0;
+    HIPCHK(c, chd::free_energy_grad(s->p, s->d.precision, s->d.model, s->psi[s->cur], (double *)s->stage, s->partials,
+                                    max_parts, &nb, c->stream));
+    std::vector<double> h((size_t)NS * nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), s->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if(dF_dpsi) HIPCHK(c, hipMemcpyAsync(dF_dpsi, s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double sum[NS] = {0, 0, 0, 0, 0};
+    for(int b = 0; b < nb; b++)
+        for(int j = 0; j < NS; j++) sum[j] += h[(size_t)NS * b + j];
+    const double vb = s->p.vbin_over_bins;
+    const double Fv = vb * (sum[0] + s->d.k * sum[1]);
+    if(std::isnan(Fv)) return fail(c, CH_ENAN, "Encountered a nan while computing the free energy");
+    if(F) *F = Fv;
+    if(dF_dparams) {
+        const bool landau = s->d.model == CH_LANDAU;
+        dF_dparams[CH_GRAD_EPSILON] = landau ? vb * sum[2] : 0.0;
+        dF_dparams[CH_GRAD_K] = vb * sum[1];
+        dF_dparams[CH_GRAD_M] = 0.0;
+        dF_dparams[CH_GRAD_DELTA] = landau ? 0.0 : vb * sum[3];
+        dF_dparams[CH_GRAD_B2] = landau ? 0.0 : vb * sum[4];
+    }
+    return CH_OK;
+}
+
 // ------------------------------------------------------------------ batches
 static constexpr int kDefaultBatchTB = 4;
 
@@ -749,11 +780,12 @@ static char *field_at(void *base, int64_t i, size_t field_bytes) { return (char 
 // K steps from tape slot `from` into slots from+1 .. from+K
 static hipError_t tape_steps(ch_sys *s, int64_t from, int64_t count) {
     const size_t fb = s->bins * s->elt;
-    const int Kt = chd::adjoint_steps_per_launch(s->d.dims, s->d.N);
+    const bool em = s->d.integrator == CH_EULER_MOBILITY;
+    const int Kt = chd::adjoint_steps_per_launch(s->d.dims, s->d.N, em);
     while(count > 0) {
         const int K = count >= Kt ? Kt : 1;
-        hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, K, field_at(s->tape, from, fb),
-                                        field_at(s->tape, from + 1, fb), s->ctx->stream);
+        hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,
+                                        field_at(s->tape, from, fb), field_at(s->tape, from + 1, fb), s->ctx->stream);
         if(e != hipSuccess) return e;
         from += K;
         count -= K;
@@ -764,8 +796,8 @@ static hipError_t tape_steps(ch_sys *s, int64_t from, int64_t count) {
 extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every) {
     if(!s || nsteps < 1 || ckpt_every < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
-    if(s->d.integrator != CH_EULER || s->d.mobility != CH_MOB_CONSTANT)
-        return fail(c, CH_EUNSUPPORTED, "the adjoint covers the euler integrator with constant mobility");
+    if(s->d.integrator == CH_PSEUDOSPECTRAL)
+        return fail(c, CH_EUNSUPPORTED, "the adjoint covers the euler and euler_mobility integrators");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
     s->rec_valid = false;
@@ -779,7 +811,8 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
         // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
         HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, tape_steps(s, 0, nsteps - 1));
-        HIPCHK(c, chd::tape_launch(s->p, s->d.precision, s->d.model, 1, field_at(s->tape, nsteps - 1, fb),
+        HIPCHK(c, chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility,
+                                   s->d.integrator == CH_EULER_MOBILITY, 1, field_at(s->tape, nsteps - 1, fb),
                                    s->psi[s->cur], c->stream));
     }
     else {
@@ -798,12 +831,14 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
 static ch_status adjoint_core(ch_sys *s, double *grads) {
     ch_ctx *c = s->ctx;
     const size_t fb = s->bins * s->elt;
-    const int ntiles = chd::adjoint_tiles(s->d.dims, s->d.N);
-    const int KA = chd::adjoint_steps_per_launch(s->d.dims, s->d.N);
+    const bool em = s->d.integrator == CH_EULER_MOBILITY;
+    const int ntiles = chd::adjoint_tiles(s->d.dims, s->d.N, em);
+    const int KA = chd::adjoint_steps_per_launch(s->d.dims, s->d.N, em);
     const int64_t W = s->rec_window, T = s->rec_steps;
     const int64_t nck = (T + W - 1) / W;
+    constexpr int NS = chd::kGradSlots;
     int li = 0;  // lambda lives in lam[li]
-    HIPCHK(c, chd::fill_zero(s->gacc, 3 * ntiles, c->stream));
+    HIPCHK(c, chd::fill_zero(s->gacc, NS * ntiles, c->stream));
     for(int64_t k = nck - 1; k >= 0; k--) {
         const int64_t start = k * W, len = std::min<int64_t>(W, T - start);
         if(!s->rec_full_tape) {
@@ -812,25 +847,27 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
         }
         for(int64_t t = len - 1; t >= 0;) {
             const int ka = (t + 1 >= KA) ? KA : 1;
-            HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, ka, s->lam[li], s->lam[li ^ 1], s->tape,
-                                          (int)t, s->gacc, c->stream));
+            HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, ka, s->lam[li],
+                                          s->lam[li ^ 1], s->tape, (int)t, s->gacc, c->stream));
             li ^= 1;
             t -= ka;
         }
     }
-    std::vector<double> g(3 * (size_t)ntiles);
+    std::vector<double> g(NS * (size_t)ntiles);
     HIPCHK(c, hipMemcpyAsync(g.data(), s->gacc, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double ge = 0, gk = 0, gm = 0;
-    for(int i = 0; i < ntiles; i++) {
-        ge += g[3 * i];
-        gk += g[3 * i + 1];
-        gm += g[3 * i + 2];
-    }
+    double sum[NS] = {0, 0, 0, 0};
+    for(int i = 0; i < ntiles; i++)
+        for(int j = 0; j < NS; j++) sum[j] += g[NS * (size_t)i + j];
     if(grads) {
-        grads[0] = s->d.model == CH_LANDAU ? -s->d.dt * s->d.M * ge : 0.0;
-        grads[1] = -2.0 * s->d.dt * s->d.M * gk;
-        grads[2] = s->d.dt * gm;
+        // slots: <nu, psi>, <nu, L psi>, M term, delta term; euler keeps w = nu / M
+        const double dt = s->d.dt, f = em ? dt : dt * s->d.M;
+        const bool landau = s->d.model == CH_LANDAU;
+        grads[CH_GRAD_EPSILON] = landau ? -f * sum[0] : 0.0;
+        grads[CH_GRAD_K] = -2.0 * f * sum[1];
+        grads[CH_GRAD_M] = dt * sum[2];
+        grads[CH_GRAD_DELTA] = landau ? 0.0 : f * sum[3];
+        grads[CH_GRAD_B2] = landau ? 0.0 : 2.0 * f * sum[0];
     }
     if(li == 1) HIPCHK(c, hipMemcpyAsync(s->lam[0], s->lam[1], fb, hipMemcpyDeviceToDevice, c->stream));
     return CH_OK;
@@ -843,7 +880,9 @@ static ch_status adjoint_prepare(ch_sys *s) {
     const size_t fb = s->bins * s->elt;
     int64_t cap0 = s->lam[0] ? 1 : 0, cap1 = s->lam[1] ? 1 : 0, capg = s->gacc ? 1 : 0;
     if(!grow(&s->lam[0], cap0, 1, fb) || !grow(&s->lam[1], cap1, 1, fb) ||
-       !grow((void **)&s->gacc, capg, 1, 3 * sizeof(double) * chd::adjoint_tiles(s->d.dims, s->d.N)))
+       !grow((void **)&s->gacc, capg, 1,
+             chd::kGradSlots * sizeof(double) *
+                 chd::adjoint_tiles(s->d.dims, s->d.N, s->d.integrator == CH_EULER_MOBILITY)))
         return fail(c, CH_ENOMEM, "no device memory for the adjoint");
     return CH_OK;
 }

@@ -53,14 +53,23 @@ hipError_t batch_launch(int precision, int model, int K, const void *a, void *b,
 hipError_t batch_observables(int precision, int model, const void *buf, const BatchRep *reps, int n, double *result,
                              hipStream_t s);
 
-// adjoint (adjoint.hip)
-int adjoint_steps_per_launch(int dims, int N);
-int adjoint_tiles(int dims, int N);
-hipError_t adjoint_launch(const Params &p, int precision, int model, int KA, const void *lam_in, void *lam_out,
-                          const void *tape, int s_top, double *gacc, hipStream_t s);
-hipError_t tape_launch(const Params &p, int precision, int model, int K, const void *in, void *tape_out,
-                       hipStream_t s);
+// adjoint (adjoint.hip).  `em`: the euler_mobility integrator (any mobility
+// rule); otherwise euler with constant mobility.  Every adjoint tile owns
+// kGradSlots FP64 accumulators.
+constexpr int kGradSlots = 4;
+int adjoint_steps_per_launch(int dims, int N, bool em);
+int adjoint_tiles(int dims, int N, bool em);
+hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bool em, int KA, const void *lam_in,
+                          void *lam_out, const void *tape, int s_top, double *gacc, hipStream_t s);
+hipError_t tape_launch(const Params &p, int precision, int model, int mob, bool em, int K, const void *in,
+                       void *tape_out, hipStream_t s);
 hipError_t fill_zero(double *a, int n, hipStream_t s);
+// free-energy gradient: grad[i] = V_bin / bins * mu_i (double), and per-block
+// partial sums partials[kFeSlots * b + j] of {f, sum_d (grad_d+ rho)^2,
+// d f / d eps, d f / d delta, rho^2}; returns the number of blocks in *nb
+constexpr int kFeSlots = 5;
+hipError_t free_energy_grad(const Params &p, int precision, int model, const void *rho, double *grad,
+                            double *partials, int max_parts, int *nb, hipStream_t s);
 
 hipError_t pointwise_fprime(const Params &p, int precision, int model, const void *in, void *out, int64_t n,
                             hipStream_t s);

@@ -33,6 +33,11 @@ __device__ __forceinline__ T2 csub(T2 a, T2 b) { return T2{a.x - b.x, a.y - b.y}
 template <typename T2>
 __device__ __forceinline__ T2 conj_(T2 a) { return T2{a.x, -a.y}; }
 
+// LDS index with one pad element every 32: the Stockham writes of the first
+// passes (stride 4 elements) then spread over every bank group.
+__device__ __forceinline__ constexpr int pidx(int i) { return i + (i >> 5); }
+constexpr int padded(int n) { return n + (n >> 5) + 1; }
+
 // Block geometry of a length-2^LOG2N line FFT: E elements, NT = E / 4 threads.
 template <int LOG2N, int NLINES>
 struct FftGeom {
@@ -79,16 +84,16 @@ struct LineFFT {
             for(int i = 0; i < 2; i++) {
                 const int q = tid + i * NT;
                 const int line = q / h, j = q % h;
-                v0[i] = buf[line * n + j];
-                v1[i] = buf[line * n + j + h];
+                v0[i] = buf[pidx(line * n + j)];
+                v1[i] = buf[pidx(line * n + j + h)];
             }
             __syncthreads();
 #pragma unroll
             for(int i = 0; i < 2; i++) {
                 const int q = tid + i * NT;
                 const int line = q / h, j = q % h;
-                buf[line * n + 2 * j] = cadd(v0[i], v1[i]);
-                buf[line * n + 2 * j + 1] = csub(v0[i], v1[i]);
+                buf[pidx(line * n + 2 * j)] = cadd(v0[i], v1[i]);
+                buf[pidx(line * n + 2 * j + 1)] = csub(v0[i], v1[i]);
             }
             __syncthreads();
         }
@@ -100,7 +105,7 @@ struct LineFFT {
             const int jm = j & (Ns - 1);
             T2 v[4];
 #pragma unroll
-            for(int r = 0; r < 4; r++) v[r] = buf[line * n + j + r * qn];
+            for(int r = 0; r < 4; r++) v[r] = buf[pidx(line * n + j + r * qn)];
 #pragma unroll
             for(int r = 1; r < 4; r++) v[r] = cmul(v[r], INV ? conj_(tw[ps][r - 1]) : tw[ps][r - 1]);
             __syncthreads();
@@ -109,10 +114,10 @@ struct LineFFT {
             const T2 d = csub(v[1], v[3]);
             const T2 a3 = INV ? T2{-d.y, d.x} : T2{d.y, -d.x};  // (+-i)(v1 - v3)
             const int o = line * n + (j - jm) * 4 + jm;
-            buf[o] = cadd(a0, a2);
-            buf[o + Ns] = cadd(a1, a3);
-            buf[o + 2 * Ns] = csub(a0, a2);
-            buf[o + 3 * Ns] = csub(a1, a3);
+            buf[pidx(o)] = cadd(a0, a2);
+            buf[pidx(o + Ns)] = cadd(a1, a3);
+            buf[pidx(o + 2 * Ns)] = csub(a0, a2);
+            buf[pidx(o + 3 * Ns)] = csub(a1, a3);
             __syncthreads();
         }
     }
@@ -173,7 +178,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT;
-    __shared__ T2 buf[G::E];
+    __shared__ T2 buf[padded(G::E)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 2);
     const int row0 = blockIdx.x * nl;
@@ -188,7 +193,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
             v = in[(size_t)row * N + x];
             if constexpr(FPRIME) v = fprime<T, MODEL>(p, v);
         }
-        bufT[e] = v;
+        bufT[2 * pidx(e >> 1) + (e & 1)] = v;
     }
     __syncthreads();
     fft.template run<false>(buf);
@@ -197,8 +202,8 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
         const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
         if(row >= rows) continue;
-        const T2 zk = buf[line * n + (k == n ? 0 : k)];
-        const T2 znk = buf[line * n + (k == 0 ? 0 : n - k)];
+        const T2 zk = buf[pidx(line * n + (k == n ? 0 : k))];
+        const T2 znk = buf[pidx(line * n + (k == 0 ? 0 : n - k))];
         // E = (Z_k + conj Z_{n-k}) / 2, O = (Z_k - conj Z_{n-k}) / (2i), X_k = E + W_N^k O
         const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
         const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
@@ -216,14 +221,14 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
-    __shared__ T2 buf[G::E + nl];
+    __shared__ T2 buf[padded(G::E + nl)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 2);
     const int row0 = blockIdx.x * nl;
     for(int e = threadIdx.x; e < nl * Nh; e += NT) {
         const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
-        buf[e] = (row < rows) ? in[(size_t)row * Nh + k] : T2{0, 0};
+        buf[pidx(e)] = (row < rows) ? in[(size_t)row * Nh + k] : T2{0, 0};
     }
     __syncthreads();
     T2 z[4];
@@ -231,8 +236,8 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
     for(int i = 0; i < 4; i++) {
         const int q = threadIdx.x + i * NT;
         const int line = q / n, k = q % n;
-        T2 xk = buf[line * Nh + k];
-        T2 xnk = buf[line * Nh + n - k];
+        T2 xk = buf[pidx(line * Nh + k)];
+        T2 xnk = buf[pidx(line * Nh + n - k)];
         if(k == 0) { xk.y = 0; xnk.y = 0; }
         // E = X_k + conj X_{n-k}; O = (X_k - conj X_{n-k}) conj(W_N^k); Z = E + i O
         const T2 Ev = T2{xk.x + xnk.x, xk.y - xnk.y};
@@ -241,7 +246,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
     }
     __syncthreads();
 #pragma unroll
-    for(int i = 0; i < 4; i++) buf[threadIdx.x + i * NT] = z[i];
+    for(int i = 0; i < 4; i++) buf[pidx(threadIdx.x + i * NT)] = z[i];
     __syncthreads();
     fft.template run<true>(buf);
     const T *bufT = reinterpret_cast<const T *>(buf);
@@ -250,7 +255,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
         const int e = threadIdx.x + i * NT;
         const int line = e / N, x = e % N;
         const int row = row0 + line;
-        if(row < rows) out[(size_t)row * N + x] = bufT[e];
+        if(row < rows) out[(size_t)row * N + x] = bufT[2 * pidx(e >> 1) + (e & 1)];
     }
 }
 
@@ -265,7 +270,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, NLC>;
     constexpr int n = G::n, NT = G::NT, nl = NLC;
-    __shared__ T2 buf[G::E];
+    __shared__ T2 buf[padded(G::E)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 1);
     const int Nh = p.N / 2 + 1;
@@ -277,7 +282,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
         const int l = l0 + lb;
         T2 v{0, 0};
         if(l < lines) v = src[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride];
-        buf[lb * n + j] = v;
+        buf[pidx(lb * n + j)] = v;
     }
     __syncthreads();
     fft.template run<MODE == 1>(buf);
@@ -294,7 +299,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
                 int iy, iz;
                 if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
                 else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
-                buf[q] = spectral_update<T, T2>(p, ix, iy, iz, buf[q], rho_hat + gi, inv_bins);
+                buf[pidx(q)] = spectral_update<T, T2>(p, ix, iy, iz, buf[pidx(q)], rho_hat + gi, inv_bins);
             }
         }
         __syncthreads();
@@ -305,7 +310,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
         const int e = threadIdx.x + i * NT;
         const int j = e / nl, lb = e % nl;
         const int l = l0 + lb;
-        if(l < lines) dst[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride] = buf[lb * n + j];
+        if(l < lines) dst[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride] = buf[pidx(lb * n + j)];
     }
 }
 

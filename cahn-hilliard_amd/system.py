@@ -193,19 +193,30 @@ class System:
         self._chk(L.lib().ch_sys_record(self.handle, int(nsteps), int(ckpt_every)))
 
     def adjoint(self, dL_dpsiT):
-        """Reverse sweep of the last record: returns (dL/dpsi0, [dL/deps, dL/dk, dL/dM])."""
+        """Reverse sweep of the last record: returns (dL/dpsi0, grads) with
+        grads[GRAD_EPSILON, GRAD_K, GRAD_M, GRAD_DELTA, GRAD_B2]."""
         g = np.ascontiguousarray(dL_dpsiT, dtype=np.float64).ravel()
         out = np.empty(self.bins, dtype=np.float64)
-        grads = np.zeros(3)
+        grads = np.zeros(L.NGRADS)
         self._chk(L.lib().ch_sys_adjoint(self.handle, g.ctypes.data_as(ctypes.c_void_p),
                                          out.ctypes.data_as(ctypes.c_void_p), grads.ctypes.data_as(ctypes.c_void_p)))
         return out, grads
 
     def adjoint_device(self, dL_dpsiT_ptr, dL_dpsi0_ptr):
-        grads = np.zeros(3)
+        grads = np.zeros(L.NGRADS)
         self._chk(L.lib().ch_sys_adjoint_device(self.handle, ctypes.c_void_p(dL_dpsiT_ptr), ctypes.c_void_p(dL_dpsi0_ptr),
                                                 grads.ctypes.data_as(ctypes.c_void_p)))
         return grads
+
+    def free_energy_grad(self):
+        """(F, dF/dpsi, explicit dF/dparams) of the current field: F as
+        average_free_energy (src/CahnHilliard.cpp:300-319)."""
+        F = ctypes.c_double()
+        g = np.empty(self.bins)
+        dp = np.zeros(L.NGRADS)
+        self._chk(L.lib().ch_sys_free_energy_grad(self.handle, ctypes.byref(F), g.ctypes.data_as(ctypes.c_void_p),
+                                                  dp.ctypes.data_as(ctypes.c_void_p)))
+        return F.value, g, dp
 
     def close(self):
         if getattr(self, "handle", None):

@@ -147,19 +147,29 @@ ch_status ch_batch_observables(ch_batch *batch, double *mass, double *free_energ
 int32_t ch_batch_steps_per_launch(const ch_batch *batch);
 
 /* ---- reverse mode (no reference counterpart: the reference has no
- * gradients).  Euler integrator with constant mobility, both models.
+ * gradients).  The euler integrator (constant mobility) and the
+ * euler_mobility integrator (every mobility rule), both models.
  *
- * ch_sys_record: advance the current state nsteps times (as ch_sys_evolve)
+ * Parameter gradients come in arrays of CH_NGRADS doubles, indexed by
+ * CH_GRAD_*: epsilon (Landau), k, M (mobility.M), delta and B2
+ * (SimpleWertheim; wertheim.delta as resolved into ch_desc.delta).  Entries a
+ * model does not have are 0. */
+enum { CH_GRAD_EPSILON = 0, CH_GRAD_K = 1, CH_GRAD_M = 2, CH_GRAD_DELTA = 3, CH_GRAD_B2 = 4, CH_NGRADS = 5 };
+/* ch_sys_record: advance the current state nsteps times (as ch_sys_evolve)
  * keeping a checkpoint every `ckpt_every` steps; ckpt_every = 0 keeps every
  * state (no recomputation).  The system ends at psi_T. */
 ch_status ch_sys_record(ch_sys *sys, int64_t nsteps, int32_t ckpt_every);
 /* Reverse sweep over the last record: given dL/dpsi_T (N^dims doubles on the
- * host), writes dL/dpsi_0 and grads[3] = {dL/d epsilon (Landau; 0 for
- * SimpleWertheim), dL/dk, dL/dM}.  Windows between checkpoints are
- * recomputed on the device.  Synchronous. */
+ * host), writes dL/dpsi_0 (may be NULL) and grads[CH_NGRADS] (may be NULL).
+ * Windows between checkpoints are recomputed on the device.  Synchronous. */
 ch_status ch_sys_adjoint(ch_sys *sys, const double *dL_dpsiT, double *dL_dpsi0, double *grads);
 /* Same with device buffers of the system's precision. */
 ch_status ch_sys_adjoint_device(ch_sys *sys, const void *dL_dpsiT_dev, void *dL_dpsi0_dev, double *grads);
+/* The free-energy loss: F = average_free_energy of the current field (as
+ * ch_sys_observables, FP64), dF/dpsi (N^dims doubles; may be NULL) -- the
+ * seed of ch_sys_adjoint for a loss on F -- and the explicit partial
+ * derivatives dF/dparams[CH_NGRADS] (may be NULL).  Synchronous. */
+ch_status ch_sys_free_energy_grad(ch_sys *sys, double *F, double *dF_dpsi, double *dF_dparams);
 
 /* == FreeEnergyModel::der_bulk_free_energy(field_type*, float*, int) and
  * FreeEnergyModel::set_mobility(field_type*, double, field_type*, int)

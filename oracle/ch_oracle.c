@@ -621,3 +621,179 @@ void och_euler_adjoint(const och_params *p, const och_model *m, const double *ps
     for(int64_t t = steps - 1; t >= 0; t--) och_euler_adjoint_step(p, m, tape + t * bins, dL_dpsi0, grad);
     free(tape);
 }
+
+/* ------------------------------------------------------------------ */
+/* parameter derivatives (SimpleWertheim.cpp:40-111, mobility/*.h)      */
+/* With c = valence delta the bond fraction solves c rho X^2 + X - 1 = 0, */
+/* so dX/ddelta = -valence rho X^2 / (2 c rho X + 1) and                */
+/* dX/drho = -c X^2 / (2 c rho X + 1).                                  */
+/* ------------------------------------------------------------------ */
+static double wertheim_dX_ddelta(const och_model *m, double rho, double X) {
+    double c = 0.5 * two_valence_delta(m);
+    return -m->valence * rho * X * X / (2.0 * c * rho * X + 1.0);
+}
+
+static double wertheim_dX_drho(const och_model *m, double rho, double X) {
+    double c = 0.5 * two_valence_delta(m);
+    return -c * X * X / (2.0 * c * rho * X + 1.0);
+}
+
+double och_der_bulk_free_energy_ddelta(const och_model *m, double rho) {
+    if(m->model == OCH_LANDAU || !(rho > 0.)) return 0.0;
+    double X = och_wertheim_X(m, rho);
+    return m->valence * wertheim_dX_ddelta(m, rho, X) / X;
+}
+
+double och_bulk_free_energy_ddelta(const och_model *m, double rho) {
+    if(m->model == OCH_LANDAU || !(rho > 0.)) return 0.0;
+    double X = och_wertheim_X(m, rho);
+    return m->valence * rho * wertheim_dX_ddelta(m, rho, X) * (1.0 / X - 0.5);
+}
+
+double och_mobility_drho(const och_params *p, const och_model *m, double rho) {
+    switch(p->mobility) {
+    case OCH_MOB_FREE_ENERGY: {
+        double X = och_wertheim_X(m, rho), v = m->valence;
+        return p->M * (pow(X, v) + rho * v * pow(X, v - 1.0) * wertheim_dX_drho(m, rho, X));
+    }
+    case OCH_MOB_REGULARISED: return p->M * p->rho_min / SQR(rho + p->rho_min);
+    case OCH_MOB_LINEAR: return p->M / p->rho_ref;
+    default: return 0.0;
+    }
+}
+
+double och_mobility_ddelta(const och_params *p, const och_model *m, double rho) {
+    if(p->mobility != OCH_MOB_FREE_ENERGY) return 0.0;
+    double X = och_wertheim_X(m, rho), v = m->valence;
+    return p->M * rho * v * pow(X, v - 1.0) * wertheim_dX_ddelta(m, rho, X);
+}
+
+static int64_t neighbour(int dims, int N, int64_t i, int d, int s) {
+    int c[3];
+    fill_coords(dims, N, i, c);
+    c[d] = (c[d] + s + N) & (N - 1);
+    return cell_idx(dims, N, c);
+}
+
+/* Transpose of one step.  euler: psi' = psi + dt M L mu.  euler_mobility:
+ * psi' = psi + dt D(Mob, mu) with D(a, u)_i = sum_d [A(i,i+e)(u_{i+e} - u_i)
+ * - A(i-e,i)(u_i - u_{i-e})] / dx^2, A = (a_i + a_j) / 2 (the flux form of
+ * EulerMobilityCPU.cpp:44-82), Mob = mobility(psi), mu = f'(psi) - 2k L psi.
+ * <lambda', D(Mob, mu)> is bilinear in (Mob, mu), so with
+ *   nu  = D(Mob, lambda')                      (d/dmu, D is symmetric in u)
+ *   eta_j = -sum_d [(mu_{j+e}-mu_j)(l_{j+e}-l_j) + (mu_j-mu_{j-e})(l_j-l_{j-e})] / (2 dx^2)
+ * (d/dMob) the transpose is lambda = lambda' + dt (f'' nu - 2k L nu + Mob' eta). */
+void och_adjoint_step(int integrator, const och_params *p, const och_model *m, const double *psi, double *lambda,
+                      double *grad) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double inv_dx2 = 1.0 / SQR(p->dx);
+    double *nu = (double *)malloc(sizeof(double) * bins);
+    double *eta = (double *)malloc(sizeof(double) * bins);
+    double *mu = (double *)malloc(sizeof(double) * bins);
+    double *lp = (double *)malloc(sizeof(double) * bins);
+    double *mob = (double *)malloc(sizeof(double) * bins);
+    for(int64_t i = 0; i < bins; i++) lp[i] = cell_laplacian(dims, N, p->dx, psi, i);
+    for(int64_t i = 0; i < bins; i++) mu[i] = och_der_bulk_free_energy(m, psi[i]) - 2.0 * p->k * lp[i];
+    if(integrator == 0) {
+        /* nu = M L lambda'; d/dM of <lambda', dt M L mu> = dt <L lambda', mu> */
+        for(int64_t i = 0; i < bins; i++) {
+            double w = cell_laplacian(dims, N, p->dx, lambda, i);
+            mob[i] = p->M;
+            nu[i] = p->M * w;
+            eta[i] = w * mu[i];
+        }
+    }
+    else {
+        och_update_mobility(p, m, psi, mob, bins);
+        for(int64_t i = 0; i < bins; i++) {
+            double sn = 0.0, se = 0.0;
+            for(int d = 0; d < dims; d++) {
+                int64_t ip = neighbour(dims, N, i, d, 1), im = neighbour(dims, N, i, d, -1);
+                sn += 0.5 * (mob[i] + mob[ip]) * (lambda[ip] - lambda[i]) - 0.5 * (mob[im] + mob[i]) * (lambda[i] - lambda[im]);
+                se += (mu[ip] - mu[i]) * (lambda[ip] - lambda[i]) + (mu[i] - mu[im]) * (lambda[i] - lambda[im]);
+            }
+            nu[i] = sn * inv_dx2;
+            eta[i] = -0.5 * se * inv_dx2;
+        }
+    }
+    double ge = 0, gk = 0, gm = 0, gd = 0, gb = 0;
+    for(int64_t i = 0; i < bins; i++) {
+        ge += nu[i] * psi[i];
+        gk += nu[i] * lp[i];
+        gb += nu[i] * 2.0 * psi[i];
+        gd += nu[i] * och_der_bulk_free_energy_ddelta(m, psi[i]);
+        if(integrator == 0) gm += eta[i];
+        else {
+            /* the mobility is linear in mobility.M: Mob = M * (Mob / M) */
+            och_params p1 = *p;
+            p1.M = 1.0;
+            double mhat;
+            och_update_mobility(&p1, m, psi + i, &mhat, 1);
+            gm += eta[i] * mhat;
+            gd += eta[i] * och_mobility_ddelta(p, m, psi[i]);
+        }
+    }
+    if(m->model == OCH_LANDAU) grad[OCH_G_EPSILON] += -p->dt * ge;
+    else {
+        grad[OCH_G_DELTA] += p->dt * gd;
+        grad[OCH_G_B2] += p->dt * gb;
+    }
+    grad[OCH_G_K] += -2.0 * p->dt * gk;
+    grad[OCH_G_M] += p->dt * gm;
+    for(int64_t i = 0; i < bins; i++) {
+        double Lnu = cell_laplacian(dims, N, p->dx, nu, i);
+        double v = och_second_der_bulk_free_energy(m, psi[i]) * nu[i] - 2.0 * p->k * Lnu;
+        if(integrator == 1) v += och_mobility_drho(p, m, psi[i]) * eta[i];
+        lambda[i] += p->dt * v;
+    }
+    free(nu); free(eta); free(mu); free(lp); free(mob);
+}
+
+void och_adjoint(int integrator, const och_params *p, const och_model *m, const double *psi0, int64_t steps,
+                 const double *dL_dpsiT, double *dL_dpsi0, double *grad) {
+    int64_t bins = ipow(p->N, p->dims);
+    double *tape = (double *)malloc(sizeof(double) * bins * (steps + 1));
+    memcpy(tape, psi0, sizeof(double) * bins);
+    for(int64_t t = 0; t < steps; t++) {
+        memcpy(tape + (t + 1) * bins, tape + t * bins, sizeof(double) * bins);
+        if(integrator == 0) och_euler_step(p, m, tape + (t + 1) * bins);
+        else och_euler_mobility_step(p, m, tape + (t + 1) * bins);
+    }
+    memcpy(dL_dpsi0, dL_dpsiT, sizeof(double) * bins);
+    for(int g = 0; g < OCH_NGRADS; g++) grad[g] = 0.0;
+    for(int64_t t = steps - 1; t >= 0; t--) och_adjoint_step(integrator, p, m, tape + t * bins, dL_dpsi0, grad);
+    free(tape);
+}
+
+/* F = V_bin / bins sum_i [f(rho_i) + k sum_d ((rho_{i+e} - rho_i) / dx)^2]
+ * (src/CahnHilliard.cpp:300-319).  dF/drho_i = V_bin / bins (f'(rho_i) - 2k L rho_i):
+ * each forward difference appears in two cells' sums. */
+double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,
+                            double *dF_dparams) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double scale = p->dx * p->dx * p->dx / bins;
+    double sg = 0, se = 0, sd = 0, sb = 0;
+    for(int64_t i = 0; i < bins; i++) {
+        for(int d = 0; d < dims; d++) {
+            double g = (rho[neighbour(dims, N, i, d, 1)] - rho[i]) / p->dx;
+            sg += g * g;
+        }
+        se += -0.5 * SQR(rho[i]);
+        sd += och_bulk_free_energy_ddelta(m, rho[i]);
+        sb += SQR(rho[i]);
+        if(dF_drho)
+            dF_drho[i] = scale * (och_der_bulk_free_energy(m, rho[i]) - 2.0 * p->k * cell_laplacian(dims, N, p->dx, rho, i));
+    }
+    if(dF_dparams) {
+        for(int g = 0; g < OCH_NGRADS; g++) dF_dparams[g] = 0.0;
+        dF_dparams[OCH_G_K] = scale * sg;
+        if(m->model == OCH_LANDAU) dF_dparams[OCH_G_EPSILON] = scale * se;
+        else {
+            dF_dparams[OCH_G_DELTA] = scale * sd;
+            dF_dparams[OCH_G_B2] = scale * sb;
+        }
+    }
+    return och_average_free_energy(p, m, rho);
+}

@@ -99,6 +99,27 @@ void och_euler_adjoint_step(const och_params *p, const och_model *m, const doubl
 void och_euler_adjoint(const och_params *p, const och_model *m, const double *psi0, int64_t steps,
                        const double *dL_dpsiT, double *dL_dpsi0, double *grad);
 
+/* ---- parameter derivatives and the adjoint of both stencil integrators ----
+ * grad[] order: epsilon (Landau), k, M (the mobility.M scale), delta and B2
+ * (SimpleWertheim); entries a model does not have stay 0. */
+enum { OCH_G_EPSILON = 0, OCH_G_K = 1, OCH_G_M = 2, OCH_G_DELTA = 3, OCH_G_B2 = 4, OCH_NGRADS = 5 };
+/* d f'/d delta and d f/d delta (SimpleWertheim; 0 for Landau) */
+double och_der_bulk_free_energy_ddelta(const och_model *m, double rho);
+double och_bulk_free_energy_ddelta(const och_model *m, double rho);
+/* d M/d rho and d M/d delta of the mobility rule (och_update_mobility) */
+double och_mobility_drho(const och_params *p, const och_model *m, double rho);
+double och_mobility_ddelta(const och_params *p, const och_model *m, double rho);
+/* integrator 0 (euler, constant mobility) or 1 (euler_mobility, any
+ * mobility rule): lambda' -> lambda in place, grad[OCH_NGRADS] accumulated */
+void och_adjoint_step(int integrator, const och_params *p, const och_model *m, const double *psi, double *lambda,
+                      double *grad);
+void och_adjoint(int integrator, const och_params *p, const och_model *m, const double *psi0, int64_t steps,
+                 const double *dL_dpsiT, double *dL_dpsi0, double *grad);
+/* F = average_free_energy(rho) with dF/drho (may be NULL) and the explicit
+ * dF/dparams[OCH_NGRADS] (may be NULL) */
+double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,
+                            double *dF_dparams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,11 @@ def lib(fast=False):
         "och_sq_loss": (D, [I, D, P, P, P]),
         "och_euler_adjoint_step": (None, [PP, MP, P, P, P]),
         "och_euler_adjoint": (None, [PP, MP, P, I64, P, P, P]),
+        "och_der_bulk_free_energy_ddelta": (D, [MP, D]), "och_bulk_free_energy_ddelta": (D, [MP, D]),
+        "och_mobility_drho": (D, [PP, MP, D]), "och_mobility_ddelta": (D, [PP, MP, D]),
+        "och_adjoint_step": (None, [I, PP, MP, P, P, P]),
+        "och_adjoint": (None, [I, PP, MP, P, I64, P, P, P]),
+        "och_free_energy_grad": (D, [PP, MP, P, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(L, n)
@@ -250,3 +255,53 @@ def euler_adjoint_step(params, model, psi, lam):
     grad = np.zeros(3)
     _L().och_euler_adjoint_step(ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi), _ptr(lam), _ptr(grad))
     return lam, grad
+
+
+# grad[] order of adjoint() / free_energy_grad(): oracle/ch_oracle.h OCH_G_*
+G_EPSILON, G_K, G_M, G_DELTA, G_B2, NGRADS = 0, 1, 2, 3, 4, 5
+
+
+def der_bulk_ddelta(model, r):
+    return _L().och_der_bulk_free_energy_ddelta(ctypes.byref(model.c()), float(r))
+
+
+def bulk_ddelta(model, r):
+    return _L().och_bulk_free_energy_ddelta(ctypes.byref(model.c()), float(r))
+
+
+def mobility_drho(params, model, r):
+    return _L().och_mobility_drho(ctypes.byref(params.c()), ctypes.byref(model.c()), float(r))
+
+
+def mobility_ddelta(params, model, r):
+    return _L().och_mobility_ddelta(ctypes.byref(params.c()), ctypes.byref(model.c()), float(r))
+
+
+def adjoint(integrator, params, model, psi0, steps, dL_dpsiT):
+    """Reverse sweep of `steps` EULER / EULER_MOBILITY steps for the loss
+    <dL_dpsiT, psi_T>: returns (dL/dpsi0, grad[NGRADS])."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    w = np.ascontiguousarray(dL_dpsiT, dtype=np.float64)
+    out = np.empty_like(psi0)
+    grad = np.zeros(NGRADS)
+    _L().och_adjoint(integrator, ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi0), steps, _ptr(w),
+                     _ptr(out), _ptr(grad))
+    return out, grad
+
+
+def adjoint_step(integrator, params, model, psi, lam):
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    lam = np.array(lam, dtype=np.float64, copy=True)
+    grad = np.zeros(NGRADS)
+    _L().och_adjoint_step(integrator, ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi), _ptr(lam),
+                          _ptr(grad))
+    return lam, grad
+
+
+def free_energy_grad(params, model, rho):
+    """(F, dF/drho, explicit dF/dparams[NGRADS]) of average_free_energy."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    g = np.empty_like(rho)
+    dp = np.zeros(NGRADS)
+    F = _L().och_free_energy_grad(ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(rho), _ptr(g), _ptr(dp))
+    return F, g, dp

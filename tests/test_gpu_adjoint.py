@@ -1,8 +1,10 @@
-"""Reverse mode of the Euler integrator on the GPU (ch_sys_record / ch_sys_adjoint).
+"""Reverse mode of the euler and euler_mobility integrators on the GPU
+(ch_sys_record / ch_sys_adjoint / ch_sys_free_energy_grad).
 
-Tolerances: FP64 dL/dpsi0 and dL/d(eps, k, M) within 1e-10 relative of the
-oracle's FP64 adjoint (oracle/ch_oracle.c: och_euler_adjoint); central finite
-differences of the GPU forward map (h = 1e-6) within 1e-6; every checkpoint
+Tolerances: FP64 dL/dpsi0 within 1e-10 relative L-inf and dL/d(eps, k, M,
+delta, B2) within 1e-9 relative of the oracle's FP64 adjoint
+(oracle/ch_oracle.c: och_adjoint); central finite differences of the GPU
+forward map (h = 1e-6) within 1e-6 (1e-5 for chained losses); every checkpoint
 spacing gives the same dL/dpsi0 bits; FP32 fields within 2e-3 of FP64.
 """
 import numpy as np
@@ -37,11 +39,12 @@ def test_adjoint_against_oracle(ch, ctx, dims, N, steps, ckpt):
     psi0 = O.init_density(dims, N, 0.0, 0.3, 0, 123456)
     w = np.random.default_rng(dims).normal(size=N ** dims)
     psiT, lam, g = gpu_adjoint(ch, ctx, kw, psi0, steps, w, ckpt)
-    ref_lam, ref_g = O.euler_adjoint(O.Params(**kw), O.Model(epsilon=0.9), psi0, steps, w)
+    ref_lam, ref_g = O.adjoint(O.EULER, O.Params(**kw), O.Model(epsilon=0.9), psi0, steps, w)
     ref_T = O.run(O.EULER, O.Params(**kw), O.Model(epsilon=0.9), psi0, steps)
     assert rel_linf(psiT, ref_T) < 1e-12
     assert rel_linf(lam, ref_lam) < 1e-10
     np.testing.assert_allclose(g, ref_g, rtol=1e-10)
+    assert g[O.G_DELTA] == 0.0 and g[O.G_B2] == 0.0
 
 
 def test_checkpoint_spacing_invariance(ch, ctx):
@@ -104,7 +107,7 @@ def test_fp32_adjoint_against_fp64(ch, ctx):
 
 
 def test_wertheim_adjoint_against_oracle(ch, ctx):
-    """SimpleWertheim f'' in the adjoint (grads[0] is 0: no epsilon)."""
+    """SimpleWertheim f'' in the adjoint; the delta and B2 gradients (no epsilon)."""
     kw = dict(dims=2, N=32, dt=1e-4, dx=1.0, k=1.0, M=1.0)
     mk = dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4)
     psi0 = O.init_density(2, 32, 0.05, 0.05, 0, 7)
@@ -115,10 +118,10 @@ def test_wertheim_adjoint_against_oracle(ch, ctx):
     lam, g = s.adjoint(w)
     s.close()
     om = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4)
-    ref_lam, ref_g = O.euler_adjoint(O.Params(**kw), om, psi0, 50, w)
+    ref_lam, ref_g = O.adjoint(O.EULER, O.Params(**kw), om, psi0, 50, w)
     assert rel_linf(lam, ref_lam) < 1e-10
-    assert g[0] == 0.0
-    np.testing.assert_allclose(g[1:], ref_g[1:], rtol=1e-9)
+    assert g[O.G_EPSILON] == 0.0
+    np.testing.assert_allclose(g, ref_g, rtol=1e-9)
 
 
 def test_autograd_gradcheck(ch):
@@ -189,3 +192,122 @@ def test_c4_loss_gradient_chain(ch, ctx):
     fd_k = (loss_at(0.9, 1.2 + h) - loss_at(0.9, 1.2 - h)) / (2 * h)
     assert grads[0] == pytest.approx(fd_eps, rel=1e-5)
     assert grads[1] == pytest.approx(fd_k, rel=1e-5)
+
+
+EM_CASES = [
+    # dims, N, steps, ckpt, model kwargs (desc), mobility kwargs (desc), init (density, A), dt
+    (3, 16, 30, 7, dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4),
+     dict(mobility="linear", rho_ref=0.05), (0.05, 0.05), 1e-4),
+    (2, 64, 40, 16, dict(free_energy="simple_wertheim", B2=-2.0, delta=30.0, valence=2),
+     dict(mobility="free_energy"), (0.3, 0.2), 2e-4),
+    (2, 32, 50, 0, dict(epsilon=0.9), dict(mobility="regularised", rho_min=0.5), (0.5, 0.2), 0.01),
+    (1, 128, 60, 25, dict(epsilon=0.8), dict(mobility="constant"), (0.0, 0.3), 0.01),
+    (2, 128, 24, 5, dict(epsilon=0.9), dict(mobility="linear", rho_ref=0.5), (0.5, 0.1), 0.01),
+]
+
+
+def _oracle_model(mk):
+    if mk.get("free_energy") == "simple_wertheim":
+        return O.Model(O.SIMPLE_WERTHEIM, B2=mk["B2"], delta=mk["delta"], valence=mk["valence"])
+    return O.Model(O.LANDAU, epsilon=mk["epsilon"])
+
+
+def _oracle_params(dims, N, dt, k, M, mob):
+    table = {"constant": O.MOB_CONSTANT, "free_energy": O.MOB_FREE_ENERGY, "regularised": O.MOB_REGULARISED,
+             "linear": O.MOB_LINEAR}
+    return O.Params(dims=dims, N=N, dt=dt, dx=1.0, k=k, M=M, mobility=table[mob["mobility"]],
+                    rho_min=mob.get("rho_min", 0.0), rho_ref=mob.get("rho_ref", 1.0))
+
+
+@pytest.mark.parametrize("case", range(len(EM_CASES)))
+def test_euler_mobility_adjoint_against_oracle(ch, ctx, case):
+    dims, N, steps, ckpt, mk, mob, (rho0, A), dt = EM_CASES[case]
+    psi0 = O.init_density(dims, N, rho0, A, 0, 123456 + case)
+    w = np.random.default_rng(10 + case).normal(size=N ** dims)
+    desc = ch.make_desc(dims=dims, N=N, integrator="euler_mobility", dt=dt, k=1.1, M=0.9, **mk, **mob)
+    s = ch.System(ctx, desc)
+    s.upload(psi0)
+    s.record(steps, ckpt)
+    psiT = s.download()
+    lam, g = s.adjoint(w)
+    s.close()
+    p, m = _oracle_params(dims, N, dt, 1.1, 0.9, mob), _oracle_model(mk)
+    ref_T = O.run(O.EULER_MOBILITY, p, m, psi0, steps)
+    ref_lam, ref_g = O.adjoint(O.EULER_MOBILITY, p, m, psi0, steps, w)
+    print(f"case {case}: psiT {rel_linf(psiT, ref_T):.2e} lam {rel_linf(lam, ref_lam):.2e} grads {g} ref {ref_g}")
+    assert rel_linf(psiT, ref_T) < 1e-12
+    assert rel_linf(lam, ref_lam) < 1e-10
+    np.testing.assert_allclose(g, ref_g, rtol=1e-9, atol=1e-12 * np.abs(ref_g).max())
+
+
+def test_euler_mobility_checkpoint_invariance(ch, ctx):
+    dims, N, steps = 2, 64, 40
+    psi0 = O.init_density(dims, N, 0.3, 0.2, 0, 3)
+    w = np.random.default_rng(0).normal(size=N * N)
+    base = None
+    for ckpt in (0, 1, 7, 40):
+        s = ch.System(ctx, ch.make_desc(dims=dims, N=N, integrator="euler_mobility", dt=2e-4, free_energy="simple_wertheim",
+                                        B2=-2.0, delta=30.0, valence=2, mobility="free_energy"))
+        s.upload(psi0)
+        s.record(steps, ckpt)
+        lam, g = s.adjoint(w)
+        s.close()
+        if base is None:
+            base = (lam, g)
+        else:
+            assert np.array_equal(lam, base[0]), ckpt
+            np.testing.assert_allclose(g, base[1], rtol=1e-12)
+
+
+@pytest.mark.parametrize("dims,N,model,precision", [(3, 16, "simple_wertheim", "fp64"), (2, 64, "landau", "fp64"),
+                                                    (2, 64, "simple_wertheim", "fp32"), (1, 256, "landau", "fp32")])
+def test_free_energy_grad_against_oracle(ch, ctx, dims, N, model, precision):
+    mk = dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4) if model == "simple_wertheim" else \
+        dict(epsilon=0.7)
+    rho = O.init_density(dims, N, 0.05, 0.05, 0, 5) if model == "simple_wertheim" else \
+        O.init_density(dims, N, 0.0, 0.3, 0, 5)
+    s = ch.System(ctx, ch.make_desc(dims=dims, N=N, integrator="euler", dx=0.8, k=1.3, precision=precision, **mk))
+    s.upload(rho)
+    F, g, dp = s.free_energy_grad()
+    Fobs = s.average_free_energy()
+    s.close()
+    p = O.Params(dims=dims, N=N, dx=0.8, k=1.3)
+    if precision == "fp32":
+        rho = rho.astype(np.float32).astype(np.float64)  # the field the device holds
+    rF, rg, rdp = O.free_energy_grad(p, _oracle_model(mk), rho)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert F == pytest.approx(rF, rel=tol, abs=1e-14)
+    assert F == pytest.approx(Fobs, rel=1e-12, abs=1e-14)
+    assert np.abs(g - rg).max() <= tol * np.abs(rg).max()
+    np.testing.assert_allclose(dp, rdp, rtol=tol, atol=1e-15)
+
+
+def test_c5_free_energy_delta_gradient_chain(ch, ctx):
+    """C5 shape at small size: SimpleWertheim 3D euler_mobility with linear
+    mobility; d F(psi_T) / d delta = explicit part + adjoint part, against
+    central differences of the GPU forward map."""
+    N, T = 16, 40
+    kw = dict(dims=3, N=N, integrator="euler_mobility", dt=1e-4, k=1.0, M=1.0, free_energy="simple_wertheim", B2=1.0,
+              valence=4, mobility="linear", rho_ref=0.05)
+    psi0 = O.init_density(3, N, 0.05, 0.05, 0, 123456)
+    delta = 1e4
+
+    def F_at(dl):
+        s = ch.System(ctx, ch.make_desc(delta=dl, **kw))
+        s.upload(psi0)
+        s.evolve(T)
+        v = s.average_free_energy()
+        s.close()
+        return v
+
+    s = ch.System(ctx, ch.make_desc(delta=delta, **kw))
+    s.upload(psi0)
+    s.record(T, 10)
+    F, g, dp = s.free_energy_grad()
+    _, grads = s.adjoint(g)
+    s.close()
+    total = dp[ch.GRAD_DELTA] + grads[ch.GRAD_DELTA]
+    h = 1e-4 * delta
+    fd = (F_at(delta + h) - F_at(delta - h)) / (2 * h)
+    print(f"dF/ddelta = {total:.10e} (explicit {dp[ch.GRAD_DELTA]:.3e}), FD {fd:.10e}")
+    assert total == pytest.approx(fd, rel=1e-5)

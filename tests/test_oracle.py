@@ -262,3 +262,141 @@ def test_euler_adjoint_dot_product():
     Jv = (O.run(O.EULER, p, m, psi + h * v, 1) - O.run(O.EULER, p, m, psi - h * v, 1)) / (2 * h)
     JTu, _ = O.euler_adjoint_step(p, m, psi, u)
     assert np.dot(Jv, u) == pytest.approx(np.dot(v, JTu), rel=1e-8)
+
+
+def test_parameter_derivatives_by_finite_differences():
+    """d f'/d delta, d f/d delta, dM/drho and dM/d delta against central differences."""
+    models = [O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4),
+              O.Model(O.SIMPLE_WERTHEIM, B2=-2.0, delta=30.0, valence=2, regularisation_delta=1e-3)]
+    for m in models:
+        for r in (0.02, 0.05, 0.3):
+            hd = 1e-6 * m.delta
+            up, dn = O.Model(**{**m.__dict__, "delta": m.delta + hd}), O.Model(**{**m.__dict__, "delta": m.delta - hd})
+            fd = (O.der_bulk(up, np.array([r]))[0] - O.der_bulk(dn, np.array([r]))[0]) / (2 * hd)
+            assert O.der_bulk_ddelta(m, r) == pytest.approx(fd, rel=1e-6)
+            fd = (O.bulk(up, r) - O.bulk(dn, r)) / (2 * hd)
+            assert O.bulk_ddelta(m, r) == pytest.approx(fd, rel=1e-6)
+            for mob in (O.MOB_CONSTANT, O.MOB_FREE_ENERGY, O.MOB_REGULARISED, O.MOB_LINEAR):
+                p = O.Params(mobility=mob, M=1.3, rho_min=0.1, rho_ref=0.05)
+                h = 1e-6 * r
+                fd = (O.mobility(p, m, [r + h])[0] - O.mobility(p, m, [r - h])[0]) / (2 * h)
+                assert O.mobility_drho(p, m, r) == pytest.approx(fd, rel=1e-6, abs=1e-12)
+                fd = (O.mobility(p, up, [r])[0] - O.mobility(p, dn, [r])[0]) / (2 * hd)
+                assert O.mobility_ddelta(p, m, r) == pytest.approx(fd, rel=1e-6, abs=1e-14)
+    lm = O.Model(O.LANDAU, epsilon=0.9)
+    assert O.der_bulk_ddelta(lm, 0.3) == 0.0 and O.bulk_ddelta(lm, 0.3) == 0.0
+
+
+ADJOINT_CASES = [
+    # integrator, dims, N, steps, model kwargs, params kwargs, (init density, amplitude)
+    (O.EULER_MOBILITY, 3, 4, 15, dict(model=O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4),
+     dict(dt=1e-4, k=1.0, mobility=O.MOB_LINEAR, M=1.0, rho_ref=0.05), (0.05, 0.05)),
+    (O.EULER_MOBILITY, 2, 8, 20, dict(model=O.SIMPLE_WERTHEIM, B2=-2.0, delta=30.0, valence=2),
+     dict(dt=2e-4, k=0.7, mobility=O.MOB_FREE_ENERGY, M=1.1), (0.3, 0.2)),
+    (O.EULER_MOBILITY, 2, 8, 25, dict(model=O.LANDAU, epsilon=0.9),
+     dict(dt=0.01, k=1.2, mobility=O.MOB_REGULARISED, M=0.9, rho_min=0.5), (0.5, 0.2)),
+    (O.EULER_MOBILITY, 1, 16, 30, dict(model=O.LANDAU, epsilon=0.8),
+     dict(dt=0.01, k=1.0, mobility=O.MOB_CONSTANT, M=1.3, dx=0.7), (0.0, 0.3)),
+    (O.EULER, 2, 8, 20, dict(model=O.SIMPLE_WERTHEIM, B2=1.0, delta=1e3, valence=4),
+     dict(dt=1e-4, k=1.0, M=1.0), (0.2, 0.1)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ADJOINT_CASES)))
+def test_adjoint_against_finite_differences(case):
+    """dL/dpsi0 and dL/d(eps, k, M, delta, B2) of L = <w, psi_T> against central
+    differences of the forward map, for both stencil integrators and every
+    mobility rule."""
+    integ, dims, N, steps, mk, pk, (rho0, A) = ADJOINT_CASES[case]
+    p = O.Params(dims=dims, N=N, **pk)
+    m = O.Model(**mk)
+    psi0 = O.init_density(dims, N, rho0, A, 0, 123456 + case)
+    rng = np.random.default_rng(case)
+    w, d = rng.normal(size=N ** dims), rng.normal(size=N ** dims)
+    lam0, grad = O.adjoint(integ, p, m, psi0, steps, w)
+
+    def loss(pp, mm, x):
+        return float(np.dot(w, O.run(integ, pp, mm, x, steps)))
+
+    h = 1e-6 * max(1.0, np.abs(psi0).max())
+    fd = (loss(p, m, psi0 + h * d) - loss(p, m, psi0 - h * d)) / (2 * h)
+    assert np.dot(lam0, d) == pytest.approx(fd, rel=1e-6)
+    names = {O.G_EPSILON: ("epsilon", "m"), O.G_K: ("k", "p"), O.G_M: ("M", "p"), O.G_DELTA: ("delta", "m"),
+             O.G_B2: ("B2", "m")}
+    for g, (attr, obj) in names.items():
+        base = getattr(m if obj == "m" else p, attr)
+        hp = 1e-6 * max(abs(base), 1.0)
+
+        def shifted(s):
+            pp = O.Params(**{**p.__dict__})
+            mm = O.Model(**{**m.__dict__})
+            target = mm if obj == "m" else pp
+            setattr(target, attr, getattr(target, attr) + s)
+            return loss(pp, mm, psi0)
+
+        fd = (shifted(hp) - shifted(-hp)) / (2 * hp)
+        has = not ((g == O.G_EPSILON and m.model != O.LANDAU) or (g in (O.G_DELTA, O.G_B2) and m.model == O.LANDAU))
+        if has:
+            assert grad[g] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(w).sum()), attr
+        else:
+            assert grad[g] == 0.0
+
+
+def test_general_adjoint_reduces_to_euler_adjoint():
+    N, steps = 16, 30
+    p = O.Params(dims=2, N=N, dt=0.01, dx=1.0, k=1.2, M=0.9)
+    m = O.Model(epsilon=0.9)
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 99)
+    w = np.random.default_rng(8).normal(size=N * N)
+    lam_a, g_a = O.euler_adjoint(p, m, psi0, steps, w)
+    lam_b, g_b = O.adjoint(O.EULER, p, m, psi0, steps, w)
+    assert np.abs(lam_a - lam_b).max() <= 1e-13 * np.abs(lam_a).max()
+    np.testing.assert_allclose(g_b[:3], g_a, rtol=1e-12)
+    # euler_mobility with constant mobility is the same operator up to round-off
+    lam_c, g_c = O.adjoint(O.EULER_MOBILITY, p, m, psi0, steps, w)
+    assert np.abs(lam_c - lam_b).max() <= 1e-11 * np.abs(lam_b).max()
+    np.testing.assert_allclose(g_c, g_b, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("mob", [O.MOB_FREE_ENERGY, O.MOB_LINEAR, O.MOB_REGULARISED])
+def test_euler_mobility_adjoint_dot_product(mob):
+    """<J v, u> = <v, J^T u> for one euler_mobility step (J by central differences)."""
+    N = 8
+    p = O.Params(dims=2, N=N, dt=1e-3, k=1.0, mobility=mob, M=1.2, rho_min=0.2, rho_ref=0.3)
+    m = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=50.0, valence=3)
+    rng = np.random.default_rng(mob)
+    psi = rng.uniform(0.1, 0.6, N * N)
+    v, u = rng.normal(size=N * N), rng.normal(size=N * N)
+    h = 1e-7
+    Jv = (O.run(O.EULER_MOBILITY, p, m, psi + h * v, 1) - O.run(O.EULER_MOBILITY, p, m, psi - h * v, 1)) / (2 * h)
+    JTu, _ = O.adjoint_step(O.EULER_MOBILITY, p, m, psi, u)
+    assert np.dot(Jv, u) == pytest.approx(np.dot(v, JTu), rel=1e-8)
+
+
+@pytest.mark.parametrize("model", [O.Model(O.LANDAU, epsilon=0.7),
+                                   O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e4, valence=4)])
+def test_free_energy_gradient_by_finite_differences(model):
+    N, dims = 8, 3
+    p = O.Params(dims=dims, N=N, dx=0.8, k=1.3)
+    rho = O.init_density(dims, N, 0.05, 0.05, 0, 7) if model.model != O.LANDAU else \
+        O.init_density(dims, N, 0.0, 0.3, 0, 7)
+    F, g, dp = O.free_energy_grad(p, model, rho)
+    assert F == O.average_free_energy(p, model, rho)
+    d = np.random.default_rng(2).normal(size=rho.size)
+    h = 1e-6 * np.abs(rho).max()
+    fd = (O.average_free_energy(p, model, rho + h * d) - O.average_free_energy(p, model, rho - h * d)) / (2 * h)
+    assert np.dot(g, d) == pytest.approx(fd, rel=1e-6)
+    for gi, (attr, obj) in {O.G_EPSILON: ("epsilon", "m"), O.G_K: ("k", "p"), O.G_DELTA: ("delta", "m"),
+                            O.G_B2: ("B2", "m")}.items():
+        base = getattr(model if obj == "m" else p, attr)
+        # F is linear in epsilon, k and B2: a large step has no truncation error
+        hp = 1e-6 * max(abs(base), 1.0) if attr == "delta" else 0.1
+        vals = []
+        for s in (hp, -hp):
+            pp, mm = O.Params(**{**p.__dict__}), O.Model(**{**model.__dict__})
+            t = mm if obj == "m" else pp
+            setattr(t, attr, getattr(t, attr) + s)
+            vals.append(O.average_free_energy(pp, mm, rho))
+        fd = (vals[0] - vals[1]) / (2 * hp)
+        assert dp[gi] == pytest.approx(fd, rel=1e-6, abs=1e-12), attr
+    assert dp[O.G_M] == 0.0
