@@ -247,7 +247,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     if(const char *tb = std::getenv("CH_TB_STEPS")) want = std::atoi(tb);
     s->tb = 1;
     while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
-    if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(2, std::atoi(r)));
+    if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
     // graph chunk: a multiple of 2 tb launches' worth of steps (parity restored)
     const char *g = std::getenv("CH_GRAPH_STEPS");
     s->graph_steps = g ? std::atoi(g) : 64;
@@ -381,7 +381,7 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K) {
     hipStream_t st = s->ctx->stream;
     if(s->d.integrator == CH_PSEUDOSPECTRAL) {
         chd::SpectralArgs a = spectral_args(s);
-        return chd::ps_run(a, st, false);
+        return chd::ps_steps(a, st, K);
     }
     const void *in = s->psi[cur];
     void *out = s->psi[cur ^ 1];
@@ -396,10 +396,37 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K) {
     return e;
 }
 
+// pseudospectral: chains of steps joined by the fused row pass; a chain of G
+// steps is captured once (one buffer, no parity) and replayed
+static ch_status evolve_spectral(ch_sys *s, int64_t nsteps) {
+    ch_ctx *c = s->ctx;
+    chd::SpectralArgs a = spectral_args(s);
+    const int G = s->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        if(!s->gexec[0]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = chd::ps_steps(a, c->stream, G);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&s->gexec[0], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[0], c->stream));
+        nsteps -= reps * G;
+    }
+    if(nsteps > 0) HIPCHK(c, chd::ps_steps(a, c->stream, nsteps));
+    return CH_OK;
+}
+
 extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
     if(!s || nsteps < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if(s->d.integrator == CH_PSEUDOSPECTRAL) return evolve_spectral(s, nsteps);
     const int G = s->graph_steps;
     if(G > 0 && nsteps >= G) {
         const int par = s->cur;
@@ -421,7 +448,12 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
         nsteps -= reps * G;  // G / tb launches, an even number: parity unchanged
     }
     for(; nsteps >= s->tb; nsteps -= s->tb) HIPCHK(c, launch_steps(s, s->cur, s->tb));
-    for(; nsteps > 0; nsteps--) HIPCHK(c, launch_steps(s, s->cur, 1));
+    // the remainder (< tb) in launches of decreasing depth: at most log2(tb)
+    for(int K = s->tb / 2; K >= 1; K /= 2)
+        if(nsteps >= K) {
+            HIPCHK(c, launch_steps(s, s->cur, K));
+            nsteps -= K;
+        }
     return CH_OK;
 }
 

@@ -164,6 +164,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// number of 2D wave-kernel region shapes selectable through Params::wave_cfg
+constexpr int kWaveCfgs = 7;
+
 template <typename T> struct Complex;
 template <> struct Complex<double> { using type = double2; };
 template <> struct Complex<float> { using type = float2; };

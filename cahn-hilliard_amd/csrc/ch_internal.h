@@ -24,6 +24,9 @@ struct SpectralArgs {
 };
 // init = true: rho_hat = r2c(rho); otherwise one time step
 hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init);
+// nsteps time steps, consecutive steps joined by the fused c2r + f' + r2c row
+// pass (same results as nsteps calls of ps_run)
+hipError_t ps_steps(const SpectralArgs &a, hipStream_t s, int64_t nsteps);
 // 2D transforms on a.work: forward r2c of a.rho, inverse (c2r) into out
 hipError_t fft2_forward(const SpectralArgs &a, hipStream_t s);
 hipError_t fft2_inverse(const SpectralArgs &a, void *out, hipStream_t s);

@@ -151,21 +151,20 @@ __device__ __forceinline__ void mode_q2(const Params &p, int ix, int iy, int iz,
     }
 }
 
+// rh: this mode of rho_hat, updated in place; returns rho_hat / N^d
 template <typename T, typename T2>
-__device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, int iz, T2 fh, T2 *rho_hat_elem,
-                                              T inv_bins) {
+__device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, int iz, T2 fh, T2 &rh, T inv_bins) {
     double k2d, deal;
     mode_q2(p, ix, iy, iz, k2d, deal);
     if(p.use_dealias) { fh.x *= (T)deal; fh.y *= (T)deal; }
     const T k2 = (T)k2d, k4 = k2 * k2;
     const T dt = (T)p.dt, M = (T)p.M, S = (T)p.S;
     const T denom = (T)1.0 + dt * M * (S * k2 + (T)2.0 * (T)p.k * k4);
-    const T2 rh = *rho_hat_elem;
     const T c = dt * M * k2;
     T2 nw;
     nw.x = (rh.x - c * (fh.x - S * rh.x)) / denom;
     nw.y = (rh.y - c * (fh.y - S * rh.y)) / denom;
-    *rho_hat_elem = nw;
+    rh = nw;
     return T2{nw.x * inv_bins, nw.y * inv_bins};
 }
 
@@ -259,9 +258,74 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
     }
 }
 
+// ---- x pass between two time steps: c2r of step n's half spectrum (as
+// k_xinv), then f' and r2c of step n + 1 (as k_xfwd), in place on W.  The real
+// row stays in LDS as T values, so the result has the same bits as k_xinv
+// followed by k_xfwd, with one read and one write of W instead of two each.
+template <typename T, int MODEL, int LOG2N>
+__global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xrow(
+    typename Complex<T>::type *__restrict__ W, const typename Complex<T>::type *__restrict__ tw, Params p, int rows) {
+    using T2 = typename Complex<T>::type;
+    using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
+    constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
+    __shared__ T2 buf[padded(G::E + nl)];
+    LineFFT<T2, LOG2N, nl> fft;
+    fft.prefetch(tw, 2);
+    const int row0 = blockIdx.x * nl;
+    for(int e = threadIdx.x; e < nl * Nh; e += NT) {
+        const int line = e / Nh, k = e % Nh;
+        const int row = row0 + line;
+        buf[pidx(e)] = (row < rows) ? W[(size_t)row * Nh + k] : T2{0, 0};
+    }
+    __syncthreads();
+    T2 z[4];
+#pragma unroll
+    for(int i = 0; i < 4; i++) {
+        const int q = threadIdx.x + i * NT;
+        const int line = q / n, k = q % n;
+        T2 xk = buf[pidx(line * Nh + k)];
+        T2 xnk = buf[pidx(line * Nh + n - k)];
+        if(k == 0) { xk.y = 0; xnk.y = 0; }
+        const T2 Ev = T2{xk.x + xnk.x, xk.y - xnk.y};
+        const T2 Ov = cmul(T2{xk.x - xnk.x, xk.y + xnk.y}, conj_(tw[k]));
+        z[i] = T2{Ev.x - Ov.y, Ev.y + Ov.x};
+    }
+    __syncthreads();
+#pragma unroll
+    for(int i = 0; i < 4; i++) buf[pidx(threadIdx.x + i * NT)] = z[i];
+    __syncthreads();
+    fft.template run<true>(buf);
+    T *bufT = reinterpret_cast<T *>(buf);
+    // rho (the c2r output, as T) -> f'(rho); each thread rewrites its own reals
+#pragma unroll
+    for(int i = 0; i < 8; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int row = row0 + e / N;
+        const int a = 2 * pidx(e >> 1) + (e & 1);
+        bufT[a] = (row < rows) ? fprime<T, MODEL>(p, bufT[a]) : (T)0;
+    }
+    __syncthreads();
+    fft.template run<false>(buf);
+    for(int e = threadIdx.x; e < nl * Nh; e += NT) {
+        const int line = e / Nh, k = e % Nh;
+        const int row = row0 + line;
+        if(row >= rows) continue;
+        const T2 zk = buf[pidx(line * n + (k == n ? 0 : k))];
+        const T2 znk = buf[pidx(line * n + (k == 0 ? 0 : n - k))];
+        const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
+        const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
+        const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
+        W[(size_t)row * Nh + k] = cadd(Ev, cmul(tw[k], Ov));
+    }
+}
+
 // ---- strided axis pass over the half spectrum, NLC lines per block.  Line l
 // starts at (l / Nh) * P + (l % Nh) and steps by `stride`.  MODE 0: forward,
 // 1: inverse, 2: forward + spectral update of rho_hat + /N^d + inverse.
+// Element e of a block is (row j = e / NLC, line l0 + e % NLC), so a wave
+// touches NLC adjacent lines of 32 rows; the lines of one 128-byte row segment
+// belong to consecutive blocks, which the XCD-aware block order keeps on one
+// XCD (one L2).  rho_hat is fetched at kernel start, ahead of the transform.
 template <typename T, int MODE, int LOG2N, int NLC>
 __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
     const typename Complex<T>::type *__restrict__ src, typename Complex<T>::type *__restrict__ dst,
@@ -271,17 +335,25 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
     using G = FftGeom<LOG2N, NLC>;
     constexpr int n = G::n, NT = G::NT, nl = NLC;
     __shared__ T2 buf[padded(G::E)];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int l0 = tile * nl;
+    if(l0 >= lines) return;  // grid padded to a multiple of 8 blocks
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 1);
     const int Nh = p.N / 2 + 1;
-    const int l0 = blockIdx.x * nl;
+    size_t gi[4];
+    T2 rh[4];
 #pragma unroll
     for(int i = 0; i < 4; i++) {
         const int e = threadIdx.x + i * NT;
         const int j = e / nl, lb = e % nl;
         const int l = l0 + lb;
+        gi[i] = (size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride;
         T2 v{0, 0};
-        if(l < lines) v = src[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride];
+        if(l < lines) {
+            v = src[gi[i]];
+            if constexpr(MODE == 2) rh[i] = rho_hat[gi[i]];
+        }
         buf[pidx(lb * n + j)] = v;
     }
     __syncthreads();
@@ -290,16 +362,17 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
         const T inv_bins = (T)p.inv_bins;
 #pragma unroll
         for(int i = 0; i < 4; i++) {
-            const int q = threadIdx.x + i * NT;
-            const int lb = q / n, j = q % n;
+            const int e = threadIdx.x + i * NT;
+            const int j = e / nl, lb = e % nl;
             const int l = l0 + lb;
             if(l < lines) {
-                const size_t gi = (size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride;
                 const int ix = l % Nh;
                 int iy, iz;
                 if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
                 else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
-                buf[pidx(q)] = spectral_update<T, T2>(p, ix, iy, iz, buf[pidx(q)], rho_hat + gi, inv_bins);
+                const int q = pidx(lb * n + j);
+                buf[q] = spectral_update<T, T2>(p, ix, iy, iz, buf[q], rh[i], inv_bins);
+                rho_hat[gi[i]] = rh[i];
             }
         }
         __syncthreads();
@@ -309,8 +382,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
     for(int i = 0; i < 4; i++) {
         const int e = threadIdx.x + i * NT;
         const int j = e / nl, lb = e % nl;
-        const int l = l0 + lb;
-        if(l < lines) dst[(size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride] = buf[pidx(lb * n + j)];
+        if(l0 + lb < lines) dst[gi[i]] = buf[pidx(lb * n + j)];
     }
 }
 
@@ -321,7 +393,9 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
     using T2 = typename Complex<T>::type;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= nh) return;
-    W[i] = spectral_update<T, T2>(p, i, 0, 0, W[i], rho_hat + i, (T)p.inv_bins);
+    T2 rh = rho_hat[i];
+    W[i] = spectral_update<T, T2>(p, i, 0, 0, W[i], rh, (T)p.inv_bins);
+    rho_hat[i] = rh;
 }
 
 // ------------------------------------------------------------------ host side
@@ -330,8 +404,14 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
 template <int LOG2N>
 struct ColLines { static constexpr int value = LOG2N >= 11 ? 1 : (LOG2N == 10 ? 2 : (LOG2N >= 8 ? 4 : (1024 >> LOG2N))); };
 
+// k_col grid: the line groups padded to a multiple of the 8 XCDs
+static inline int col_blocks(int lines, int nlc) { return ((lines + nlc - 1) / nlc + 7) / 8 * 8; }
+
+// init: rho_hat = r2c(rho).  Otherwise `nsteps` time steps: the x pass of f'
+// (rho), then per step the axis passes with the fused spectral update, joined
+// to the next step by k_xrow; the last step ends with the c2r into rho.
 template <typename T, int LOG2N>
-static hipError_t ps_run_t(const SpectralArgs &a, hipStream_t s, bool init) {
+static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init) {
     using T2 = typename Complex<T>::type;
     const Params &p = a.p;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1;
@@ -345,73 +425,84 @@ static hipError_t ps_run_t(const SpectralArgs &a, hipStream_t s, bool init) {
     T *rho = (T *)a.rho;
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
     const int bx = (rows + GX::NL - 1) / GX::NL;
+    const bool landau = a.model == CH_LANDAU;
     if(init) {
         hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, dims == 1 ? RH : W,
                            tw2, p, rows);
-    }
-    else if(a.model == CH_LANDAU) {
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows);
-    }
-    else {
-        hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p,
-                           rows);
-    }
-    if(dims == 1) {
-        if(init) return hipGetLastError();
-        hipLaunchKernelGGL((k_update1d<T>), dim3((Nh + 255) / 256), dim3(256), 0, s, W, RH, p, Nh);
-    }
-    else if(dims == 2) {
-        const int lines = Nh;
-        const int bc = (lines + NLC - 1) / NLC;
-        if(init) {
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, RH, RH, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
-            return hipGetLastError();
+        if(dims == 2) {
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, RH, RH,
+                               tw2, p, Nh, (long)N * Nh, (long)Nh, 1);
         }
-        hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                           (long)N * Nh, (long)Nh, 1);
-    }
-    else {
-        const int lines = N * Nh;
-        const int bc = (lines + NLC - 1) / NLC;
-        // y forward (line l: plane l / Nh, column l % Nh), then the fused z pass
-        hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                           (long)N * Nh, (long)Nh, 1);
-        if(init) {
+        else if(dims == 3) {
+            const int lines = N * Nh, bc = col_blocks(lines, NLC);
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
             hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, RH, RH, tw2, p, lines,
                                (long)Nh, (long)N * Nh, 2);
-            return hipGetLastError();
         }
-        hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines, (long)Nh,
-                           (long)N * Nh, 2);
-        hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                           (long)N * Nh, (long)Nh, 1);
+        return hipGetLastError();
+    }
+    if(nsteps < 1) return hipSuccess;
+    if(landau)
+        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows);
+    else
+        hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p,
+                           rows);
+    for(int64_t t = 0; t < nsteps; t++) {
+        if(dims == 1) {
+            hipLaunchKernelGGL((k_update1d<T>), dim3((Nh + 255) / 256), dim3(256), 0, s, W, RH, p, Nh);
+        }
+        else if(dims == 2) {
+            hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, W, RH,
+                               tw2, p, Nh, (long)N * Nh, (long)Nh, 1);
+        }
+        else {
+            // y forward (line l: plane l / Nh, column l % Nh), the fused z pass, y inverse
+            const int lines = N * Nh, bc = col_blocks(lines, NLC);
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
+            hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                               (long)Nh, (long)N * Nh, 2);
+            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
+        }
+        if(t + 1 < nsteps) {
+            if(landau)
+                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows);
+            else
+                hipLaunchKernelGGL((k_xrow<T, CH_SIMPLE_WERTHEIM, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows);
+        }
     }
     hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, rho, tw2, p, rows);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t ps_run_p(const SpectralArgs &a, hipStream_t s, bool init) {
+static hipError_t ps_chain_p(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init) {
     switch(a.p.log2N) {
-    case 2: return ps_run_t<T, 2>(a, s, init);
-    case 3: return ps_run_t<T, 3>(a, s, init);
-    case 4: return ps_run_t<T, 4>(a, s, init);
-    case 5: return ps_run_t<T, 5>(a, s, init);
-    case 6: return ps_run_t<T, 6>(a, s, init);
-    case 7: return ps_run_t<T, 7>(a, s, init);
-    case 8: return ps_run_t<T, 8>(a, s, init);
-    case 9: return ps_run_t<T, 9>(a, s, init);
-    case 10: return ps_run_t<T, 10>(a, s, init);
-    case 11: return ps_run_t<T, 11>(a, s, init);
-    case 12: return ps_run_t<T, 12>(a, s, init);
+    case 2: return ps_chain_t<T, 2>(a, s, nsteps, init);
+    case 3: return ps_chain_t<T, 3>(a, s, nsteps, init);
+    case 4: return ps_chain_t<T, 4>(a, s, nsteps, init);
+    case 5: return ps_chain_t<T, 5>(a, s, nsteps, init);
+    case 6: return ps_chain_t<T, 6>(a, s, nsteps, init);
+    case 7: return ps_chain_t<T, 7>(a, s, nsteps, init);
+    case 8: return ps_chain_t<T, 8>(a, s, nsteps, init);
+    case 9: return ps_chain_t<T, 9>(a, s, nsteps, init);
+    case 10: return ps_chain_t<T, 10>(a, s, nsteps, init);
+    case 11: return ps_chain_t<T, 11>(a, s, nsteps, init);
+    case 12: return ps_chain_t<T, 12>(a, s, nsteps, init);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init) {
-    if(a.precision == CH_FP64) return ps_run_p<double>(a, s, init);
-    return ps_run_p<float>(a, s, init);
+    if(a.precision == CH_FP64) return ps_chain_p<double>(a, s, 1, init);
+    return ps_chain_p<float>(a, s, 1, init);
+}
+
+hipError_t ps_steps(const SpectralArgs &a, hipStream_t s, int64_t nsteps) {
+    if(a.precision == CH_FP64) return ps_chain_p<double>(a, s, nsteps, false);
+    return ps_chain_p<float>(a, s, nsteps, false);
 }
 
 // ---- 2D inverse transform (column IFFT + c2r) of a.work into `out`
@@ -424,7 +515,7 @@ static hipError_t fft2_inverse_t(const SpectralArgs &a, void *out, hipStream_t s
     using GC = FftGeom<LOG2N, NLC>;
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)a.work;
-    hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3((Nh + NLC - 1) / NLC), dim3(GC::NT), 0, s, W, W, W, tw2, a.p, Nh,
+    hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, W, W, tw2, a.p, Nh,
                        (long)N * Nh, (long)Nh, 1);
     hipLaunchKernelGGL((k_xinv<T, LX>), dim3((N + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, (T *)out, tw2, a.p, N);
     return hipGetLastError();

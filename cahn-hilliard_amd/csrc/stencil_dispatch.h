@@ -29,12 +29,21 @@ hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t
 }
 
 // 2D grids with N >= 64 use the register/DPP wave kernel.  Region shapes
-// (p.wave_cfg): 0 = 1 column x 16 rows x 4 waves (64 x 64), 1 = 2 x 8 x 4
-// (128 x 32), 2 = 2 x 8 x 8 (128 x 64); K in {1, 2, 4, 8} steps per launch
-// where the region leaves an interior.
+// (p.wave_cfg), columns per lane x rows per wave x waves (region W x H):
+//   0 = 1 x 16 x 4 (64 x 64)    1 = 2 x 8 x 4 (128 x 32)   2 = 2 x 8 x 8 (128 x 64)
+//   3 = 1 x 8 x 8 (64 x 64)     4 = 1 x 4 x 16 (64 x 64)   5 = 2 x 4 x 16 (128 x 64)
+//   6 = 1 x 8 x 16 (64 x 128)
+// Fewer cells per lane means more waves per CU to hide latency, at the price
+// of more row exchanges through LDS.  K in {1, 2, 4, 8} steps per launch where
+// the region leaves an interior.
 using WaveCfg0 = WaveCfg<1, 16, 4>;
 using WaveCfg1 = WaveCfg<2, 8, 4>;
 using WaveCfg2 = WaveCfg<2, 8, 8>;
+using WaveCfg3 = WaveCfg<1, 8, 8>;
+using WaveCfg4 = WaveCfg<1, 4, 16>;
+using WaveCfg5 = WaveCfg<2, 4, 16>;
+using WaveCfg6 = WaveCfg<1, 8, 16>;
+static_assert(kWaveCfgs == 7, "one WaveCfg per p.wave_cfg value");
 
 template <class CFG, int K>
 constexpr bool wave_ok() { return CFG::RW - 4 * K > 0 && CFG::RH - 4 * K > 0; }
@@ -56,8 +65,17 @@ template <typename T, int DIMS, int MODEL, int MOB, bool EM>
 hipError_t stencil_k(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     if constexpr(DIMS == 2) {
         if(p.N >= 64) {
-            if(p.wave_cfg == 1 && p.N >= 128) return wave_k<T, MODEL, MOB, EM, WaveCfg1>(p, K, in, out, s);
-            if(p.wave_cfg == 2 && p.N >= 128) return wave_k<T, MODEL, MOB, EM, WaveCfg2>(p, K, in, out, s);
+            if(p.N >= 128) {
+                switch(p.wave_cfg) {
+                case 1: return wave_k<T, MODEL, MOB, EM, WaveCfg1>(p, K, in, out, s);
+                case 2: return wave_k<T, MODEL, MOB, EM, WaveCfg2>(p, K, in, out, s);
+                case 5: return wave_k<T, MODEL, MOB, EM, WaveCfg5>(p, K, in, out, s);
+                case 6: return wave_k<T, MODEL, MOB, EM, WaveCfg6>(p, K, in, out, s);
+                default: break;
+                }
+            }
+            if(p.wave_cfg == 3) return wave_k<T, MODEL, MOB, EM, WaveCfg3>(p, K, in, out, s);
+            if(p.wave_cfg == 4) return wave_k<T, MODEL, MOB, EM, WaveCfg4>(p, K, in, out, s);
             return wave_k<T, MODEL, MOB, EM, WaveCfg0>(p, K, in, out, s);
         }
     }
