@@ -29,7 +29,7 @@
 #include <type_traits>
 
 #include "ch_internal.h"
-#include "stencil.h"
+#include "stencil_dispatch.h"
 
 namespace chd {
 
@@ -512,18 +512,28 @@ static hipError_t launch_tape(const Params &p, const void *in, void *tape_out, h
     return hipGetLastError();
 }
 
+// 2D grids with N >= 64 recompute with the register/DPP wave kernel (the
+// forward's kernel family: same bits), K = 1 or 4 states per launch
 template <typename T, int DIMS, int MODEL, int MOB, bool EM>
 static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     const int N = p.N;
+    if constexpr(DIMS == 2) {
+        if(N >= 64) {
+            if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg0, true>(p, in, out, s);
+            if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg0, true>(p, in, out, s);
+            return hipErrorInvalidValue;
+        }
+    }
+    if constexpr(DIMS == 1 && !EM) {
+        if(K == 4 && N >= 64) return launch_tape<T, 1, MODEL, MOB, EM, 64, 1, 1, 4>(p, in, out, s);
+    }
+    if(K != 1) return hipErrorInvalidValue;
     if constexpr(DIMS == 1) {
-        if constexpr(!EM) if(K == 4 && N >= 64) return launch_tape<T, 1, MODEL, MOB, EM, 64, 1, 1, 4>(p, in, out, s);
         if(N >= 64) return launch_tape<T, 1, MODEL, MOB, EM, 64, 1, 1, 1>(p, in, out, s);
         if(N >= 16) return launch_tape<T, 1, MODEL, MOB, EM, 16, 1, 1, 1>(p, in, out, s);
         return launch_tape<T, 1, MODEL, MOB, EM, 4, 1, 1, 1>(p, in, out, s);
     }
     else if constexpr(DIMS == 2) {
-        if constexpr(!EM) if(K == 4 && N >= 64) return launch_tape<T, 2, MODEL, MOB, EM, 64, 16, 1, 4>(p, in, out, s);
-        if(N >= 64) return launch_tape<T, 2, MODEL, MOB, EM, 64, 16, 1, 1>(p, in, out, s);
         if(N >= 16) return launch_tape<T, 2, MODEL, MOB, EM, 16, 16, 1, 1>(p, in, out, s);
         return launch_tape<T, 2, MODEL, MOB, EM, 4, 4, 1, 1>(p, in, out, s);
     }
@@ -533,9 +543,10 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
     }
 }
 
+int tape_steps_per_launch(int dims, int N, bool em) { return (N >= 64 && (dims == 2 || (dims == 1 && !em))) ? 4 : 1; }
+
 hipError_t tape_launch(const Params &p, int precision, int model, int mob, bool em, int K, const void *in,
                        void *tape_out, hipStream_t s) {
-    if(em && K != 1) return hipErrorInvalidValue;
     auto go = [&](auto tag) -> hipError_t {
         using T = decltype(tag);
         if(em)

@@ -813,7 +813,7 @@ static char *field_at(void *base, int64_t i, size_t field_bytes) { return (char 
 static hipError_t tape_steps(ch_sys *s, int64_t from, int64_t count) {
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
-    const int Kt = chd::adjoint_steps_per_launch(s->d.dims, s->d.N, em);
+    const int Kt = chd::tape_steps_per_launch(s->d.dims, s->d.N, em);
     while(count > 0) {
         const int K = count >= Kt ? Kt : 1;
         hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,

@@ -84,7 +84,9 @@ __device__ __forceinline__ void row_neighbours(const T (&v)[C][RPW], int j, cons
 
 // K steps on one region whose top-left grid cell is (gx0, gy0) (halo mode) or
 // which is the whole grid (WHOLE).
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE>
+// TAPE: `out` holds K consecutive fields and the state after every step is
+// stored (the tile interior is valid after each of the K steps).
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false>
 __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restrict__ out, const Params &p, int gx0,
                                             int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex) {
     constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
@@ -103,6 +105,22 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
 #pragma unroll
         for(int c = 0; c < C; c++) psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
     }
+
+    // the tile interior (the whole grid in WHOLE mode) of psi into dst
+    auto store_interior = [&](T *__restrict__ dst) {
+#pragma unroll
+        for(int j = 0; j < RPW; j++) {
+            const int r = w * RPW + j;
+            const int gy = gy0 + r;
+            if(!(WHOLE || (r >= H && r < RH - H && gy < N))) continue;
+#pragma unroll
+            for(int c = 0; c < C; c++) {
+                const int lx = C * lane + c;
+                const int gx = gx0 + lx;
+                if(WHOLE || (lx >= H && lx < RW - H && gx < N)) dst[(size_t)gy * N + gx] = psi[c][j];
+            }
+        }
+    };
 
     const T Mc = (T)p.M;
 #pragma unroll 1
@@ -175,50 +193,41 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
                 }
             }
         }
-    }
-
-#pragma unroll
-    for(int j = 0; j < RPW; j++) {
-        const int r = w * RPW + j;
-        const int gy = gy0 + r;
-        if(!(WHOLE || (r >= H && r < RH - H && gy < N))) continue;
-#pragma unroll
-        for(int c = 0; c < C; c++) {
-            const int lx = C * lane + c;
-            const int gx = gx0 + lx;
-            if(WHOLE || (lx >= H && lx < RW - H && gx < N)) out[(size_t)gy * N + gx] = psi[c][j];
+        if constexpr(TAPE) {
+            if(s < K - 1) store_interior(out + (size_t)s * N * N);
         }
     }
+    store_interior(TAPE ? out + (size_t)(K - 1) * N * N : out);
 }
 
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE>
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false>
 __global__ __launch_bounds__(64 * CFG::NW) void k_stencil2d_wave(const T *__restrict__ in, T *__restrict__ out, Params p,
                                                                int tiles_x) {
     constexpr int W = CFG::RW - 4 * K, TH = CFG::RH - 4 * K;
     static_assert(WHOLE || (W > 0 && TH > 0), "tile too small for K");
     __shared__ WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> ex;
     if constexpr(WHOLE) {
-        wave_region<T, MODEL, MOB, EM, K, CFG, true>(in, out, p, 0, 0, ex);
+        wave_region<T, MODEL, MOB, EM, K, CFG, true, TAPE>(in, out, p, 0, 0, ex);
     }
     else {
         const int tile = xcd_tile(blockIdx.x, gridDim.x);
         const int tx = tile % tiles_x, ty = tile / tiles_x;
-        wave_region<T, MODEL, MOB, EM, K, CFG, false>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K, ex);
+        wave_region<T, MODEL, MOB, EM, K, CFG, false, TAPE>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K, ex);
     }
 }
 
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG>
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool TAPE = false>
 hipError_t launch_wave_cfg(const Params &p, const void *in, void *out, hipStream_t s) {
     if(CFG::RW == CFG::RH && p.N == CFG::RW) {
-        hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, true>), dim3(1), dim3(64 * CFG::NW), 0, s,
+        hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, true, TAPE>), dim3(1), dim3(64 * CFG::NW), 0, s,
                            (const T *)in, (T *)out, p, 1);
         return hipGetLastError();
     }
     constexpr int W = CFG::RW - 4 * K, TH = CFG::RH - 4 * K;
     const int tiles_x = (p.N + W - 1) / W;
     const int tiles_y = (p.N + TH - 1) / TH;
-    hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, false>), dim3(tiles_x * tiles_y), dim3(64 * CFG::NW),
-                       0, s, (const T *)in, (T *)out, p, tiles_x);
+    hipLaunchKernelGGL((k_stencil2d_wave<T, MODEL, MOB, EM, K, CFG, false, TAPE>), dim3(tiles_x * tiles_y),
+                       dim3(64 * CFG::NW), 0, s, (const T *)in, (T *)out, p, tiles_x);
     return hipGetLastError();
 }
 

@@ -6,10 +6,15 @@
 //   # step = S, t = T, size = NXxNYxNZ | obs
 // followed by the values at the default ostream precision (6 significant
 // digits), a newline every N values; VTK as STRUCTURED_POINTS ASCII.
+// output.lossless = true (no reference counterpart) writes the native values
+// with 17 significant digits instead, so a load_from restart continues from
+// the exact FP64 state (with distance_scaling_factor = 1, where the unit
+// factor is 1).
 #pragma once
 
 #include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <set>
 #include <string>
 
@@ -52,6 +57,7 @@ public:
         if(!std::filesystem::exists(_out)) critical("Output path '{}' does not exist", outp);
         if(!std::filesystem::is_directory(_out)) critical("Output path '{}' is not a directory", outp);
         _print_vtk = _config_optional_value<bool>(config, "output.print_vtk", false);
+        _lossless = _config_optional_value<bool>(config, "output.lossless", false);
         std::string trajp = _print_vtk ? _config_value<std::string>(config, "output.trajectory_path")
                                        : _config_optional_value<std::string>(config, "output.trajectory_path", outp);
         _traj = std::filesystem::path(trajp);
@@ -111,6 +117,7 @@ public:
         const int grid = nx * ny * nz;
         o << "# step = " << t << ", t = " << fmt_general(t * dt, 5) << ", size = " << nx << "x" << ny << "x" << nz
           << " | " << obs << std::endl;
+        if(_lossless) o << std::setprecision(17);
         for(int idx = 0; idx < grid; idx++) {
             if(idx > 0 && idx % ny == 0) o << std::endl;
             double v = species == -1 ? field.field_sum(idx) : field(idx, species);
@@ -155,6 +162,7 @@ private:
     int _log_n0 = 0, _traj_printed = 0;
     double _log_fact = 1.0;
     bool _print_vtk = false;
+    bool _lossless = false;
     std::filesystem::path _out, _traj;
 };
 
