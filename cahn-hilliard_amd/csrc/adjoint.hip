@@ -28,107 +28,11 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "adjoint_wave.h"
 #include "ch_internal.h"
 #include "stencil_dispatch.h"
 
 namespace chd {
-
-// d^2 f / d rho^2 in FP64 (Landau: -eps + 3 psi^2; SimpleWertheim: 2 B2 + 1/rho
-// + valence X'/X with X' = -c X^2 / (2 c rho X + 1), c = valence delta)
-template <int MODEL>
-__device__ __forceinline__ double fsecond(const Params &p, double r) {
-    if constexpr(MODEL == CH_LANDAU) {
-        return -p.eps + 3.0 * r * r;
-    }
-    else {
-        double d = 2 * p.B2 + ((r < p.reg) ? 1.0 / p.reg : 1.0 / r);
-        if(r > 0.) {
-            const double c = 0.5 * p.tvd, X = wertheim_X(p, r);
-            d += p.valence * (-c * X * X / (2.0 * c * r * X + 1.0)) / X;
-        }
-        return d;
-    }
-}
-
-// d f'/d delta (SimpleWertheim, rho > 0): valence X_delta / X with
-// X_delta = -valence rho X^2 / (2 c rho X + 1), c = valence delta
-template <int MODEL>
-__device__ __forceinline__ double dfprime_ddelta(const Params &p, double r) {
-    if constexpr(MODEL == CH_LANDAU) {
-        return 0.0;
-    }
-    else {
-        if(!(r > 0.)) return 0.0;
-        const double c = 0.5 * p.tvd, X = wertheim_X(p, r);
-        return -(double)p.valence * p.valence * r * X / (2.0 * c * r * X + 1.0);
-    }
-}
-
-// d f/d delta (SimpleWertheim, rho > 0): valence rho X_delta (1/X - 1/2)
-template <int MODEL>
-__device__ __forceinline__ double dbulk_ddelta(const Params &p, double r) {
-    if constexpr(MODEL == CH_LANDAU) {
-        return 0.0;
-    }
-    else {
-        if(!(r > 0.)) return 0.0;
-        const double c = 0.5 * p.tvd, X = wertheim_X(p, r);
-        const double Xd = -(double)p.valence * r * X * X / (2.0 * c * r * X + 1.0);
-        return p.valence * r * Xd * (1.0 / X - 0.5);
-    }
-}
-
-// mobility(rho) / M, d mobility / d rho and d mobility / d delta (FP64)
-template <int MOB>
-struct MobDeriv {
-    double hat, drho, ddelta;
-};
-
-template <int MOB>
-__device__ __forceinline__ MobDeriv<MOB> mob_deriv(const Params &p, double r) {
-    MobDeriv<MOB> m{1.0, 0.0, 0.0};
-    if constexpr(MOB == CH_MOB_FREE_ENERGY) {
-        const double c = 0.5 * p.tvd, X = wertheim_X(p, r), v = p.valence;
-        const double den = 2.0 * c * r * X + 1.0;
-        const double Xv1 = pow(X, v - 1.0);
-        m.hat = r * Xv1 * X;
-        m.drho = p.M * (Xv1 * X + r * v * Xv1 * (-c * X * X / den));
-        m.ddelta = p.M * r * v * Xv1 * (-v * r * X * X / den);
-    }
-    else if constexpr(MOB == CH_MOB_REGULARISED) {
-        m.hat = r / (r + p.rho_min);
-        m.drho = p.M * p.rho_min / ((r + p.rho_min) * (r + p.rho_min));
-    }
-    else if constexpr(MOB == CH_MOB_LINEAR) {
-        m.hat = r / p.rho_ref;
-        m.drho = p.M / p.rho_ref;
-    }
-    return m;
-}
-
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-    for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// adds the block's sums of the kGradSlots per-thread values to its own slots
-template <int NT>
-__device__ __forceinline__ void block_accumulate(const double (&g)[kGradSlots], double (&red)[kGradSlots][NT / 64],
-                                                 double *__restrict__ slots) {
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for(int i = 0; i < kGradSlots; i++) {
-        const double v = wave_sum64(g[i]);
-        if((threadIdx.x & 63) == 0) red[i][w] = v;
-    }
-    __syncthreads();
-    if(threadIdx.x < kGradSlots) {
-        double a = 0;
-        for(int i = 0; i < NT / 64; i++) a += red[threadIdx.x][i];
-        slots[threadIdx.x] += a;
-    }
-}
 
 // slots: <nu, psi>, <nu, L psi>, the M term, the delta term (nu = w for euler;
 // the host applies the factors dt, M)
@@ -447,11 +351,52 @@ static hipError_t adj_em_tiles(const Params &p, const void *li, void *lo, const 
     }
 }
 
+// 2D euler with N >= 64: the register/DPP adjoint on the forward's default
+// region shape for this N
+template <class CFG>
+static int wave_adj_tiles(int N, int KA) {
+    if(CFG::RW == CFG::RH && N == CFG::RW) return 1;
+    const int W = CFG::RW - 4 * KA, TH = CFG::RH - 4 * KA;
+    return ((N + W - 1) / W) * ((N + TH - 1) / TH);
+}
+
+template <typename T, int MODEL, int KA, class CFG>
+static hipError_t launch_adj_wave(const Params &p, const void *li, void *lo, const void *tape, int s_top, double *gacc,
+                                  hipStream_t s) {
+    constexpr int NT = 64 * CFG::NW;
+    if(CFG::RW == CFG::RH && p.N == CFG::RW) {
+        hipLaunchKernelGGL((k_adjoint2d_wave<T, MODEL, KA, CFG, true>), dim3(1), dim3(NT), 0, s, (const T *)li, (T *)lo,
+                           (const T *)tape, s_top, p, 1, gacc);
+        return hipGetLastError();
+    }
+    constexpr int W = CFG::RW - 4 * KA;
+    const int tiles_x = (p.N + W - 1) / W;
+    hipLaunchKernelGGL((k_adjoint2d_wave<T, MODEL, KA, CFG, false>), dim3(wave_adj_tiles<CFG>(p.N, KA)), dim3(NT), 0, s,
+                       (const T *)li, (T *)lo, (const T *)tape, s_top, p, tiles_x, gacc);
+    return hipGetLastError();
+}
+
+template <typename T, int MODEL>
+static hipError_t adj_wave(const Params &p, int KA, const void *li, void *lo, const void *tape, int s_top,
+                           double *gacc, hipStream_t s) {
+    if(wave_default_cfg(p.N) == 5) {
+        if(KA == 4) return launch_adj_wave<T, MODEL, 4, WaveCfg5>(p, li, lo, tape, s_top, gacc, s);
+        return launch_adj_wave<T, MODEL, 1, WaveCfg5>(p, li, lo, tape, s_top, gacc, s);
+    }
+    if(KA == 4) return launch_adj_wave<T, MODEL, 4, WaveCfg4>(p, li, lo, tape, s_top, gacc, s);
+    return launch_adj_wave<T, MODEL, 1, WaveCfg4>(p, li, lo, tape, s_top, gacc, s);
+}
+
 int adjoint_steps_per_launch(int dims, int N, bool em) { return (!em && dims <= 2 && N >= 64) ? 4 : 1; }
 
-// number of tiles (gradient accumulator slots) of the adjoint launches
+// number of tiles (gradient accumulator slots) of the adjoint launches: the
+// most any launch depth of this shape uses
 int adjoint_tiles(int dims, int N, bool em) {
     if(dims == 1) return N >= 64 ? N / 64 : (N >= 16 ? N / 16 : N / 4);
+    if(dims == 2 && !em && N >= 64) {
+        if(wave_default_cfg(N) == 5) return std::max(wave_adj_tiles<WaveCfg5>(N, 4), wave_adj_tiles<WaveCfg5>(N, 1));
+        return std::max(wave_adj_tiles<WaveCfg4>(N, 4), wave_adj_tiles<WaveCfg4>(N, 1));
+    }
     if(dims == 2) return N >= 64 ? (N / 64) * (N / 16) : (N >= 16 ? (N / 16) * (N / 16) : (N / 4) * (N / 4));
     if(em) return N >= 16 ? (N / 16) * (N / 4) * (N / 4) : (N / 4) * (N / 4) * (N / 4);
     return N >= 16 ? (N / 16) * (N / 8) * (N / 8) : (N / 4) * (N / 4) * (N / 4);
@@ -494,6 +439,10 @@ hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bo
                     p, lam_in, lam_out, psi, gacc, s);
             });
         }
+        if(p.dims == 2 && p.N >= 64) {
+            if(model == CH_LANDAU) return adj_wave<T, CH_LANDAU>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
+            return adj_wave<T, CH_SIMPLE_WERTHEIM>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
+        }
         return by_dims_model_mob<T>(p.dims, model, CH_MOB_CONSTANT, [&](auto dt, auto mt, auto) {
             return adj_tiles<T, decltype(dt)::value, decltype(mt)::value>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
         });
@@ -519,8 +468,14 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
     const int N = p.N;
     if constexpr(DIMS == 2) {
         if(N >= 64) {
-            if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg0, true>(p, in, out, s);
-            if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg0, true>(p, in, out, s);
+            if(wave_default_cfg(N) == 5) {
+                if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg5, true>(p, in, out, s);
+                if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg5, true>(p, in, out, s);
+            }
+            else {
+                if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg4, true>(p, in, out, s);
+                if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg4, true>(p, in, out, s);
+            }
             return hipErrorInvalidValue;
         }
     }

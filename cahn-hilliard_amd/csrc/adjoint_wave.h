@@ -1,0 +1,159 @@
+// adjoint_wave.h -- register/DPP reverse sweep of the 2D euler integrator.
+//
+// The transpose of psi_{n+1} = psi_n + dt M L(f'(psi_n) - 2k L psi_n) is
+//     w = L lambda_{n+1},  lambda_n = lambda_{n+1} + dt M (f''(psi_n) w - 2k L w)
+// with the gradient terms <w, psi_n>, <w, L psi_n>, <w, mu_n>, <w, d f'/d delta>
+// summed over the tile interior.  Same geometry as the forward wave kernel
+// (stencil_wave.h): lane l owns C adjacent columns, wave w owns RPW rows, x
+// neighbours through DPP wave shifts, y neighbours from registers and, for the
+// first / last row of a wave, through LDS.  KA backward steps run per launch on
+// a region carrying a 2 KA halo of lambda; psi_n comes from the tape, the next
+// step's rows fetched while the current step computes.
+#pragma once
+
+#include "adjoint_math.h"
+#include "stencil_wave.h"
+
+namespace chd {
+
+template <typename T, int MODEL, int KA, class CFG, bool WHOLE>
+__global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__restrict__ lam_in, T *__restrict__ lam_out,
+                                                               const T *__restrict__ tape, int s_top, Params p,
+                                                               int tiles_x, double *__restrict__ gacc) {
+    constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
+    constexpr int H = WHOLE ? 0 : 2 * KA;
+    constexpr int W = RW - 4 * KA, TH = RH - 4 * KA;
+    static_assert(WHOLE || (W > 0 && TH > 0), "region too small for KA");
+    __shared__ WaveExchange<T, 3, NW, C> ex;  // lambda, psi, w
+    __shared__ double red[kGradSlots][NW];
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int N = p.N, mask = p.mask;
+    int tile = 0, gx0 = 0, gy0 = 0;
+    if constexpr(!WHOLE) {
+        tile = xcd_tile(blockIdx.x, gridDim.x);
+        gx0 = (tile % tiles_x) * W - H;
+        gy0 = (tile / tiles_x) * TH - H;
+    }
+    const int wa = WHOLE ? (w + NW - 1) % NW : w - 1;
+    const int wb = WHOLE ? (w + 1) % NW : w + 1;
+    const bool has_a = WHOLE || w > 0, has_b = WHOLE || w < NW - 1;
+    const size_t NN = (size_t)N * N;
+
+    auto load = [&](const T *__restrict__ f, T(&dst)[C][RPW]) {
+#pragma unroll
+        for(int j = 0; j < RPW; j++) {
+            const size_t row = (size_t)((gy0 + w * RPW + j) & mask) * N;
+#pragma unroll
+            for(int c = 0; c < C; c++) dst[c][j] = f[row + ((gx0 + C * lane + c) & mask)];
+        }
+    };
+    // the tile interior owns each grid cell once
+    auto row_in = [&](int j) {
+        const int r = w * RPW + j;
+        return WHOLE || (r >= H && r < RH - H && gy0 + r < N);
+    };
+    auto col_in = [&](int c) {
+        const int lx = C * lane + c;
+        return WHOLE || (lx >= H && lx < RW - H && gx0 + lx < N);
+    };
+
+    T lam[C][RPW], psi[C][RPW], nxt[C][RPW], wv[C][RPW];
+    load(lam_in, lam);
+    load(tape + (size_t)s_top * NN, nxt);
+    const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
+    double ge = 0, gk = 0, gm = 0, gd = 0;
+#pragma unroll 1
+    for(int st = 0; st < KA; st++) {
+#pragma unroll
+        for(int j = 0; j < RPW; j++)
+#pragma unroll
+            for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
+        if(st + 1 < KA) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
+
+        // ---- phase 1: w = L lambda, gradient terms
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            ex.first[0][w][c][lane] = lam[c][0];
+            ex.last[0][w][c][lane] = lam[c][RPW - 1];
+            ex.first[1][w][c][lane] = psi[c][0];
+            ex.last[1][w][c][lane] = psi[c][RPW - 1];
+        }
+        __syncthreads();
+        T la[C], lb[C], pa[C], pb[C];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            la[c] = has_a ? ex.last[0][wa][c][lane] : lam[c][0];
+            lb[c] = has_b ? ex.first[0][wb][c][lane] : lam[c][RPW - 1];
+            pa[c] = has_a ? ex.last[1][wa][c][lane] : psi[c][0];
+            pb[c] = has_b ? ex.first[1][wb][c][lane] : psi[c][RPW - 1];
+        }
+#pragma unroll
+        for(int j = 0; j < RPW; j++) {
+            T m[C][2], q[C][2], pm[C][2], pq[C][2];
+            row_neighbours<T, C, RPW, WHOLE>(lam, j, la, lb, m, q);
+            row_neighbours<T, C, RPW, WHOLE>(psi, j, pa, pb, pm, pq);
+            const bool rin = row_in(j);
+#pragma unroll
+            for(int c = 0; c < C; c++) {
+                wv[c][j] = laplacian<T, 2>(lam[c][j], m[c], q[c], inv_dx2);
+                if(rin && col_in(c)) {
+                    const double wd = (double)wv[c][j], ps = (double)psi[c][j];
+                    const double lps = (double)laplacian<T, 2>(psi[c][j], pm[c], pq[c], inv_dx2);
+                    ge += wd * ps;
+                    gk += wd * lps;
+                    gm += wd * ((double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * lps);
+                    if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
+                }
+            }
+        }
+        // ---- phase 2: lambda += dt M (f''(psi) w - 2k L w)
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            ex.first[2][w][c][lane] = wv[c][0];
+            ex.last[2][w][c][lane] = wv[c][RPW - 1];
+        }
+        __syncthreads();
+        T wa_[C], wb_[C];
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            wa_[c] = has_a ? ex.last[2][wa][c][lane] : wv[c][0];
+            wb_[c] = has_b ? ex.first[2][wb][c][lane] : wv[c][RPW - 1];
+        }
+#pragma unroll
+        for(int j = 0; j < RPW; j++) {
+            T m[C][2], q[C][2];
+            row_neighbours<T, C, RPW, WHOLE>(wv, j, wa_, wb_, m, q);
+#pragma unroll
+            for(int c = 0; c < C; c++) {
+                const T lw = laplacian<T, 2>(wv[c][j], m[c], q[c], inv_dx2);
+                const T fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
+                lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
+            }
+        }
+    }
+
+#pragma unroll
+    for(int j = 0; j < RPW; j++) {
+        if(!row_in(j)) continue;
+        const size_t row = (size_t)(gy0 + w * RPW + j) * N;
+#pragma unroll
+        for(int c = 0; c < C; c++)
+            if(col_in(c)) lam_out[row + gx0 + C * lane + c] = lam[c][j];
+    }
+
+    const double g[kGradSlots] = {ge, gk, gm, gd};
+#pragma unroll
+    for(int i = 0; i < kGradSlots; i++) {
+        const double v = wave_sum64(g[i]);
+        if(lane == 0) red[i][w] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < kGradSlots) {
+        double a = 0;
+        for(int i = 0; i < NW; i++) a += red[threadIdx.x][i];
+        gacc[kGradSlots * tile + threadIdx.x] += a;
+    }
+}
+
+}  // namespace chd

@@ -17,10 +17,9 @@
 
 #include "ch_internal.h"
 
-// default temporal-blocking depth (time steps per stencil launch) and region
-// shape of the 2D wave kernel
-static constexpr int kDefaultTB = 4;
-static constexpr int kDefaultWaveCfg = 2;
+// default temporal-blocking depth (time steps per stencil launch); the 2D wave
+// kernel's region shape defaults to chd::wave_default_cfg(N)
+static constexpr int kDefaultTB = 8;
 
 struct ch_ctx {
     int device = 0;
@@ -176,7 +175,7 @@ static chd::Params make_params(const ch_desc *d) {
     p.inv_dx2 = 1.0 / (d->dx * d->dx);
     p.dt = d->dt;
     p.Mdt = d->M * d->dt;
-    p.wave_cfg = kDefaultWaveCfg;
+    p.wave_cfg = chd::wave_default_cfg(d->N);
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;

@@ -166,6 +166,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 
 // number of 2D wave-kernel region shapes selectable through Params::wave_cfg
 constexpr int kWaveCfgs = 7;
+// default shape: 128 x 64 regions of 16 waves (2 columns per lane) for N >= 512,
+// 64 x 64 regions of 16 waves below, where more, smaller blocks fill the CUs
+// (measured on MI355X over shapes 0-6 at 8 steps per launch)
+__host__ __device__ constexpr int wave_default_cfg(int N) { return N >= 512 ? 5 : 4; }
 
 template <typename T> struct Complex;
 template <> struct Complex<double> { using type = double2; };

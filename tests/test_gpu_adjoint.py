@@ -33,7 +33,8 @@ def gpu_adjoint(ch, ctx, kw, psi0, steps, w, ckpt=100, precision="fp64", eps=0.9
     return psiT, lam, g
 
 
-@pytest.mark.parametrize("dims,N,steps,ckpt", [(2, 64, 60, 16), (2, 32, 40, 0), (1, 128, 80, 25), (3, 16, 30, 7)])
+@pytest.mark.parametrize("dims,N,steps,ckpt", [(2, 64, 60, 16), (2, 32, 40, 0), (1, 128, 80, 25), (3, 16, 30, 7),
+                                              (2, 128, 45, 10), (2, 256, 30, 7), (2, 512, 21, 0)])
 def test_adjoint_against_oracle(ch, ctx, dims, N, steps, ckpt):
     kw = dict(dims=dims, N=N, dt=0.01 if dims < 3 else 0.002, dx=1.0, k=1.2, M=0.9)
     psi0 = O.init_density(dims, N, 0.0, 0.3, 0, 123456)
