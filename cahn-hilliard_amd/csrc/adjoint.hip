@@ -376,26 +376,60 @@ static hipError_t launch_adj_wave(const Params &p, const void *li, void *lo, con
     return hipGetLastError();
 }
 
+template <typename T, int MODEL, class CFG>
+static hipError_t adj_wave_cfg(const Params &p, int KA, const void *li, void *lo, const void *tape, int s_top,
+                               double *gacc, hipStream_t s) {
+    switch(KA) {
+    case 1: return launch_adj_wave<T, MODEL, 1, CFG>(p, li, lo, tape, s_top, gacc, s);
+    case 2: return launch_adj_wave<T, MODEL, 2, CFG>(p, li, lo, tape, s_top, gacc, s);
+    case 4: return launch_adj_wave<T, MODEL, 4, CFG>(p, li, lo, tape, s_top, gacc, s);
+    case 8: return launch_adj_wave<T, MODEL, 8, CFG>(p, li, lo, tape, s_top, gacc, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename T, int MODEL>
 static hipError_t adj_wave(const Params &p, int KA, const void *li, void *lo, const void *tape, int s_top,
                            double *gacc, hipStream_t s) {
-    if(wave_default_cfg(p.N) == 5) {
-        if(KA == 4) return launch_adj_wave<T, MODEL, 4, WaveCfg5>(p, li, lo, tape, s_top, gacc, s);
-        return launch_adj_wave<T, MODEL, 1, WaveCfg5>(p, li, lo, tape, s_top, gacc, s);
-    }
-    if(KA == 4) return launch_adj_wave<T, MODEL, 4, WaveCfg4>(p, li, lo, tape, s_top, gacc, s);
-    return launch_adj_wave<T, MODEL, 1, WaveCfg4>(p, li, lo, tape, s_top, gacc, s);
+    if(wave_default_cfg(p.N) == 5) return adj_wave_cfg<T, MODEL, WaveCfg5>(p, KA, li, lo, tape, s_top, gacc, s);
+    return adj_wave_cfg<T, MODEL, WaveCfg4>(p, KA, li, lo, tape, s_top, gacc, s);
 }
 
-int adjoint_steps_per_launch(int dims, int N, bool em) { return (!em && dims <= 2 && N >= 64) ? 4 : 1; }
+static bool adj_is_wave(int dims, int N, bool em) { return dims == 2 && !em && N >= 64; }
+static bool tape_is_wave(int dims, int N) { return dims == 2 && N >= 64; }
+
+// backward steps of the next adjoint launch when `remaining` are left: the
+// wave kernel has depths 8, 4, 2, 1; the LDS kernel 4 (1D / 2D euler, N >= 64)
+// and 1
+int adjoint_depth(int dims, int N, bool em, int64_t remaining) {
+    if(adj_is_wave(dims, N, em)) {
+        int k = 8;
+        while(k > remaining) k /= 2;
+        return k;
+    }
+    return (!em && dims <= 2 && N >= 64 && remaining >= 4) ? 4 : 1;
+}
+
+// states of the next recompute launch when `remaining` are left: wave kernel
+// 8, 4, 2, 1; the LDS kernel 4 (1D euler, N >= 64) and 1
+int tape_depth(int dims, int N, bool em, int64_t remaining) {
+    if(tape_is_wave(dims, N)) {
+        int k = 8;
+        while(k > remaining) k /= 2;
+        return k;
+    }
+    return (dims == 1 && !em && N >= 64 && remaining >= 4) ? 4 : 1;
+}
 
 // number of tiles (gradient accumulator slots) of the adjoint launches: the
 // most any launch depth of this shape uses
 int adjoint_tiles(int dims, int N, bool em) {
     if(dims == 1) return N >= 64 ? N / 64 : (N >= 16 ? N / 16 : N / 4);
-    if(dims == 2 && !em && N >= 64) {
-        if(wave_default_cfg(N) == 5) return std::max(wave_adj_tiles<WaveCfg5>(N, 4), wave_adj_tiles<WaveCfg5>(N, 1));
-        return std::max(wave_adj_tiles<WaveCfg4>(N, 4), wave_adj_tiles<WaveCfg4>(N, 1));
+    if(adj_is_wave(dims, N, em)) {
+        int m = 0;
+        for(int k = 1; k <= 8; k *= 2)
+            m = std::max(m, wave_default_cfg(N) == 5 ? wave_adj_tiles<WaveCfg5>(N, k) : wave_adj_tiles<WaveCfg4>(N, k));
+        return m;
     }
     if(dims == 2) return N >= 64 ? (N / 64) * (N / 16) : (N >= 16 ? (N / 16) * (N / 16) : (N / 4) * (N / 4));
     if(em) return N >= 16 ? (N / 16) * (N / 4) * (N / 4) : (N / 4) * (N / 4) * (N / 4);
@@ -439,10 +473,11 @@ hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bo
                     p, lam_in, lam_out, psi, gacc, s);
             });
         }
-        if(p.dims == 2 && p.N >= 64) {
+        if(adj_is_wave(p.dims, p.N, em)) {
             if(model == CH_LANDAU) return adj_wave<T, CH_LANDAU>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
             return adj_wave<T, CH_SIMPLE_WERTHEIM>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
         }
+        if(KA != 1 && !(KA == 4 && p.dims <= 2 && p.N >= 64)) return hipErrorInvalidValue;
         return by_dims_model_mob<T>(p.dims, model, CH_MOB_CONSTANT, [&](auto dt, auto mt, auto) {
             return adj_tiles<T, decltype(dt)::value, decltype(mt)::value>(p, KA, lam_in, lam_out, tape, s_top, gacc, s);
         });
@@ -461,22 +496,26 @@ static hipError_t launch_tape(const Params &p, const void *in, void *tape_out, h
     return hipGetLastError();
 }
 
+template <typename T, int MODEL, int MOB, bool EM, class CFG>
+static hipError_t tape_wave(const Params &p, int K, const void *in, void *out, hipStream_t s) {
+    switch(K) {
+    case 1: return launch_wave_cfg<T, MODEL, MOB, EM, 1, CFG, true>(p, in, out, s);
+    case 2: return launch_wave_cfg<T, MODEL, MOB, EM, 2, CFG, true>(p, in, out, s);
+    case 4: return launch_wave_cfg<T, MODEL, MOB, EM, 4, CFG, true>(p, in, out, s);
+    case 8: return launch_wave_cfg<T, MODEL, MOB, EM, 8, CFG, true>(p, in, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 // 2D grids with N >= 64 recompute with the register/DPP wave kernel (the
-// forward's kernel family: same bits), K = 1 or 4 states per launch
+// forward's kernel family: same bits), 1, 2, 4 or 8 states per launch
 template <typename T, int DIMS, int MODEL, int MOB, bool EM>
 static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, hipStream_t s) {
     const int N = p.N;
     if constexpr(DIMS == 2) {
         if(N >= 64) {
-            if(wave_default_cfg(N) == 5) {
-                if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg5, true>(p, in, out, s);
-                if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg5, true>(p, in, out, s);
-            }
-            else {
-                if(K == 4) return launch_wave_cfg<T, MODEL, MOB, EM, 4, WaveCfg4, true>(p, in, out, s);
-                if(K == 1) return launch_wave_cfg<T, MODEL, MOB, EM, 1, WaveCfg4, true>(p, in, out, s);
-            }
-            return hipErrorInvalidValue;
+            if(wave_default_cfg(N) == 5) return tape_wave<T, MODEL, MOB, EM, WaveCfg5>(p, K, in, out, s);
+            return tape_wave<T, MODEL, MOB, EM, WaveCfg4>(p, K, in, out, s);
         }
     }
     if constexpr(DIMS == 1 && !EM) {
@@ -497,8 +536,6 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
         return launch_tape<T, 3, MODEL, MOB, EM, 4, 4, 4, 1>(p, in, out, s);
     }
 }
-
-int tape_steps_per_launch(int dims, int N, bool em) { return (N >= 64 && (dims == 2 || (dims == 1 && !em))) ? 4 : 1; }
 
 hipError_t tape_launch(const Params &p, int precision, int model, int mob, bool em, int K, const void *in,
                        void *tape_out, hipStream_t s) {

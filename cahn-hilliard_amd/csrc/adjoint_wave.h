@@ -3,7 +3,9 @@
 // The transpose of psi_{n+1} = psi_n + dt M L(f'(psi_n) - 2k L psi_n) is
 //     w = L lambda_{n+1},  lambda_n = lambda_{n+1} + dt M (f''(psi_n) w - 2k L w)
 // with the gradient terms <w, psi_n>, <w, L psi_n>, <w, mu_n>, <w, d f'/d delta>
-// summed over the tile interior.  Same geometry as the forward wave kernel
+// summed over the tile interior.  L is symmetric on the periodic grid, so
+// <w, L psi> = <L w, psi> and <w, mu> = <w, f'(psi)> - 2k <L w, psi>: the terms
+// use the L w of the update and need no Laplacian of psi.  Same geometry as the forward wave kernel
 // (stencil_wave.h): lane l owns C adjacent columns, wave w owns RPW rows, x
 // neighbours through DPP wave shifts, y neighbours from registers and, for the
 // first / last row of a wave, through LDS.  KA backward steps run per launch on
@@ -24,7 +26,7 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
     constexpr int H = WHOLE ? 0 : 2 * KA;
     constexpr int W = RW - 4 * KA, TH = RH - 4 * KA;
     static_assert(WHOLE || (W > 0 && TH > 0), "region too small for KA");
-    __shared__ WaveExchange<T, 3, NW, C> ex;  // lambda, psi, w
+    __shared__ WaveExchange<T, 2, NW, C> ex;  // lambda, w
     __shared__ double red[kGradSlots][NW];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -71,63 +73,55 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
             for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
         if(st + 1 < KA) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
 
-        // ---- phase 1: w = L lambda, gradient terms
+        // ---- phase 1: w = L lambda
 #pragma unroll
         for(int c = 0; c < C; c++) {
             ex.first[0][w][c][lane] = lam[c][0];
             ex.last[0][w][c][lane] = lam[c][RPW - 1];
-            ex.first[1][w][c][lane] = psi[c][0];
-            ex.last[1][w][c][lane] = psi[c][RPW - 1];
         }
         __syncthreads();
-        T la[C], lb[C], pa[C], pb[C];
+        T la[C], lb[C];
 #pragma unroll
         for(int c = 0; c < C; c++) {
             la[c] = has_a ? ex.last[0][wa][c][lane] : lam[c][0];
             lb[c] = has_b ? ex.first[0][wb][c][lane] : lam[c][RPW - 1];
-            pa[c] = has_a ? ex.last[1][wa][c][lane] : psi[c][0];
-            pb[c] = has_b ? ex.first[1][wb][c][lane] : psi[c][RPW - 1];
         }
 #pragma unroll
         for(int j = 0; j < RPW; j++) {
-            T m[C][2], q[C][2], pm[C][2], pq[C][2];
+            T m[C][2], q[C][2];
             row_neighbours<T, C, RPW, WHOLE>(lam, j, la, lb, m, q);
-            row_neighbours<T, C, RPW, WHOLE>(psi, j, pa, pb, pm, pq);
-            const bool rin = row_in(j);
 #pragma unroll
-            for(int c = 0; c < C; c++) {
-                wv[c][j] = laplacian<T, 2>(lam[c][j], m[c], q[c], inv_dx2);
-                if(rin && col_in(c)) {
-                    const double wd = (double)wv[c][j], ps = (double)psi[c][j];
-                    const double lps = (double)laplacian<T, 2>(psi[c][j], pm[c], pq[c], inv_dx2);
-                    ge += wd * ps;
-                    gk += wd * lps;
-                    gm += wd * ((double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * lps);
-                    if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
-                }
-            }
+            for(int c = 0; c < C; c++) wv[c][j] = laplacian<T, 2>(lam[c][j], m[c], q[c], inv_dx2);
         }
-        // ---- phase 2: lambda += dt M (f''(psi) w - 2k L w)
+        // ---- phase 2: lambda += dt M (f''(psi) w - 2k L w), gradient terms
 #pragma unroll
         for(int c = 0; c < C; c++) {
-            ex.first[2][w][c][lane] = wv[c][0];
-            ex.last[2][w][c][lane] = wv[c][RPW - 1];
+            ex.first[1][w][c][lane] = wv[c][0];
+            ex.last[1][w][c][lane] = wv[c][RPW - 1];
         }
         __syncthreads();
         T wa_[C], wb_[C];
 #pragma unroll
         for(int c = 0; c < C; c++) {
-            wa_[c] = has_a ? ex.last[2][wa][c][lane] : wv[c][0];
-            wb_[c] = has_b ? ex.first[2][wb][c][lane] : wv[c][RPW - 1];
+            wa_[c] = has_a ? ex.last[1][wa][c][lane] : wv[c][0];
+            wb_[c] = has_b ? ex.first[1][wb][c][lane] : wv[c][RPW - 1];
         }
 #pragma unroll
         for(int j = 0; j < RPW; j++) {
             T m[C][2], q[C][2];
             row_neighbours<T, C, RPW, WHOLE>(wv, j, wa_, wb_, m, q);
+            const bool rin = row_in(j);
 #pragma unroll
             for(int c = 0; c < C; c++) {
                 const T lw = laplacian<T, 2>(wv[c][j], m[c], q[c], inv_dx2);
                 const T fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
+                if(rin && col_in(c)) {
+                    const double wd = (double)wv[c][j], ps = (double)psi[c][j], lwd = (double)lw;
+                    ge += wd * ps;
+                    gk += lwd * ps;
+                    gm += wd * (double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * (lwd * ps);
+                    if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
+                }
                 lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
             }
         }

@@ -45,6 +45,8 @@ struct ch_sys {
     int64_t ckpt_cap = 0, tape_cap = 0;  // capacities in fields
     int64_t rec_steps = 0, rec_window = 0;
     bool rec_valid = false, rec_full_tape = false;
+    hipStream_t aux = nullptr;  // window recompute, overlapped with the adjoint
+    hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -213,6 +215,12 @@ static void twiddle(int m, int N, long double &c, long double &s) {
 }
 
 static void free_sys(ch_sys *s) {
+    if(s->aux) {
+        (void)hipStreamSynchronize(s->aux);
+        (void)hipStreamDestroy(s->aux);
+    }
+    for(hipEvent_t e : {s->ev_start, s->ev_rec[0], s->ev_rec[1], s->ev_adj[0], s->ev_adj[1]})
+        if(e) (void)hipEventDestroy(e);
     for(int i = 0; i < 2; i++) {
         if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
@@ -808,15 +816,14 @@ static bool grow(void **buf, int64_t &cap, int64_t fields, size_t field_bytes) {
 
 static char *field_at(void *base, int64_t i, size_t field_bytes) { return (char *)base + (size_t)i * field_bytes; }
 
-// K steps from tape slot `from` into slots from+1 .. from+K
-static hipError_t tape_steps(ch_sys *s, int64_t from, int64_t count) {
+// `count` states after tape slot `from` of `tape` into slots from+1 .. from+count
+static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count, hipStream_t st) {
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
-    const int Kt = chd::tape_steps_per_launch(s->d.dims, s->d.N, em);
     while(count > 0) {
-        const int K = count >= Kt ? Kt : 1;
+        const int K = chd::tape_depth(s->d.dims, s->d.N, em, count);
         hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,
-                                        field_at(s->tape, from, fb), field_at(s->tape, from + 1, fb), s->ctx->stream);
+                                        field_at(tape, from, fb), field_at(tape, from + 1, fb), st);
         if(e != hipSuccess) return e;
         from += K;
         count -= K;
@@ -835,13 +842,15 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     s->rec_full_tape = ckpt_every == 0;
     s->rec_window = ckpt_every == 0 ? nsteps : std::min<int64_t>(ckpt_every, nsteps);
     const int64_t nck = (nsteps + s->rec_window - 1) / s->rec_window;
-    if(!grow(&s->tape, s->tape_cap, s->rec_window, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)))
+    // checkpointed: two window tapes, one recomputed while the other is swept
+    const int64_t tape_fields = s->rec_full_tape ? nsteps : 2 * s->rec_window;
+    if(!grow(&s->tape, s->tape_cap, tape_fields, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)))
         return fail(c, CH_ENOMEM, "no device memory for %lld checkpoints / a %lld-state tape", (long long)nck,
-                    (long long)s->rec_window);
+                    (long long)tape_fields);
     if(s->rec_full_tape) {
         // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
         HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, tape_steps(s, 0, nsteps - 1));
+        HIPCHK(c, tape_steps(s, s->tape, 0, nsteps - 1, c->stream));
         HIPCHK(c, chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility,
                                    s->d.integrator == CH_EULER_MOBILITY, 1, field_at(s->tape, nsteps - 1, fb),
                                    s->psi[s->cur], c->stream));
@@ -859,34 +868,53 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     return CH_OK;
 }
 
+// Reverse sweep, windows last to first.  Checkpointed records recompute window
+// k - 1 on the auxiliary stream into one tape buffer while the adjoint of
+// window k runs on the context stream from the other; events order the reuse
+// of each buffer.
 static ch_status adjoint_core(ch_sys *s, double *grads) {
     ch_ctx *c = s->ctx;
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
     const int ntiles = chd::adjoint_tiles(s->d.dims, s->d.N, em);
-    const int KA = chd::adjoint_steps_per_launch(s->d.dims, s->d.N, em);
     const int64_t W = s->rec_window, T = s->rec_steps;
     const int64_t nck = (T + W - 1) / W;
     constexpr int NS = chd::kGradSlots;
+    const bool recompute = !s->rec_full_tape;
+    hipStream_t ms = c->stream;
     int li = 0;  // lambda lives in lam[li]
-    HIPCHK(c, chd::fill_zero(s->gacc, NS * ntiles, c->stream));
+    HIPCHK(c, chd::fill_zero(s->gacc, NS * ntiles, ms));
+    if(recompute) {
+        HIPCHK(c, hipEventRecord(s->ev_start, ms));
+        HIPCHK(c, hipStreamWaitEvent(s->aux, s->ev_start, 0));
+    }
+    bool used[2] = {false, false};
     for(int64_t k = nck - 1; k >= 0; k--) {
         const int64_t start = k * W, len = std::min<int64_t>(W, T - start);
-        if(!s->rec_full_tape) {
-            HIPCHK(c, hipMemcpyAsync(s->tape, field_at(s->ckpt, k, fb), fb, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(c, tape_steps(s, 0, len - 1));
+        const int b = (int)(k & 1);
+        void *tape = recompute ? field_at(s->tape, b * W, fb) : s->tape;
+        if(recompute) {
+            if(used[b]) HIPCHK(c, hipStreamWaitEvent(s->aux, s->ev_adj[b], 0));
+            HIPCHK(c, hipMemcpyAsync(tape, field_at(s->ckpt, k, fb), fb, hipMemcpyDeviceToDevice, s->aux));
+            HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux));
+            HIPCHK(c, hipEventRecord(s->ev_rec[b], s->aux));
+            HIPCHK(c, hipStreamWaitEvent(ms, s->ev_rec[b], 0));
         }
         for(int64_t t = len - 1; t >= 0;) {
-            const int ka = (t + 1 >= KA) ? KA : 1;
+            const int ka = chd::adjoint_depth(s->d.dims, s->d.N, em, t + 1);
             HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, ka, s->lam[li],
-                                          s->lam[li ^ 1], s->tape, (int)t, s->gacc, c->stream));
+                                          s->lam[li ^ 1], tape, (int)t, s->gacc, ms));
             li ^= 1;
             t -= ka;
         }
+        if(recompute) {
+            HIPCHK(c, hipEventRecord(s->ev_adj[b], ms));
+            used[b] = true;
+        }
     }
     std::vector<double> g(NS * (size_t)ntiles);
-    HIPCHK(c, hipMemcpyAsync(g.data(), s->gacc, g.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.data(), s->gacc, g.size() * sizeof(double), hipMemcpyDeviceToHost, ms));
+    HIPCHK(c, hipStreamSynchronize(ms));
     double sum[NS] = {0, 0, 0, 0};
     for(int i = 0; i < ntiles; i++)
         for(int j = 0; j < NS; j++) sum[j] += g[NS * (size_t)i + j];
@@ -900,7 +928,7 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
         grads[CH_GRAD_DELTA] = landau ? 0.0 : f * sum[3];
         grads[CH_GRAD_B2] = landau ? 0.0 : 2.0 * f * sum[0];
     }
-    if(li == 1) HIPCHK(c, hipMemcpyAsync(s->lam[0], s->lam[1], fb, hipMemcpyDeviceToDevice, c->stream));
+    if(li == 1) HIPCHK(c, hipMemcpyAsync(s->lam[0], s->lam[1], fb, hipMemcpyDeviceToDevice, ms));
     return CH_OK;
 }
 
@@ -909,6 +937,15 @@ static ch_status adjoint_prepare(ch_sys *s) {
     if(!s->rec_valid) return fail(c, CH_ESTATE, "ch_sys_adjoint needs a ch_sys_record first");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
+    if(!s->aux) {
+        // default priority, like the context stream
+        HIPCHK(c, hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+        HIPCHK(c, hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming));
+        for(int i = 0; i < 2; i++) {
+            HIPCHK(c, hipEventCreateWithFlags(&s->ev_rec[i], hipEventDisableTiming));
+            HIPCHK(c, hipEventCreateWithFlags(&s->ev_adj[i], hipEventDisableTiming));
+        }
+    }
     int64_t cap0 = s->lam[0] ? 1 : 0, cap1 = s->lam[1] ? 1 : 0, capg = s->gacc ? 1 : 0;
     if(!grow(&s->lam[0], cap0, 1, fb) || !grow(&s->lam[1], cap1, 1, fb) ||
        !grow((void **)&s->gacc, capg, 1,

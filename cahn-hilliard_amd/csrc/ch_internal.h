@@ -60,13 +60,13 @@ hipError_t batch_observables(int precision, int model, const void *buf, const Ba
 // rule); otherwise euler with constant mobility.  Every adjoint tile owns
 // kGradSlots FP64 accumulators.
 constexpr int kGradSlots = 4;
-int adjoint_steps_per_launch(int dims, int N, bool em);
+// depth (steps) of the next adjoint / recompute launch with `remaining` steps left
+int adjoint_depth(int dims, int N, bool em, int64_t remaining);
+int tape_depth(int dims, int N, bool em, int64_t remaining);
 int adjoint_tiles(int dims, int N, bool em);
 hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bool em, int KA, const void *lam_in,
                           void *lam_out, const void *tape, int s_top, double *gacc, hipStream_t s);
-// recompute: K states from `in` into K consecutive tape fields; K is 1 or
-// tape_steps_per_launch()
-int tape_steps_per_launch(int dims, int N, bool em);
+// recompute: K states from `in` into K consecutive tape fields (K from tape_depth)
 hipError_t tape_launch(const Params &p, int precision, int model, int mob, bool em, int K, const void *in,
                        void *tape_out, hipStream_t s);
 hipError_t fill_zero(double *a, int n, hipStream_t s);
