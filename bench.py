@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
@@ -153,6 +155,7 @@ def w_batch(ch, ctx, args):
     for i, f in enumerate(fields):
         b.upload(i, f)
     cells = sum(d.N * d.N for d in descs)
+    shape, spl = b.autotune()  # untimed (it also runs by itself on the first evolve)
     b.evolve(64)
     sync(ctx)
     nst = 1000 if args.steps >= 50 else 200
@@ -161,10 +164,38 @@ def w_batch(ch, ctx, args):
     sync(ctx)
     w = time.perf_counter() - t0
     b.observables()
-    spl = b.steps_per_launch
     b.close()
+    # FP32 field error: the same replicas from the same start, FP32 vs FP64 batches
+    out = {}
+    for prec in ("fp32", "fp64"):
+        dd = [ch.make_desc(dims=2, N=d.N, integrator="euler", precision=prec, dt=0.01, dx=1.0, k=1.0,
+                           epsilon=d.epsilon) for d in descs]
+        bb = ch.Batch(ctx, dd)
+        for i, f in enumerate(fields):
+            bb.upload(i, f)
+        bb.evolve(nst)
+        out[prec] = [bb.download(i) for i in range(len(dd))]
+        bb.close()
+    err = max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(out["fp32"], out["fp64"]))
     return {"Gcell-updates/s": round(cells * nst / w / 1e9, 3), "replicas": len(descs), "cells": cells,
-            "time_steps": nst, "steps_per_launch": spl, "us_per_time_step": round(w / nst * 1e6, 3)}
+            "time_steps": nst, "autotuned": {"shape": shape, "steps_per_launch": spl},
+            "us_per_time_step": round(w / nst * 1e6, 3),
+            "fp32_rel_linf_vs_fp64": float(f"{err:.3e}"), "error_after_steps": nst}
+
+
+def fp32_error(ch, ctx, integrator, dt, N, steps):
+    """max |psi_fp32 - psi_fp64| / max |psi_fp64| after `steps` from the same start."""
+    from cahn_hilliard_amd.init import initial_density
+    rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
+    res = []
+    for prec in ("fp32", "fp64"):
+        s = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator=integrator, precision=prec, dt=dt, dx=1.0, k=1.0,
+                                        epsilon=0.9))
+        s.upload(rho0)
+        s.evolve(steps)
+        res.append(s.download())
+        s.close()
+    return float(f"{np.abs(res[0] - res[1]).max() / np.abs(res[1]).max():.3e}")
 
 
 def w_adjoint(ch, ctx, args):
@@ -185,6 +216,11 @@ def w_adjoint(ch, ctx, args):
     t_fwd = time.perf_counter() - t0
     out = {}
     for ckpt in (100, 0):
+        # an untimed first call allocates the checkpoints, tapes and the
+        # recompute stream (set-up, like an FFT plan)
+        s.upload(rho0)
+        s.record(T, ckpt)
+        s.adjoint(s.download())
         s.upload(rho0)
         sync(ctx)
         t0 = time.perf_counter()
@@ -204,16 +240,19 @@ def w_wertheim(ch, ctx, args):
     record (checkpoint every 50 steps), the free-energy gradient as the seed, the
     reverse sweep, plus the explicit dF/d delta."""
     from cahn_hilliard_amd.init import initial_density
-    N, nst = 128, 200 if args.steps >= 50 else 50
+    N, nst = 128, 2000 if args.steps >= 100 else 200
     d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
                      valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
     tot = tot_grad = 0.0
     dFdd = []
+    s = ch.System(ctx, d)
+    # untimed set-up: graphs, checkpoints, tapes and the recompute stream
+    s.upload(initial_density(3, N, 0.05, 0.05, 0, 123456))
+    s.evolve(nst)
+    s.record(nst, 50)
+    s.adjoint(s.free_energy_grad()[1])
     for r in range(8):
         rho0 = initial_density(3, N, 0.05, 0.05, 0, 123456 + r)
-        s = ch.System(ctx, d)
-        s.upload(rho0)
-        s.evolve(4)
         s.upload(rho0)
         sync(ctx)
         t0 = time.perf_counter()
@@ -228,7 +267,7 @@ def w_wertheim(ch, ctx, args):
         _, grads = s.adjoint(g)
         tot_grad += time.perf_counter() - t0
         dFdd.append(float(dp[ch.GRAD_DELTA] + grads[ch.GRAD_DELTA]))
-        s.close()
+    s.close()
     return {"Gcell-updates/s": round(8 * N ** 3 * nst / tot / 1e9, 3), "replicas": 8, "time_steps": nst,
             "dF_ddelta": [float(f"{v:.6e}") for v in dFdd],
             "fwd_plus_adjoint_over_fwd": round(tot_grad / tot, 3)}
@@ -287,9 +326,13 @@ def main():
         extra = {}
         e32 = w_euler(ch, ctx, args, "fp32")
         extra["euler_fp32"] = {"Gcell-updates/s": round(e32["gcell"], 3),
-                               "us_per_time_step": round(e32["wall"] / (e32["steps"] * args.inner) * 1e6, 3)}
+                               "us_per_time_step": round(e32["wall"] / (e32["steps"] * args.inner) * 1e6, 3),
+                               "fp32_rel_linf_vs_fp64": fp32_error(ch, ctx, "euler", 0.01, N, 1000),
+                               "error_after_steps": 1000}
         for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim"):
             extra[name] = table[name]()
+        extra["pseudospectral_fp32"]["fp32_rel_linf_vs_fp64"] = fp32_error(ch, ctx, "pseudospectral", 0.1, N, 1000)
+        extra["pseudospectral_fp32"]["error_after_steps"] = 1000
         extra["batch"]["vs_fp64_single"] = round(extra["batch"]["Gcell-updates/s"] / r["gcell"], 3)
         out["extra"] = extra
     if not args.no_cpu:

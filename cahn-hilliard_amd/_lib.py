@@ -99,6 +99,7 @@ def lib():
         "ch_batch_evolve": (ctypes.c_int, [P, I64]),
         "ch_batch_observables": (ctypes.c_int, [P, P, P]),
         "ch_batch_steps_per_launch": (ctypes.c_int32, [P]),
+        "ch_batch_autotune": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
         "ch_sys_record": (ctypes.c_int, [P, I64, I32]),
         "ch_sys_adjoint": (ctypes.c_int, [P, P, P, P]),
         "ch_sys_adjoint_device": (ctypes.c_int, [P, P, P, P]),

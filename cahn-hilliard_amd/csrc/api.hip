@@ -614,16 +614,20 @@ extern "C" ch_status ch_sys_free_energy_grad(ch_sys *s, double *F, double *dF_dp
 }
 
 // ------------------------------------------------------------------ batches
-static constexpr int kDefaultBatchTB = 4;
+// the steps-per-launch depths an item list exists for (index = log2 K)
+static constexpr int kBatchDepths = 4;
 
 struct ch_batch {
     ch_ctx *ctx = nullptr;
-    int n = 0, precision = CH_FP64, model = CH_LANDAU, tb = 1, nitems = 0, graph_steps = 0;
+    int n = 0, precision = CH_FP64, model = CH_LANDAU, graph_steps = 0;
+    int tb = 4, shape = 0;  // steps per launch and region split (autotuned)
+    bool tuned = false;     // false: tune on the first evolve
     size_t elt = 8;
     int64_t total = 0;                       // elements over all replicas
     std::vector<chd::BatchRep> host_reps;
     chd::BatchRep *reps = nullptr;
-    int2 *items = nullptr;
+    int2 *items[kBatchDepths] = {nullptr, nullptr, nullptr, nullptr};  // work list per K = 1, 2, 4, 8
+    int nitems[kBatchDepths] = {0, 0, 0, 0};
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;
     void *stage = nullptr;                   // largest replica, doubles
@@ -631,12 +635,16 @@ struct ch_batch {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
+static int log2i(int k) { return k >= 8 ? 3 : (k >= 4 ? 2 : (k >= 2 ? 1 : 0)); }
+
 static void free_batch(ch_batch *b) {
     for(int i = 0; i < 2; i++) {
         if(b->gexec[i]) (void)hipGraphExecDestroy(b->gexec[i]);
         if(b->buf[i]) (void)hipFree(b->buf[i]);
     }
-    void *bufs[] = {b->reps, b->items, b->stage, b->result};
+    for(int i = 0; i < kBatchDepths; i++)
+        if(b->items[i]) (void)hipFree(b->items[i]);
+    void *bufs[] = {b->reps, b->stage, b->result};
     for(void *x : bufs)
         if(x) (void)hipFree(x);
     delete b;
@@ -661,36 +669,49 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     b->precision = descs[0].precision;
     b->model = descs[0].model;
     b->elt = b->precision == CH_FP64 ? 8 : 4;
-    int want = kDefaultBatchTB;
-    if(const char *e = std::getenv("CH_BATCH_TB")) want = std::atoi(e);
-    b->tb = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
-    const int W = chd::batch_tile_w(b->tb), TH = chd::batch_tile_h(b->tb);
-    std::vector<int2> items;
+    // CH_BATCH_TB / CH_BATCH_SHAPE pin the configuration (no autotuning)
+    const char *etb = std::getenv("CH_BATCH_TB"), *esh = std::getenv("CH_BATCH_SHAPE");
+    if(etb || esh) {
+        b->tuned = true;
+        if(etb) {
+            const int want = std::atoi(etb);
+            b->tb = want >= 8 ? 8 : (want >= 4 ? 4 : (want >= 2 ? 2 : 1));
+        }
+        if(esh) b->shape = std::max(0, std::min(chd::kBatchShapes - 1, std::atoi(esh)));
+    }
+    std::vector<int2> items[kBatchDepths];
     int64_t maxbins = 0;
     for(int r = 0; r < n; r++) {
         chd::BatchRep rep{};
         rep.p = make_params(&descs[r]);
         rep.off = b->total;
         const int N = descs[r].N;
-        rep.whole = N == 64 ? 1 : 0;
-        rep.tiles_x = rep.whole ? 1 : (N + W - 1) / W;
-        rep.tiles = rep.whole ? 1 : rep.tiles_x * ((N + TH - 1) / TH);
-        for(int t = 0; t < rep.tiles; t++) items.push_back(int2{r, t});
+        rep.whole = N == chd::kBatchRegion ? 1 : 0;
+        for(int i = 0; i < kBatchDepths; i++) {
+            if(rep.whole) {
+                items[i].push_back(int2{r, 0});
+                continue;
+            }
+            const int T = chd::batch_tile(1 << i), nt = (N + T - 1) / T;
+            for(int ty = 0; ty < nt; ty++)
+                for(int tx = 0; tx < nt; tx++) items[i].push_back(int2{r, tx | (ty << 16)});
+        }
         b->total += (int64_t)N * N;
         maxbins = std::max<int64_t>(maxbins, (int64_t)N * N);
         b->host_reps.push_back(rep);
     }
-    b->nitems = (int)items.size();
     const char *g = std::getenv("CH_GRAPH_STEPS");
     b->graph_steps = g ? std::atoi(g) : 64;
     if(b->graph_steps < 0) b->graph_steps = 0;
-    b->graph_steps -= b->graph_steps % (2 * b->tb);
     bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
               hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
-              hipMalloc((void **)&b->items, items.size() * sizeof(int2)) == hipSuccess &&
               hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
-    if(ok) ok = hipMemcpy(b->reps, b->host_reps.data(), n * sizeof(chd::BatchRep), hipMemcpyHostToDevice) == hipSuccess &&
-                hipMemcpy(b->items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    for(int i = 0; ok && i < kBatchDepths; i++) {
+        b->nitems[i] = (int)items[i].size();
+        ok = hipMalloc((void **)&b->items[i], items[i].size() * sizeof(int2)) == hipSuccess &&
+             hipMemcpy(b->items[i], items[i].data(), items[i].size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if(ok) ok = hipMemcpy(b->reps, b->host_reps.data(), n * sizeof(chd::BatchRep), hipMemcpyHostToDevice) == hipSuccess;
     if(!ok) {
         free_batch(b);
         return fail(ctx, CH_ENOMEM, "device allocation failed for a batch of %d replicas", n);
@@ -745,18 +766,84 @@ extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host)
     return CH_OK;
 }
 
+static hipError_t batch_launch(ch_batch *b, int shape, int K, int from) {
+    const int i = log2i(K);
+    return chd::batch_launch(b->precision, b->model, shape, K, b->buf[from], b->buf[from ^ 1], b->reps, b->items[i],
+                             b->nitems[i], b->ctx->stream);
+}
+
 static hipError_t batch_step(ch_batch *b, int &cur, int K) {
-    hipError_t e = chd::batch_launch(b->precision, b->model, K, b->buf[cur], b->buf[cur ^ 1], b->reps, b->items,
-                                     b->nitems, b->ctx->stream);
+    hipError_t e = batch_launch(b, b->shape, K, cur);
     cur ^= 1;
     return e;
+}
+
+// Times every (shape, K) candidate on the batch's own data: each trial
+// launch reads the current buffer and writes the other one, so the state is
+// unchanged.  Keeps the fastest per time step.
+static ch_status batch_tune(ch_batch *b) {
+    ch_ctx *c = b->ctx;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    int bs = b->shape, bk = b->tb;
+    hipError_t err = hipSuccess;
+    for(int shape = 0; shape < chd::kBatchShapes && err == hipSuccess; shape++) {
+        for(int K : {2, 4, 8}) {
+            constexpr int warm = 2, reps = 8;
+            for(int r = 0; r < warm && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
+            if(err == hipSuccess) err = hipEventRecord(e0, c->stream);
+            for(int r = 0; r < reps && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
+            if(err == hipSuccess) err = hipEventRecord(e1, c->stream);
+            if(err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0;
+            if(err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if(err != hipSuccess) break;
+            const float per_step = ms / (reps * K);
+            if(per_step < best) {
+                best = per_step;
+                bs = shape;
+                bk = K;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if(err != hipSuccess) return fail(c, CH_EHIP, "batch autotuning failed: %s", hipGetErrorString(err));
+    if(bs != b->shape || bk != b->tb) {
+        for(int i = 0; i < 2; i++)
+            if(b->gexec[i]) {
+                (void)hipGraphExecDestroy(b->gexec[i]);
+                b->gexec[i] = nullptr;
+            }
+    }
+    b->shape = bs;
+    b->tb = bk;
+    b->tuned = true;
+    return CH_OK;
+}
+
+extern "C" ch_status ch_batch_autotune(ch_batch *b, int32_t *shape, int32_t *steps_per_launch) {
+    if(!b) return CH_EINVAL;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    ch_status st = batch_tune(b);
+    if(st != CH_OK) return st;
+    if(shape) *shape = b->shape;
+    if(steps_per_launch) *steps_per_launch = b->tb;
+    return CH_OK;
 }
 
 extern "C" ch_status ch_batch_evolve(ch_batch *b, int64_t nsteps) {
     if(!b || nsteps < 0) return CH_EINVAL;
     ch_ctx *c = b->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    const int G = b->graph_steps;
+    if(!b->tuned && nsteps > 0) {
+        ch_status st = batch_tune(b);
+        if(st != CH_OK) return st;
+    }
+    // graph chunk: a multiple of 2 tb launches' worth of steps (parity restored)
+    const int G = b->graph_steps - b->graph_steps % (2 * b->tb);
     if(G > 0 && nsteps >= G) {
         const int par = b->cur;
         if(!b->gexec[par]) {
@@ -777,7 +864,11 @@ extern "C" ch_status ch_batch_evolve(ch_batch *b, int64_t nsteps) {
         nsteps -= reps * G;
     }
     for(; nsteps >= b->tb; nsteps -= b->tb) HIPCHK(c, batch_step(b, b->cur, b->tb));
-    for(; nsteps > 0; nsteps--) HIPCHK(c, batch_step(b, b->cur, 1));
+    for(int K = b->tb / 2; K >= 1; K /= 2)
+        if(nsteps >= K) {
+            HIPCHK(c, batch_step(b, b->cur, K));
+            nsteps -= K;
+        }
     return CH_OK;
 }
 

@@ -44,14 +44,14 @@ constexpr int kObsParts = 1024;
 struct BatchRep {
     Params p;
     int64_t off;   // element offset of the replica inside each ping-pong buffer
-    int tiles_x;   // halo tiles per row (halo mode)
-    int tiles;     // halo tiles in total (1 in whole mode)
     int whole;     // 1: N == 64, one periodic item
 };
-int batch_tile_w(int K);
-int batch_tile_h(int K);
-hipError_t batch_launch(int precision, int model, int K, const void *a, void *b, const BatchRep *reps, const int2 *items,
-                        int nitems, hipStream_t s);
+// batch regions are kBatchRegion x kBatchRegion cells; interior side of a K-step item
+constexpr int kBatchRegion = 64;
+constexpr int kBatchShapes = 3;
+int batch_tile(int K);
+hipError_t batch_launch(int precision, int model, int shape, int K, const void *a, void *b, const BatchRep *reps,
+                        const int2 *items, int nitems, hipStream_t s);
 // per-replica FP64 observables: result[2 r] = sum rho, result[2 r + 1] = sum f-density
 hipError_t batch_observables(int precision, int model, const void *buf, const BatchRep *reps, int n, double *result,
                              hipStream_t s);

@@ -268,6 +268,13 @@ class Batch:
     def steps_per_launch(self):
         return int(L.lib().ch_batch_steps_per_launch(self.handle))
 
+    def autotune(self):
+        """Time the kernel's region splits and launch depths on the current
+        fields; returns (shape, steps_per_launch) of the fastest."""
+        sh, k = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(L.lib().ch_batch_autotune(self.handle, ctypes.byref(sh), ctypes.byref(k)))
+        return sh.value, k.value
+
     def close(self):
         if getattr(self, "handle", None):
             L.lib().ch_batch_destroy(self.handle)

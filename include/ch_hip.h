@@ -145,6 +145,13 @@ ch_status ch_batch_evolve(ch_batch *batch, int64_t nsteps);
 /* FP64 average mass and free energy of every replica (arrays of n); synchronous */
 ch_status ch_batch_observables(ch_batch *batch, double *mass, double *free_energy);
 int32_t ch_batch_steps_per_launch(const ch_batch *batch);
+/* Autotuner (no reference counterpart): times every region split of the
+ * batch kernel (shape 0: 16 rows x 4 waves, 1: 8 x 8, 2: 4 x 16 per 64 x 64
+ * region) at 2, 4 and 8 steps per launch on the batch's current fields and
+ * keeps the fastest per time step; the fields are not changed.  Runs by
+ * itself on the first ch_batch_evolve unless the environment pins the choice
+ * (CH_BATCH_TB, CH_BATCH_SHAPE).  Synchronous; either output may be NULL. */
+ch_status ch_batch_autotune(ch_batch *batch, int32_t *shape, int32_t *steps_per_launch);
 
 /* ---- reverse mode (no reference counterpart: the reference has no
  * gradients).  The euler integrator (constant mobility) and the

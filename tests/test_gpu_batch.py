@@ -33,10 +33,13 @@ def make_replicas(ch, precision, Ns, seed=0):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
+    """Every region split (CH_BATCH_SHAPE) and launch depth (CH_BATCH_TB) gives
+    each replica the bits of the same system run alone."""
     Ns = [64, 128, 64, 256, 128, 512, 64]
     descs, fields = make_replicas(ch, precision, Ns)
-    for tb in ("1", "2", "4"):
+    for shape, tb in (("0", "1"), ("0", "2"), ("0", "4"), ("1", "4"), ("2", "4"), ("1", "8"), ("2", "8")):
         monkeypatch.setenv("CH_BATCH_TB", tb)
+        monkeypatch.setenv("CH_BATCH_SHAPE", shape)
         b = ch.Batch(ctx, descs)
         assert b.steps_per_launch == int(tb)
         for r, f in enumerate(fields):
@@ -50,7 +53,7 @@ def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
             s.evolve(77)
             ref = s.download()
             s.close()
-            assert np.array_equal(outs[r], ref), (tb, r, Ns[r])
+            assert np.array_equal(outs[r], ref), (shape, tb, r, Ns[r])
 
 
 def test_batch_against_oracle(ch, ctx):
@@ -85,3 +88,27 @@ def test_batch_rejects_unsupported(ch, ctx):
         ch.Batch(ctx, [ch.make_desc(dims=2, N=64), ch.make_desc(dims=2, N=64, precision="fp32")])
     with pytest.raises(ch.CHError):
         ch.Batch(ctx, [ch.make_desc(dims=3, N=64)])
+
+
+def test_autotune_keeps_state_and_results(ch, ctx):
+    """The autotuner's trial launches leave the fields unchanged, it picks a
+    valid configuration, and the tuned batch gives the single-system bits."""
+    Ns = [64, 256, 128]
+    descs, fields = make_replicas(ch, "fp32", Ns, seed=4)
+    b = ch.Batch(ctx, descs)
+    for r, f in enumerate(fields):
+        b.upload(r, f)
+    shape, k = b.autotune()
+    print(f"autotuned: shape {shape}, {k} steps per launch")
+    assert shape in (0, 1, 2) and k in (2, 4, 8) and b.steps_per_launch == k
+    for r, f in enumerate(fields):
+        assert np.array_equal(b.download(r), f.astype(np.float32).astype(np.float64))
+    b.evolve(45)
+    outs = [b.download(r) for r in range(len(Ns))]
+    b.close()
+    for r, (d, f) in enumerate(zip(descs, fields)):
+        s = ch.System(ctx, d)
+        s.upload(f)
+        s.evolve(45)
+        assert np.array_equal(outs[r], s.download())
+        s.close()
