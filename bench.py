@@ -183,6 +183,19 @@ def w_batch(ch, ctx, args):
             "fp32_rel_linf_vs_fp64": float(f"{err:.3e}"), "error_after_steps": nst}
 
 
+def w_copy(ch, ctx, args):
+    """Calibration for the HBM byte counters: 20 device copies of 256 MiB (one
+    read and one write of 256 MiB per dispatch, well above the 256 MB MALL)."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(20):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"bytes_per_copy": n, "copies": 20}
+
+
 def fp32_error(ch, ctx, integrator, dt, N, steps):
     """max |psi_fp32 - psi_fp64| / max |psi_fp64| after `steps` from the same start."""
     from cahn_hilliard_amd.init import initial_density
@@ -284,7 +297,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--workload", default=None,
                     choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
-                             "wertheim"])
+                             "wertheim", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -298,7 +311,7 @@ def main():
              "pseudospectral": lambda: w_pseudospectral(ch, ctx, args, "fp64"),
              "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
              "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
-             "wertheim": lambda: w_wertheim(ch, ctx, args)}
+             "wertheim": lambda: w_wertheim(ch, ctx, args), "copy": lambda: w_copy(ch, ctx, args)}
     if args.workload and args.workload != "euler":
         r = table[args.workload]()
         print(json.dumps({"workload": args.workload, "result": {k: v for k, v in r.items() if k != "wall"}}), flush=True)
