@@ -320,6 +320,17 @@ def main():
     r = w_euler(ch, ctx, args)
     N = args.N
     achieved = 16.0 * N * N * r["spl"] / r["t_launch"] / 1e9
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs of this
+    # kernel configuration (profiles/hbm_traffic.json), null for any other
+    cfg = int(os.environ.get("CH_WAVE_CFG", 5 if N >= 512 else 4))
+    traffic = None
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")) as f:
+            rec = json.load(f).get(f"euler_fp64_N{N}_K{r['spl']}_cfg{cfg}")
+        if rec:
+            traffic = rec["hbm_bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
     out = {
         "metric": "Gcell-updates/s & %HBM roofline, Landau 2D N=1024; fwd+adjoint/fwd cost",
         "value": round(r["gcell"], 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -330,7 +341,7 @@ def main():
                                f"+ FP64 mass/free-energy reductions",
                    "N": N, "dims": 2, "integrator": "euler", "time_steps_per_step": args.inner},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_stencil2d_wave (fused Euler steps)", "time_steps_per_launch": r["spl"],
                      "avg_launch_us": round(r["t_launch"] * 1e6, 3)},
         "parity": {"mass": r["mass"], "free_energy": r["fe"]},
