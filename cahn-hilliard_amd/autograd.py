@@ -2,9 +2,10 @@
 
     psi_T = EulerEvolve.apply(psi0, eps, k, M, nsteps, ckpt_every, base_desc_kwargs)
 
-Forward runs `nsteps` Euler steps on the GPU (ch_sys_record, with checkpoints);
-backward calls ch_sys_adjoint_device with the incoming dL/dpsi_T and returns
-dL/dpsi0 and dL/d(eps, k, M).  The tensors live on the GPU in the system's
+Forward runs `nsteps` steps of the selected integrator (euler by default;
+euler_mobility and pseudospectral through desc_kwargs) on the GPU
+(ch_sys_record, with checkpoints); backward calls ch_sys_adjoint_device with
+the incoming dL/dpsi_T and returns dL/dpsi0 and dL/d(eps, k, M).  The tensors live on the GPU in the system's
 precision (float64 -> FP64 fields, float32 -> FP32 fields).  This wrapper is
 for checking gradients (torch.autograd.gradcheck, composition with torch
 losses); the step and its adjoint are the HIP kernels.
@@ -57,6 +58,6 @@ class EulerEvolve(torch.autograd.Function):
 
 
 def euler_evolve(psi0, eps, k, M, nsteps, ckpt_every=100, **desc_kwargs):
-    """Differentiable nsteps of the Landau (or SimpleWertheim) Euler integrator."""
+    """Differentiable nsteps of an integrator (euler unless desc_kwargs names another)."""
     desc_kwargs.setdefault("integrator", "euler")
     return EulerEvolve.apply(psi0, eps, k, M, nsteps, ckpt_every, tuple(sorted(desc_kwargs.items())))

@@ -47,6 +47,11 @@ struct ch_sys {
     bool rec_valid = false, rec_full_tape = false;
     hipStream_t aux = nullptr;  // window recompute, overlapped with the adjoint
     hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr};
+    // pseudospectral reverse mode: checkpointed spectra, the recompute's spectral
+    // state and work spectrum, the reverse step's three spectra and two fields
+    void *ckpt_hat = nullptr, *ps_rh = nullptr, *ps_w = nullptr, *ps_spec[3] = {nullptr, nullptr, nullptr};
+    void *ps_real[2] = {nullptr, nullptr};
+    int64_t ckpt_hat_cap = 0;
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -226,7 +231,8 @@ static void free_sys(ch_sys *s) {
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
     void *bufs[] = {s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
-                    s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc};
+                    s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc, s->ckpt_hat, s->ps_rh, s->ps_w,
+                    s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1]};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
     delete s;
@@ -925,19 +931,42 @@ static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count,
 extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every) {
     if(!s || nsteps < 1 || ckpt_every < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
-    if(s->d.integrator == CH_PSEUDOSPECTRAL)
-        return fail(c, CH_EUNSUPPORTED, "the adjoint covers the euler and euler_mobility integrators");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
+    const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;
+    const size_t hb = (size_t)s->hat * 2 * s->elt;
     s->rec_valid = false;
     s->rec_full_tape = ckpt_every == 0;
     s->rec_window = ckpt_every == 0 ? nsteps : std::min<int64_t>(ckpt_every, nsteps);
     const int64_t nck = (nsteps + s->rec_window - 1) / s->rec_window;
-    // checkpointed: two window tapes, one recomputed while the other is swept
-    const int64_t tape_fields = s->rec_full_tape ? nsteps : 2 * s->rec_window;
-    if(!grow(&s->tape, s->tape_cap, tape_fields, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)))
+    // checkpointed: two window tapes, one recomputed while the other is swept;
+    // the pseudospectral full tape also keeps the final state
+    const int64_t tape_fields = s->rec_full_tape ? nsteps + (ps ? 1 : 0) : 2 * s->rec_window;
+    if(!grow(&s->tape, s->tape_cap, tape_fields, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)) ||
+       (ps && !s->rec_full_tape && !grow(&s->ckpt_hat, s->ckpt_hat_cap, nck, hb)))
         return fail(c, CH_ENOMEM, "no device memory for %lld checkpoints / a %lld-state tape", (long long)nck,
                     (long long)tape_fields);
+    if(ps) {
+        chd::SpectralArgs a = spectral_args(s);
+        if(s->rec_full_tape) {
+            // psi_0 .. psi_T on the tape, the live spectrum evolved to step T
+            HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[0], fb, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, chd::ps_steps_tape(a, c->stream, nsteps, s->tape));
+            HIPCHK(c, hipMemcpyAsync(s->psi[0], field_at(s->tape, nsteps, fb), fb, hipMemcpyDeviceToDevice, c->stream));
+        }
+        else {
+            for(int64_t k = 0; k < nck; k++) {
+                HIPCHK(c, hipMemcpyAsync(field_at(s->ckpt, k, fb), s->psi[0], fb, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(field_at(s->ckpt_hat, k, hb), s->rho_hat, hb, hipMemcpyDeviceToDevice, c->stream));
+                const int64_t steps = std::min<int64_t>(s->rec_window, nsteps - k * s->rec_window);
+                ch_status st = ch_sys_evolve(s, steps);
+                if(st != CH_OK) return st;
+            }
+        }
+        s->rec_steps = nsteps;
+        s->rec_valid = true;
+        return CH_OK;
+    }
     if(s->rec_full_tape) {
         // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
         HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
@@ -967,7 +996,9 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
     ch_ctx *c = s->ctx;
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
-    const int ntiles = chd::adjoint_tiles(s->d.dims, s->d.N, em);
+    const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;
+    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->d.dims, s->d.N, em);
+    const size_t hb = (size_t)s->hat * 2 * s->elt;
     const int64_t W = s->rec_window, T = s->rec_steps;
     const int64_t nck = (T + W - 1) / W;
     constexpr int NS = chd::kGradSlots;
@@ -987,11 +1018,30 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
         if(recompute) {
             if(used[b]) HIPCHK(c, hipStreamWaitEvent(s->aux, s->ev_adj[b], 0));
             HIPCHK(c, hipMemcpyAsync(tape, field_at(s->ckpt, k, fb), fb, hipMemcpyDeviceToDevice, s->aux));
-            HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux));
+            if(ps) {
+                // the chain from (psi, rho_hat) of checkpoint k, in scratch spectra
+                HIPCHK(c, hipMemcpyAsync(s->ps_rh, field_at(s->ckpt_hat, k, hb), hb, hipMemcpyDeviceToDevice, s->aux));
+                chd::SpectralArgs a = spectral_args(s);
+                a.rho = tape;
+                a.rho_hat = s->ps_rh;
+                a.work = s->ps_w;
+                HIPCHK(c, chd::ps_steps_tape(a, s->aux, len - 1, tape));
+            }
+            else {
+                HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux));
+            }
             HIPCHK(c, hipEventRecord(s->ev_rec[b], s->aux));
             HIPCHK(c, hipStreamWaitEvent(ms, s->ev_rec[b], 0));
         }
-        for(int64_t t = len - 1; t >= 0;) {
+        if(ps) {
+            const chd::SpectralArgs a = spectral_args(s);
+            for(int64_t t = len - 1; t >= 0; t--) {
+                HIPCHK(c, chd::ps_adjoint_step(a, s->lam[li], s->lam[li ^ 1], field_at(tape, t, fb), s->ps_spec,
+                                               s->ps_real, s->gacc, ms));
+                li ^= 1;
+            }
+        }
+        for(int64_t t = ps ? -1 : len - 1; t >= 0;) {
             const int ka = chd::adjoint_depth(s->d.dims, s->d.N, em, t + 1);
             HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, ka, s->lam[li],
                                           s->lam[li ^ 1], tape, (int)t, s->gacc, ms));
@@ -1009,7 +1059,16 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
     double sum[NS] = {0, 0, 0, 0};
     for(int i = 0; i < ntiles; i++)
         for(int j = 0; j < NS; j++) sum[j] += g[NS * (size_t)i + j];
-    if(grads) {
+    if(grads && ps) {
+        // slots: <b, rho>, k term, M term, delta term (dt is inside the spectral multipliers)
+        const bool landau = s->d.model == CH_LANDAU;
+        grads[CH_GRAD_EPSILON] = landau ? -sum[0] : 0.0;
+        grads[CH_GRAD_K] = sum[1];
+        grads[CH_GRAD_M] = sum[2];
+        grads[CH_GRAD_DELTA] = landau ? 0.0 : sum[3];
+        grads[CH_GRAD_B2] = landau ? 0.0 : 2.0 * sum[0];
+    }
+    else if(grads) {
         // slots: <nu, psi>, <nu, L psi>, M term, delta term; euler keeps w = nu / M
         const double dt = s->d.dt, f = em ? dt : dt * s->d.M;
         const bool landau = s->d.model == CH_LANDAU;
@@ -1037,12 +1096,19 @@ static ch_status adjoint_prepare(ch_sys *s) {
             HIPCHK(c, hipEventCreateWithFlags(&s->ev_adj[i], hipEventDisableTiming));
         }
     }
+    const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;
+    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->d.dims, s->d.N, s->d.integrator == CH_EULER_MOBILITY);
     int64_t cap0 = s->lam[0] ? 1 : 0, cap1 = s->lam[1] ? 1 : 0, capg = s->gacc ? 1 : 0;
     if(!grow(&s->lam[0], cap0, 1, fb) || !grow(&s->lam[1], cap1, 1, fb) ||
-       !grow((void **)&s->gacc, capg, 1,
-             chd::kGradSlots * sizeof(double) *
-                 chd::adjoint_tiles(s->d.dims, s->d.N, s->d.integrator == CH_EULER_MOBILITY)))
+       !grow((void **)&s->gacc, capg, 1, chd::kGradSlots * sizeof(double) * ntiles))
         return fail(c, CH_ENOMEM, "no device memory for the adjoint");
+    if(ps && !s->ps_rh) {
+        const size_t hb = (size_t)s->hat * 2 * s->elt;
+        bool ok = hipMalloc(&s->ps_rh, hb) == hipSuccess && hipMalloc(&s->ps_w, hb) == hipSuccess;
+        for(int i = 0; ok && i < 3; i++) ok = hipMalloc(&s->ps_spec[i], hb) == hipSuccess;
+        for(int i = 0; ok && i < 2; i++) ok = hipMalloc(&s->ps_real[i], fb) == hipSuccess;
+        if(!ok) return fail(c, CH_ENOMEM, "no device memory for the pseudospectral adjoint");
+    }
     return CH_OK;
 }
 

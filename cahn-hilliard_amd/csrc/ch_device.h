@@ -164,6 +164,37 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// ---------------------------------------------------------------- spectra
+// q^2 and the dealias factor of a half-spectrum mode, exactly as the reference
+// builds its tables (PseudospectralCPU.cpp:34-83); ix is the half-axis index
+__device__ __forceinline__ void mode_q2(const Params &p, int ix, int iy, int iz, double &k2, double &deal) {
+    const double PI = 3.14159265358979323846;
+    const int N = p.N;
+    const double Ndx = N * p.dx;
+    const double k_cut = N * PI / (N * p.dx) * 2.0 / 3.0;
+    if(p.dims == 1) {
+        double k = 2.0 * PI * ix / Ndx;
+        deal = (k <= k_cut) ? 1.0 : 0.0;
+        k2 = k * k;
+    }
+    else if(p.dims == 2) {
+        int kx = (iy < N / 2) ? iy : N - iy;
+        double k = 2.0 * PI * sqrt((double)(kx * kx + ix * ix)) / Ndx;
+        deal = (k < k_cut) ? 1.0 : 0.0;
+        k2 = k * k;
+    }
+    else {
+        int kx_i = (iz < N / 2) ? iz : iz - N;
+        int ky_i = (iy < N / 2) ? iy : iy - N;
+        double kx = 2.0 * PI * kx_i / Ndx, ky = 2.0 * PI * ky_i / Ndx, kz = 2.0 * PI * ix / Ndx;
+        k2 = kx * kx + ky * ky + kz * kz;
+        double k = sqrt(k2);
+        deal = (k <= k_cut) ? 1.0 : 0.0;
+    }
+}
+
+// rh: this mode of rho_hat, updated in place; returns rho_hat / N^d
+
 // number of 2D wave-kernel region shapes selectable through Params::wave_cfg
 constexpr int kWaveCfgs = 7;
 // default shape: 128 x 64 regions of 16 waves (2 columns per lane) for N >= 512,

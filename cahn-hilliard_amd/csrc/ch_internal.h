@@ -27,6 +27,19 @@ hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init);
 // nsteps time steps, consecutive steps joined by the fused c2r + f' + r2c row
 // pass (same results as nsteps calls of ps_run)
 hipError_t ps_steps(const SpectralArgs &a, hipStream_t s, int64_t nsteps);
+// the same chain from the real state a.rho and spectral state a.rho_hat, with
+// state t (1 <= t <= nsteps) written to field t of `tape` (a.rho untouched)
+hipError_t ps_steps_tape(const SpectralArgs &a, hipStream_t s, int64_t nsteps, void *tape);
+// whole-field transforms (any dims, unnormalised): r2c of in (of f'(in) when
+// fprime) into spec; c2r of spec (overwritten) into out
+hipError_t spec_r2c(const SpectralArgs &a, const void *real_in, void *spec_out, bool fprime, hipStream_t s);
+hipError_t spec_c2r(const SpectralArgs &a, void *spec, void *real_out, hipStream_t s);
+// pseudospectral reverse step (ps_adjoint.hip): lam_in = dL/drho_{n+1} ->
+// lam_out = dL/drho_n with rho_n = `rho`; spec[3] half spectra and real[2]
+// fields of scratch; kPsGradBlocks x kGradSlots partial sums added to gacc
+constexpr int kPsGradBlocks = 256;
+hipError_t ps_adjoint_step(const SpectralArgs &a, const void *lam_in, void *lam_out, const void *rho, void *const spec[3],
+                           void *const real[2], double *gacc, hipStream_t s);
 // 2D transforms on a.work: forward r2c of a.rho, inverse (c2r) into out
 hipError_t fft2_forward(const SpectralArgs &a, hipStream_t s);
 hipError_t fft2_inverse(const SpectralArgs &a, void *out, hipStream_t s);

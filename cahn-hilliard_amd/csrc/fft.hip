@@ -123,35 +123,8 @@ struct LineFFT {
     }
 };
 
-// ---- spectral update (PseudospectralCPU.cpp:119-133), mode -> q^2 exactly as
-// the reference builds its tables (PseudospectralCPU.cpp:34-83)
-__device__ __forceinline__ void mode_q2(const Params &p, int ix, int iy, int iz, double &k2, double &deal) {
-    const double PI = 3.14159265358979323846;
-    const int N = p.N;
-    const double Ndx = N * p.dx;
-    const double k_cut = N * PI / (N * p.dx) * 2.0 / 3.0;
-    if(p.dims == 1) {
-        double k = 2.0 * PI * ix / Ndx;
-        deal = (k <= k_cut) ? 1.0 : 0.0;
-        k2 = k * k;
-    }
-    else if(p.dims == 2) {
-        int kx = (iy < N / 2) ? iy : N - iy;
-        double k = 2.0 * PI * sqrt((double)(kx * kx + ix * ix)) / Ndx;
-        deal = (k < k_cut) ? 1.0 : 0.0;
-        k2 = k * k;
-    }
-    else {
-        int kx_i = (iz < N / 2) ? iz : iz - N;
-        int ky_i = (iy < N / 2) ? iy : iy - N;
-        double kx = 2.0 * PI * kx_i / Ndx, ky = 2.0 * PI * ky_i / Ndx, kz = 2.0 * PI * ix / Ndx;
-        k2 = kx * kx + ky * ky + kz * kz;
-        double k = sqrt(k2);
-        deal = (k <= k_cut) ? 1.0 : 0.0;
-    }
-}
-
-// rh: this mode of rho_hat, updated in place; returns rho_hat / N^d
+// ---- spectral update (PseudospectralCPU.cpp:119-133); mode_q2 (ch_device.h)
+// gives q^2 exactly as the reference builds its tables
 template <typename T, typename T2>
 __device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, int iz, T2 fh, T2 &rh, T inv_bins) {
     double k2d, deal;
@@ -262,9 +235,12 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
 // k_xinv), then f' and r2c of step n + 1 (as k_xfwd), in place on W.  The real
 // row stays in LDS as T values, so the result has the same bits as k_xinv
 // followed by k_xfwd, with one read and one write of W instead of two each.
+// rho_out (may be null): the c2r output rows are also stored there (the
+// recompute of a reverse sweep keeps every state)
 template <typename T, int MODEL, int LOG2N>
 __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xrow(
-    typename Complex<T>::type *__restrict__ W, const typename Complex<T>::type *__restrict__ tw, Params p, int rows) {
+    typename Complex<T>::type *__restrict__ W, const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+    T *__restrict__ rho_out) {
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
@@ -302,6 +278,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
         const int e = threadIdx.x + i * NT;
         const int row = row0 + e / N;
         const int a = 2 * pidx(e >> 1) + (e & 1);
+        if(rho_out && row < rows) rho_out[(size_t)row * N + e % N] = bufT[a];
         bufT[a] = (row < rows) ? fprime<T, MODEL>(p, bufT[a]) : (T)0;
     }
     __syncthreads();
@@ -410,8 +387,10 @@ static inline int col_blocks(int lines, int nlc) { return ((lines + nlc - 1) / n
 // init: rho_hat = r2c(rho).  Otherwise `nsteps` time steps: the x pass of f'
 // (rho), then per step the axis passes with the fused spectral update, joined
 // to the next step by k_xrow; the last step ends with the c2r into rho.
+// tape (may be null): state t + 1 of the chain goes to tape field t + 1 (the
+// last one from k_xinv, instead of rho)
 template <typename T, int LOG2N>
-static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init) {
+static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init, T *tape = nullptr) {
     using T2 = typename Complex<T>::type;
     const Params &p = a.p;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1;
@@ -426,6 +405,7 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
     const int bx = (rows + GX::NL - 1) / GX::NL;
     const bool landau = a.model == CH_LANDAU;
+    const size_t bins = (size_t)rows * N;
     if(init) {
         hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, dims == 1 ? RH : W,
                            tw2, p, rows);
@@ -467,30 +447,33 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
                                (long)N * Nh, (long)Nh, 1);
         }
         if(t + 1 < nsteps) {
+            T *ro = tape ? tape + (size_t)(t + 1) * bins : nullptr;
             if(landau)
-                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows);
+                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows, ro);
             else
-                hipLaunchKernelGGL((k_xrow<T, CH_SIMPLE_WERTHEIM, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows);
+                hipLaunchKernelGGL((k_xrow<T, CH_SIMPLE_WERTHEIM, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows,
+                                   ro);
         }
     }
-    hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, rho, tw2, p, rows);
+    hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tape ? tape + (size_t)nsteps * bins : rho, tw2,
+                       p, rows);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t ps_chain_p(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init) {
+static hipError_t ps_chain_p(const SpectralArgs &a, hipStream_t s, int64_t nsteps, bool init, T *tape = nullptr) {
     switch(a.p.log2N) {
-    case 2: return ps_chain_t<T, 2>(a, s, nsteps, init);
-    case 3: return ps_chain_t<T, 3>(a, s, nsteps, init);
-    case 4: return ps_chain_t<T, 4>(a, s, nsteps, init);
-    case 5: return ps_chain_t<T, 5>(a, s, nsteps, init);
-    case 6: return ps_chain_t<T, 6>(a, s, nsteps, init);
-    case 7: return ps_chain_t<T, 7>(a, s, nsteps, init);
-    case 8: return ps_chain_t<T, 8>(a, s, nsteps, init);
-    case 9: return ps_chain_t<T, 9>(a, s, nsteps, init);
-    case 10: return ps_chain_t<T, 10>(a, s, nsteps, init);
-    case 11: return ps_chain_t<T, 11>(a, s, nsteps, init);
-    case 12: return ps_chain_t<T, 12>(a, s, nsteps, init);
+    case 2: return ps_chain_t<T, 2>(a, s, nsteps, init, tape);
+    case 3: return ps_chain_t<T, 3>(a, s, nsteps, init, tape);
+    case 4: return ps_chain_t<T, 4>(a, s, nsteps, init, tape);
+    case 5: return ps_chain_t<T, 5>(a, s, nsteps, init, tape);
+    case 6: return ps_chain_t<T, 6>(a, s, nsteps, init, tape);
+    case 7: return ps_chain_t<T, 7>(a, s, nsteps, init, tape);
+    case 8: return ps_chain_t<T, 8>(a, s, nsteps, init, tape);
+    case 9: return ps_chain_t<T, 9>(a, s, nsteps, init, tape);
+    case 10: return ps_chain_t<T, 10>(a, s, nsteps, init, tape);
+    case 11: return ps_chain_t<T, 11>(a, s, nsteps, init, tape);
+    case 12: return ps_chain_t<T, 12>(a, s, nsteps, init, tape);
     }
     return hipErrorInvalidValue;
 }
@@ -503,6 +486,88 @@ hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init) {
 hipError_t ps_steps(const SpectralArgs &a, hipStream_t s, int64_t nsteps) {
     if(a.precision == CH_FP64) return ps_chain_p<double>(a, s, nsteps, false);
     return ps_chain_p<float>(a, s, nsteps, false);
+}
+
+hipError_t ps_steps_tape(const SpectralArgs &a, hipStream_t s, int64_t nsteps, void *tape) {
+    if(nsteps < 1) return hipSuccess;
+    if(a.precision == CH_FP64) return ps_chain_p<double>(a, s, nsteps, false, (double *)tape);
+    return ps_chain_p<float>(a, s, nsteps, false, (float *)tape);
+}
+
+// ---- whole-field transforms for any dims: r2c (optionally of f'(in)) and
+// c2r (unnormalised; the spectrum is overwritten)
+template <typename T, int LOG2N>
+static hipError_t spec_t(const SpectralArgs &a, const void *real_in, void *spec, void *real_out, bool fprime,
+                         bool inverse, hipStream_t s) {
+    using T2 = typename Complex<T>::type;
+    const Params &p = a.p;
+    constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
+    using GX = FftGeom<LX, PackedLines<LX>::value>;
+    constexpr int NLC = ColLines<LOG2N>::value;
+    using GC = FftGeom<LOG2N, NLC>;
+    const int dims = p.dims;
+    const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
+    const int bx = (rows + GX::NL - 1) / GX::NL;
+    const T2 *tw2 = (const T2 *)a.twiddles;
+    T2 *W = (T2 *)spec;
+    const int lines = dims == 3 ? N * Nh : Nh;
+    const int bc = col_blocks(lines, NLC);
+    if(!inverse) {
+        if(!fprime)
+            hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
+                               tw2, p, rows);
+        else if(a.model == CH_LANDAU)
+            hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
+                               tw2, p, rows);
+        else
+            hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s,
+                               (const T *)real_in, W, tw2, p, rows);
+        if(dims >= 2)
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
+        if(dims == 3)
+            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
+                               (long)Nh, (long)N * Nh, 2);
+    }
+    else {
+        if(dims == 3)
+            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
+                               (long)Nh, (long)N * Nh, 2);
+        if(dims >= 2)
+            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
+                               (long)N * Nh, (long)Nh, 1);
+        hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, (T *)real_out, tw2, p, rows);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t spec_p(const SpectralArgs &a, const void *in, void *spec, void *out, bool fprime, bool inverse,
+                         hipStream_t s) {
+    switch(a.p.log2N) {
+    case 2: return spec_t<T, 2>(a, in, spec, out, fprime, inverse, s);
+    case 3: return spec_t<T, 3>(a, in, spec, out, fprime, inverse, s);
+    case 4: return spec_t<T, 4>(a, in, spec, out, fprime, inverse, s);
+    case 5: return spec_t<T, 5>(a, in, spec, out, fprime, inverse, s);
+    case 6: return spec_t<T, 6>(a, in, spec, out, fprime, inverse, s);
+    case 7: return spec_t<T, 7>(a, in, spec, out, fprime, inverse, s);
+    case 8: return spec_t<T, 8>(a, in, spec, out, fprime, inverse, s);
+    case 9: return spec_t<T, 9>(a, in, spec, out, fprime, inverse, s);
+    case 10: return spec_t<T, 10>(a, in, spec, out, fprime, inverse, s);
+    case 11: return spec_t<T, 11>(a, in, spec, out, fprime, inverse, s);
+    case 12: return spec_t<T, 12>(a, in, spec, out, fprime, inverse, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t spec_r2c(const SpectralArgs &a, const void *real_in, void *spec_out, bool fprime, hipStream_t s) {
+    if(a.precision == CH_FP64) return spec_p<double>(a, real_in, spec_out, nullptr, fprime, false, s);
+    return spec_p<float>(a, real_in, spec_out, nullptr, fprime, false, s);
+}
+
+hipError_t spec_c2r(const SpectralArgs &a, void *spec, void *real_out, hipStream_t s) {
+    if(a.precision == CH_FP64) return spec_p<double>(a, nullptr, spec, real_out, false, true, s);
+    return spec_p<float>(a, nullptr, spec, real_out, false, true, s);
 }
 
 // ---- 2D inverse transform (column IFFT + c2r) of a.work into `out`

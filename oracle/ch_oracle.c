@@ -797,3 +797,87 @@ double och_free_energy_grad(const och_params *p, const och_model *m, const doubl
     }
     return och_average_free_energy(p, m, rho);
 }
+
+/* ------------------------------------------------------------------ */
+/* adjoint of the pseudospectral step (PseudospectralCPU.cpp:112-136)  */
+/* On the real field the step is rho' = Ainv(A F rho + B F f'(rho))    */
+/* with per-mode A = (1 + dt M S q^2) / D, B = -dt M q^2 deal / D,      */
+/* D = 1 + dt M (S q^2 + 2 k q^4): real multipliers even in q, so the   */
+/* operators are symmetric and                                          */
+/*   lambda = F^-1(A F lambda') + f''(rho) F^-1(B F lambda').           */
+/* k and M enter through A and B: with rho_hat' = A rho_hat + B f_hat,  */
+/*   d rho_hat'/dk = -2 dt M q^4 rho_hat' / D,                           */
+/*   d rho_hat'/dM = (-dt q^2 (deal f_hat - S rho_hat)                  */
+/*                    - dt (S q^2 + 2 k q^4) rho_hat') / D,              */
+/* and <lambda', F^-1 X> = (1/bins) sum over the full spectrum of       */
+/* conj(lambda_hat') X (half spectrum: weight 2 off the kx = 0, N/2     */
+/* planes).                                                             */
+/* ------------------------------------------------------------------ */
+void och_ps_adjoint_step(const och_params *p, const och_model *m, const double *rho, double *lambda, double *grad) {
+    int dims = p->dims, N = p->N, Nh = N / 2 + 1;
+    int64_t bins = ipow(N, dims), hat = ipow(N, dims - 1) * Nh;
+    och_ps *tab = och_ps_create(p, m, rho);  /* q^2 and dealias tables, rho_hat = r2c(rho) */
+    double *lh = (double *)malloc(sizeof(double) * 2 * hat);
+    double *fh = (double *)malloc(sizeof(double) * 2 * hat);
+    double *ah = (double *)malloc(sizeof(double) * 2 * hat);
+    double *bh = (double *)malloc(sizeof(double) * 2 * hat);
+    double *f = (double *)malloc(sizeof(double) * bins);
+    double *a = (double *)malloc(sizeof(double) * bins);
+    double *b = (double *)malloc(sizeof(double) * bins);
+    r2c(dims, N, lambda, lh);
+    for(int64_t i = 0; i < bins; i++) f[i] = och_der_bulk_free_energy(m, rho[i]);
+    r2c(dims, N, f, fh);
+    const double dt = p->dt, M = p->M, S = p->S;
+    double gk = 0, gM = 0;
+    for(int64_t q = 0; q < hat; q++) {
+        const double k2 = tab->k2[q], k4 = k2 * k2;
+        const double deal = p->use_dealias ? tab->dealiaser[q] : 1.0;
+        const double D = 1.0 + dt * M * (S * k2 + 2.0 * p->k * k4);
+        const double A = (1.0 + dt * M * S * k2) / D, B = -dt * M * k2 * deal / D;
+        const double *r = tab->rho_hat + 2 * q, *fq = fh + 2 * q, *l = lh + 2 * q;
+        const double nr = A * r[0] + B * fq[0], ni = A * r[1] + B * fq[1];  /* rho_hat' */
+        const int kx = (int)(q % Nh);
+        const double w = ((kx == 0 || kx == N / 2) ? 1.0 : 2.0) / bins;
+        const double dk = -2.0 * dt * M * k4 / D;
+        gk += w * dk * (l[0] * nr + l[1] * ni);
+        const double xr = (-dt * k2 * (deal * fq[0] - S * r[0]) - dt * (S * k2 + 2.0 * p->k * k4) * nr) / D;
+        const double xi = (-dt * k2 * (deal * fq[1] - S * r[1]) - dt * (S * k2 + 2.0 * p->k * k4) * ni) / D;
+        gM += w * (l[0] * xr + l[1] * xi);
+        ah[2 * q] = A * l[0] / bins; ah[2 * q + 1] = A * l[1] / bins;
+        bh[2 * q] = B * l[0] / bins; bh[2 * q + 1] = B * l[1] / bins;
+    }
+    c2r(dims, N, ah, a);
+    c2r(dims, N, bh, b);
+    double ge = 0, gd = 0;
+    for(int64_t i = 0; i < bins; i++) {
+        ge += b[i] * rho[i];
+        gd += b[i] * och_der_bulk_free_energy_ddelta(m, rho[i]);
+        lambda[i] = a[i] + och_second_der_bulk_free_energy(m, rho[i]) * b[i];
+    }
+    if(m->model == OCH_LANDAU) grad[OCH_G_EPSILON] += -ge;
+    else {
+        grad[OCH_G_DELTA] += gd;
+        grad[OCH_G_B2] += 2.0 * ge;
+    }
+    grad[OCH_G_K] += gk;
+    grad[OCH_G_M] += gM;
+    och_ps_destroy(tab);
+    free(lh); free(fh); free(ah); free(bh); free(f); free(a); free(b);
+}
+
+void och_ps_adjoint(const och_params *p, const och_model *m, const double *rho0, int64_t steps,
+                    const double *dL_drhoT, double *dL_drho0, double *grad) {
+    int64_t bins = ipow(p->N, p->dims);
+    double *tape = (double *)malloc(sizeof(double) * bins * (steps + 1));
+    memcpy(tape, rho0, sizeof(double) * bins);
+    och_ps *s = och_ps_create(p, m, rho0);
+    for(int64_t t = 0; t < steps; t++) {
+        memcpy(tape + (t + 1) * bins, tape + t * bins, sizeof(double) * bins);
+        och_ps_step(s, tape + (t + 1) * bins);
+    }
+    och_ps_destroy(s);
+    memcpy(dL_drho0, dL_drhoT, sizeof(double) * bins);
+    for(int g = 0; g < OCH_NGRADS; g++) grad[g] = 0.0;
+    for(int64_t t = steps - 1; t >= 0; t--) och_ps_adjoint_step(p, m, tape + t * bins, dL_drho0, grad);
+    free(tape);
+}

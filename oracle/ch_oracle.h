@@ -115,6 +115,11 @@ void och_adjoint_step(int integrator, const och_params *p, const och_model *m, c
                       double *grad);
 void och_adjoint(int integrator, const och_params *p, const och_model *m, const double *psi0, int64_t steps,
                  const double *dL_dpsiT, double *dL_dpsi0, double *grad);
+/* adjoint of the pseudospectral step on the real field (k and M through the
+ * spectral multipliers; S is held fixed), and the full forward + reverse sweep */
+void och_ps_adjoint_step(const och_params *p, const och_model *m, const double *rho, double *lambda, double *grad);
+void och_ps_adjoint(const och_params *p, const och_model *m, const double *rho0, int64_t steps,
+                    const double *dL_drhoT, double *dL_drho0, double *grad);
 /* F = average_free_energy(rho) with dF/drho (may be NULL) and the explicit
  * dF/dparams[OCH_NGRADS] (may be NULL) */
 double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,

@@ -103,6 +103,8 @@ def lib(fast=False):
         "och_adjoint_step": (None, [I, PP, MP, P, P, P]),
         "och_adjoint": (None, [I, PP, MP, P, I64, P, P, P]),
         "och_free_energy_grad": (D, [PP, MP, P, P, P]),
+        "och_ps_adjoint_step": (None, [PP, MP, P, P, P]),
+        "och_ps_adjoint": (None, [PP, MP, P, I64, P, P, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(L, n)
@@ -278,14 +280,17 @@ def mobility_ddelta(params, model, r):
 
 
 def adjoint(integrator, params, model, psi0, steps, dL_dpsiT):
-    """Reverse sweep of `steps` EULER / EULER_MOBILITY steps for the loss
-    <dL_dpsiT, psi_T>: returns (dL/dpsi0, grad[NGRADS])."""
+    """Reverse sweep of `steps` EULER / EULER_MOBILITY / PSEUDOSPECTRAL steps for
+    the loss <dL_dpsiT, psi_T>: returns (dL/dpsi0, grad[NGRADS])."""
     psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
     w = np.ascontiguousarray(dL_dpsiT, dtype=np.float64)
     out = np.empty_like(psi0)
     grad = np.zeros(NGRADS)
-    _L().och_adjoint(integrator, ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi0), steps, _ptr(w),
-                     _ptr(out), _ptr(grad))
+    args = (ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi0), steps, _ptr(w), _ptr(out), _ptr(grad))
+    if integrator == PSEUDOSPECTRAL:
+        _L().och_ps_adjoint(*args)
+    else:
+        _L().och_adjoint(integrator, *args)
     return out, grad
 
 
@@ -293,8 +298,11 @@ def adjoint_step(integrator, params, model, psi, lam):
     psi = np.ascontiguousarray(psi, dtype=np.float64)
     lam = np.array(lam, dtype=np.float64, copy=True)
     grad = np.zeros(NGRADS)
-    _L().och_adjoint_step(integrator, ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi), _ptr(lam),
-                          _ptr(grad))
+    args = (ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(psi), _ptr(lam), _ptr(grad))
+    if integrator == PSEUDOSPECTRAL:
+        _L().och_ps_adjoint_step(*args)
+    else:
+        _L().och_adjoint_step(integrator, *args)
     return lam, grad
 
 

@@ -312,3 +312,116 @@ def test_c5_free_energy_delta_gradient_chain(ch, ctx):
     fd = (F_at(delta + h) - F_at(delta - h)) / (2 * h)
     print(f"dF/ddelta = {total:.10e} (explicit {dp[ch.GRAD_DELTA]:.3e}), FD {fd:.10e}")
     assert total == pytest.approx(fd, rel=1e-5)
+
+
+PS_CASES = [
+    # dims, N, steps, ckpt, desc kwargs, (density, A)
+    (2, 64, 20, 7, dict(epsilon=0.9, dt=0.1, k=1.2, M=0.8), (0.0, 0.3)),
+    (2, 128, 12, 0, dict(epsilon=0.7, dt=0.05, k=0.9, M=1.1, S=0.3, use_dealias=True), (0.0, 0.4)),
+    (1, 256, 30, 0, dict(epsilon=0.8, dt=0.1, dx=0.8), (0.1, 0.3)),
+    (3, 16, 10, 4, dict(epsilon=0.9, dt=0.1, use_dealias=True), (0.0, 0.3)),
+    (2, 64, 15, 5, dict(free_energy="simple_wertheim", B2=1.0, delta=50.0, valence=4, dt=1e-3), (0.3, 0.2)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(PS_CASES)))
+def test_pseudospectral_adjoint_against_oracle(ch, ctx, case):
+    dims, N, steps, ckpt, kw, (rho0, A) = PS_CASES[case]
+    psi0 = O.init_density(dims, N, rho0, A, 0, 99 + case)
+    w = np.random.default_rng(70 + case).normal(size=N ** dims)
+    s = ch.System(ctx, ch.make_desc(dims=dims, N=N, integrator="pseudospectral", **kw))
+    s.upload(psi0)
+    s.record(steps, ckpt)
+    psiT = s.download()
+    lam, g = s.adjoint(w)
+    s.close()
+    okw = {k: v for k, v in kw.items() if k in ("dt", "dx", "k", "M", "S")}
+    p = O.Params(dims=dims, N=N, use_dealias=int(kw.get("use_dealias", False)), **okw)
+    m = O.Model(O.SIMPLE_WERTHEIM, B2=kw["B2"], delta=kw["delta"], valence=kw["valence"]) \
+        if kw.get("free_energy") == "simple_wertheim" else O.Model(O.LANDAU, epsilon=kw["epsilon"])
+    ref_T = O.run(O.PSEUDOSPECTRAL, p, m, psi0, steps)
+    ref_lam, ref_g = O.adjoint(O.PSEUDOSPECTRAL, p, m, psi0, steps, w)
+    print(f"case {case}: psiT {rel_linf(psiT, ref_T):.2e} lam {rel_linf(lam, ref_lam):.2e} g {g} ref {ref_g}")
+    assert rel_linf(psiT, ref_T) < 1e-10
+    assert rel_linf(lam, ref_lam) < 1e-10
+    np.testing.assert_allclose(g, ref_g, rtol=1e-9, atol=1e-12 * np.abs(ref_g).max())
+
+
+def test_pseudospectral_checkpoint_invariance(ch, ctx):
+    N, steps = 64, 24
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 17)
+    w = np.random.default_rng(5).normal(size=N * N)
+    base = None
+    for ckpt in (0, 1, 5, 24):
+        s = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator="pseudospectral", dt=0.1, use_dealias=True))
+        s.upload(psi0)
+        s.record(steps, ckpt)
+        lam, g = s.adjoint(w)
+        s.close()
+        if base is None:
+            base = (lam, g)
+        else:
+            assert np.array_equal(lam, base[0]), ckpt
+            np.testing.assert_allclose(g, base[1], rtol=1e-12)
+
+
+def test_pseudospectral_adjoint_finite_differences(ch, ctx):
+    """Central differences (h = 1e-6) of L = <w, psi_T> through the GPU forward."""
+    N, steps = 64, 40
+    kw = dict(dims=2, N=N, integrator="pseudospectral", dt=0.1, dx=1.0)
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 123456)
+    w = np.random.default_rng(11).normal(size=N * N)
+    d = np.random.default_rng(12).normal(size=N * N)
+    params = dict(epsilon=0.9, k=1.2, M=1.0)
+
+    def loss(x, **pp):
+        s = ch.System(ctx, ch.make_desc(**pp, **kw))
+        s.upload(x)
+        s.evolve(steps)
+        v = float(np.dot(w, s.download()))
+        s.close()
+        return v
+
+    s = ch.System(ctx, ch.make_desc(**params, **kw))
+    s.upload(psi0)
+    s.record(steps, 10)
+    lam, g = s.adjoint(w)
+    s.close()
+    h = 1e-6
+    fd = (loss(psi0 + h * d, **params) - loss(psi0 - h * d, **params)) / (2 * h)
+    assert np.dot(lam, d) == pytest.approx(fd, rel=1e-6)
+    for i, name in ((ch.GRAD_EPSILON, "epsilon"), (ch.GRAD_K, "k"), (ch.GRAD_M, "M")):
+        up, dn = dict(params), dict(params)
+        up[name] += h
+        dn[name] -= h
+        fd = (loss(psi0, **up) - loss(psi0, **dn)) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-6), name
+
+
+def test_pseudospectral_fp32_adjoint_against_fp64(ch, ctx):
+    N, steps = 64, 20
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 21)
+    w = np.random.default_rng(13).normal(size=N * N)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator="pseudospectral", precision=prec, dt=0.1))
+        s.upload(psi0)
+        s.record(steps, 5)
+        out[prec] = s.adjoint(w)
+        s.close()
+    err = rel_linf(out["fp32"][0], out["fp64"][0])
+    print(f"pseudospectral fp32 adjoint rel L-inf vs fp64 = {err:.3e}")
+    assert err < 1e-3
+    np.testing.assert_allclose(out["fp32"][1][:3], out["fp64"][1][:3], rtol=1e-3)
+
+
+def test_pseudospectral_autograd_gradcheck(ch):
+    import torch
+    from cahn_hilliard_amd.autograd import euler_evolve
+    N = 16
+    psi0 = torch.tensor(O.init_density(2, N, 0.0, 0.3, 0, 3), dtype=torch.float64, device="cuda", requires_grad=True)
+    eps = torch.tensor(0.9, dtype=torch.float64, device="cuda", requires_grad=True)
+    k = torch.tensor(1.1, dtype=torch.float64, device="cuda", requires_grad=True)
+    M = torch.tensor(0.8, dtype=torch.float64, device="cuda", requires_grad=True)
+    fn = lambda a, b, c, d: euler_evolve(a, b, c, d, 10, 4, dims=2, N=N, dt=0.1, integrator="pseudospectral")
+    assert torch.autograd.gradcheck(fn, (psi0, eps, k, M), eps=1e-6, atol=1e-8, rtol=1e-6)

@@ -400,3 +400,67 @@ def test_free_energy_gradient_by_finite_differences(model):
         fd = (vals[0] - vals[1]) / (2 * hp)
         assert dp[gi] == pytest.approx(fd, rel=1e-6, abs=1e-12), attr
     assert dp[O.G_M] == 0.0
+
+
+PS_ADJOINT_CASES = [
+    # dims, N, steps, model kwargs, params kwargs, (density, A)
+    (2, 16, 12, dict(model=O.LANDAU, epsilon=0.9), dict(dt=0.1, k=1.2, M=0.8), (0.0, 0.3)),
+    (2, 16, 10, dict(model=O.LANDAU, epsilon=0.7), dict(dt=0.05, k=0.9, M=1.1, S=0.3, use_dealias=1), (0.0, 0.4)),
+    (1, 32, 15, dict(model=O.LANDAU, epsilon=0.8), dict(dt=0.1, k=1.0, M=1.0, dx=0.8), (0.1, 0.3)),
+    (3, 8, 8, dict(model=O.LANDAU, epsilon=0.9), dict(dt=0.1, k=1.0, M=1.0, use_dealias=1), (0.0, 0.3)),
+    (2, 16, 10, dict(model=O.SIMPLE_WERTHEIM, B2=1.0, delta=50.0, valence=4), dict(dt=1e-3, k=1.0, M=1.0),
+     (0.3, 0.2)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(PS_ADJOINT_CASES)))
+def test_pseudospectral_adjoint_against_finite_differences(case):
+    """dL/drho0 and dL/d(eps, k, M, delta, B2) of L = <w, rho_T> for the
+    pseudospectral integrator against central differences of the forward map."""
+    dims, N, steps, mk, pk, (rho0, A) = PS_ADJOINT_CASES[case]
+    p = O.Params(dims=dims, N=N, **pk)
+    m = O.Model(**mk)
+    psi0 = O.init_density(dims, N, rho0, A, 0, 777 + case)
+    rng = np.random.default_rng(50 + case)
+    w, d = rng.normal(size=N ** dims), rng.normal(size=N ** dims)
+    lam0, grad = O.adjoint(O.PSEUDOSPECTRAL, p, m, psi0, steps, w)
+
+    def loss(pp, mm, x):
+        return float(np.dot(w, O.run(O.PSEUDOSPECTRAL, pp, mm, x, steps)))
+
+    h = 1e-6 * max(1.0, np.abs(psi0).max())
+    fd = (loss(p, m, psi0 + h * d) - loss(p, m, psi0 - h * d)) / (2 * h)
+    assert np.dot(lam0, d) == pytest.approx(fd, rel=1e-6)
+    names = {O.G_EPSILON: ("epsilon", "m"), O.G_K: ("k", "p"), O.G_M: ("M", "p"), O.G_DELTA: ("delta", "m"),
+             O.G_B2: ("B2", "m")}
+    for g, (attr, obj) in names.items():
+        base = getattr(m if obj == "m" else p, attr)
+        hp = 1e-6 * max(abs(base), 1.0)
+
+        def shifted(s):
+            pp = O.Params(**{**p.__dict__})
+            mm = O.Model(**{**m.__dict__})
+            target = mm if obj == "m" else pp
+            setattr(target, attr, getattr(target, attr) + s)
+            return loss(pp, mm, psi0)
+
+        fd = (shifted(hp) - shifted(-hp)) / (2 * hp)
+        has = not ((g == O.G_EPSILON and m.model != O.LANDAU) or (g in (O.G_DELTA, O.G_B2) and m.model == O.LANDAU))
+        if has:
+            assert grad[g] == pytest.approx(fd, rel=1e-5, abs=1e-9 * np.abs(w).sum()), attr
+        else:
+            assert grad[g] == 0.0
+
+
+def test_pseudospectral_adjoint_dot_product():
+    """<J v, u> = <v, J^T u> for one pseudospectral step, with dealiasing and S."""
+    N = 16
+    p = O.Params(dims=2, N=N, dt=0.1, k=1.0, M=1.0, S=0.2, use_dealias=1)
+    m = O.Model(epsilon=0.9)
+    rng = np.random.default_rng(9)
+    psi = rng.uniform(-0.5, 0.5, N * N)
+    v, u = rng.normal(size=N * N), rng.normal(size=N * N)
+    h = 1e-6
+    Jv = (O.run(O.PSEUDOSPECTRAL, p, m, psi + h * v, 1) - O.run(O.PSEUDOSPECTRAL, p, m, psi - h * v, 1)) / (2 * h)
+    JTu, _ = O.adjoint_step(O.PSEUDOSPECTRAL, p, m, psi, u)
+    assert np.dot(Jv, u) == pytest.approx(np.dot(v, JTu), rel=1e-8)
