@@ -146,14 +146,14 @@ __device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, i
 template <typename T, int MODEL, bool FPRIME, int LOG2N>
 __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xfwd(
     const T *__restrict__ in, typename Complex<T>::type *__restrict__ out, const typename Complex<T>::type *__restrict__ tw,
-    Params p, int rows) {
+    Params p, int rows, long rs, long ks) {
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT;
     __shared__ T2 buf[padded(G::E)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 2);
-    const int row0 = blockIdx.x * nl;
+    const int row0 = xcd_tile(blockIdx.x, gridDim.x) * nl;
     T *bufT = reinterpret_cast<T *>(buf);
 #pragma unroll
     for(int i = 0; i < 8; i++) {
@@ -180,7 +180,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
         const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
         const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
         const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
-        out[(size_t)row * Nh + k] = cadd(Ev, cmul(tw[k], Ov));
+        out[(size_t)row * rs + (size_t)k * ks] = cadd(Ev, cmul(tw[k], Ov));
     }
 }
 
@@ -189,18 +189,18 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
 template <typename T, int LOG2N>
 __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xinv(
     const typename Complex<T>::type *__restrict__ in, T *__restrict__ out, const typename Complex<T>::type *__restrict__ tw,
-    Params p, int rows) {
+    Params p, int rows, long rs, long ks) {
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
     __shared__ T2 buf[padded(G::E + nl)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 2);
-    const int row0 = blockIdx.x * nl;
+    const int row0 = xcd_tile(blockIdx.x, gridDim.x) * nl;
     for(int e = threadIdx.x; e < nl * Nh; e += NT) {
         const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
-        buf[pidx(e)] = (row < rows) ? in[(size_t)row * Nh + k] : T2{0, 0};
+        buf[pidx(e)] = (row < rows) ? in[(size_t)row * rs + (size_t)k * ks] : T2{0, 0};
     }
     __syncthreads();
     T2 z[4];
@@ -240,18 +240,18 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
 template <typename T, int MODEL, int LOG2N>
 __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) void k_xrow(
     typename Complex<T>::type *__restrict__ W, const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
-    T *__restrict__ rho_out) {
+    long rs, long ks, T *__restrict__ rho_out) {
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, PackedLines<LOG2N>::value>;
     constexpr int n = G::n, N = 2 * n, nl = G::NL, NT = G::NT, Nh = n + 1;
     __shared__ T2 buf[padded(G::E + nl)];
     LineFFT<T2, LOG2N, nl> fft;
     fft.prefetch(tw, 2);
-    const int row0 = blockIdx.x * nl;
+    const int row0 = xcd_tile(blockIdx.x, gridDim.x) * nl;
     for(int e = threadIdx.x; e < nl * Nh; e += NT) {
         const int line = e / Nh, k = e % Nh;
         const int row = row0 + line;
-        buf[pidx(e)] = (row < rows) ? W[(size_t)row * Nh + k] : T2{0, 0};
+        buf[pidx(e)] = (row < rows) ? W[(size_t)row * rs + (size_t)k * ks] : T2{0, 0};
     }
     __syncthreads();
     T2 z[4];
@@ -292,22 +292,24 @@ __global__ __launch_bounds__((FftGeom<LOG2N, PackedLines<LOG2N>::value>::NT)) vo
         const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
         const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
         const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
-        W[(size_t)row * Nh + k] = cadd(Ev, cmul(tw[k], Ov));
+        W[(size_t)row * rs + (size_t)k * ks] = cadd(Ev, cmul(tw[k], Ov));
     }
 }
 
-// ---- strided axis pass over the half spectrum, NLC lines per block.  Line l
-// starts at (l / Nh) * P + (l % Nh) and steps by `stride`.  MODE 0: forward,
+// ---- axis pass over the half spectrum, NLC lines per block.  Line l starts
+// at (l / Nh) * P + (l % Nh) * ls and steps by `stride`.  MODE 0: forward,
 // 1: inverse, 2: forward + spectral update of rho_hat + /N^d + inverse.
-// Element e of a block is (row j = e / NLC, line l0 + e % NLC), so a wave
-// touches NLC adjacent lines of 32 rows; the lines of one 128-byte row segment
-// belong to consecutive blocks, which the XCD-aware block order keeps on one
-// XCD (one L2).  rho_hat is fetched at kernel start, ahead of the transform.
-template <typename T, int MODE, int LOG2N, int NLC>
+// Strided lines (stride > 1): element e of a block is (row j = e / NLC, line
+// l0 + e % NLC), so a wave touches NLC adjacent lines of 32 rows; the lines of
+// one 128-byte row segment belong to consecutive blocks, which the XCD-aware
+// block order keeps on one XCD (one L2).  Contiguous lines (CONTIG, stride 1):
+// element e is (j = e % n, line l0 + e / n), a wave reads 64 consecutive
+// entries.  rho_hat is fetched at kernel start, ahead of the transform.
+template <typename T, int MODE, int LOG2N, int NLC, bool CONTIG = false>
 __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
     const typename Complex<T>::type *__restrict__ src, typename Complex<T>::type *__restrict__ dst,
     typename Complex<T>::type *__restrict__ rho_hat, const typename Complex<T>::type *__restrict__ tw, Params p,
-    int lines, long P, long stride, int axis) {
+    int lines, long P, long ls, long stride, int axis) {
     using T2 = typename Complex<T>::type;
     using G = FftGeom<LOG2N, NLC>;
     constexpr int n = G::n, NT = G::NT, nl = NLC;
@@ -323,9 +325,9 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
 #pragma unroll
     for(int i = 0; i < 4; i++) {
         const int e = threadIdx.x + i * NT;
-        const int j = e / nl, lb = e % nl;
+        const int j = CONTIG ? e % n : e / nl, lb = CONTIG ? e / n : e % nl;
         const int l = l0 + lb;
-        gi[i] = (size_t)(l / Nh) * P + (l % Nh) + (size_t)j * stride;
+        gi[i] = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls + (size_t)j * stride;
         T2 v{0, 0};
         if(l < lines) {
             v = src[gi[i]];
@@ -340,7 +342,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
 #pragma unroll
         for(int i = 0; i < 4; i++) {
             const int e = threadIdx.x + i * NT;
-            const int j = e / nl, lb = e % nl;
+            const int j = CONTIG ? e % n : e / nl, lb = CONTIG ? e / n : e % nl;
             const int l = l0 + lb;
             if(l < lines) {
                 const int ix = l % Nh;
@@ -358,7 +360,7 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
 #pragma unroll
     for(int i = 0; i < 4; i++) {
         const int e = threadIdx.x + i * NT;
-        const int j = e / nl, lb = e % nl;
+        const int j = CONTIG ? e % n : e / nl, lb = CONTIG ? e / n : e % nl;
         if(l0 + lb < lines) dst[gi[i]] = buf[pidx(lb * n + j)];
     }
 }
@@ -384,6 +386,52 @@ struct ColLines { static constexpr int value = LOG2N >= 11 ? 1 : (LOG2N == 10 ? 
 // k_col grid: the line groups padded to a multiple of the 8 XCDs
 static inline int col_blocks(int lines, int nlc) { return ((lines + nlc - 1) / nlc + 7) / 8 * 8; }
 
+// Half-spectrum layout: (z, y, kx) with kx fastest, FFTW's r2c layout
+// (PseudospectralCPU.cpp:90-96), in 1D, 2D and 3D.  (A transposed 2D layout,
+// whole contiguous lines for the column pass, measured slower overall on
+// MI355X: the column pass gained 5 %, the x passes lost 18 %.)
+struct SpecLayout {
+    long rs, ks;              // x pass: entry (row, kx) at row * rs + kx * ks
+    long yP, yls, ystride;    // y pass: line l at (l / Nh) * yP + (l % Nh) * yls, step ystride
+    long zP, zls, zstride;    // z pass (3D)
+    bool ycontig;
+};
+
+static SpecLayout spec_layout(int dims, int N) {
+    const long Nh = N / 2 + 1;
+    (void)dims;
+    return SpecLayout{Nh, 1, (long)N * Nh, 1, Nh, Nh, 1, (long)N * Nh, false};
+}
+
+template <typename T, int MODE, int LOG2N>
+static void launch_col(const typename Complex<T>::type *src, typename Complex<T>::type *dst,
+                       typename Complex<T>::type *rh, const typename Complex<T>::type *tw, const Params &p, int lines,
+                       long P, long ls, long stride, int axis, bool contig, hipStream_t s) {
+    constexpr int NLC = ColLines<LOG2N>::value;
+    using GC = FftGeom<LOG2N, NLC>;
+    const int bc = col_blocks(lines, NLC);
+    if(contig)
+        hipLaunchKernelGGL((k_col<T, MODE, LOG2N, NLC, true>), dim3(bc), dim3(GC::NT), 0, s, src, dst, rh, tw, p, lines,
+                           P, ls, stride, axis);
+    else
+        hipLaunchKernelGGL((k_col<T, MODE, LOG2N, NLC, false>), dim3(bc), dim3(GC::NT), 0, s, src, dst, rh, tw, p,
+                           lines, P, ls, stride, axis);
+}
+
+// y pass (2D: the last axis) and z pass (3D) of the whole half spectrum
+template <typename T, int MODE, int LOG2N>
+static void pass_y(const SpecLayout &L, typename Complex<T>::type *W, typename Complex<T>::type *RH,
+                   const typename Complex<T>::type *tw, const Params &p, int lines, hipStream_t s,
+                   typename Complex<T>::type *dst = nullptr) {
+    launch_col<T, MODE, LOG2N>(W, dst ? dst : W, RH, tw, p, lines, L.yP, L.yls, L.ystride, 1, L.ycontig, s);
+}
+template <typename T, int MODE, int LOG2N>
+static void pass_z(const SpecLayout &L, typename Complex<T>::type *W, typename Complex<T>::type *RH,
+                   const typename Complex<T>::type *tw, const Params &p, int lines, hipStream_t s,
+                   typename Complex<T>::type *dst = nullptr) {
+    launch_col<T, MODE, LOG2N>(W, dst ? dst : W, RH, tw, p, lines, L.zP, L.zls, L.zstride, 2, false, s);
+}
+
 // init: rho_hat = r2c(rho).  Otherwise `nsteps` time steps: the x pass of f'
 // (rho), then per step the axis passes with the fused spectral update, joined
 // to the next step by k_xrow; the last step ends with the c2r into rho.
@@ -396,10 +444,10 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1;
     constexpr int LX = LOG2N - 1;                     // x pass: complex length N / 2
     using GX = FftGeom<LX, PackedLines<LX>::value>;
-    constexpr int NLC = ColLines<LOG2N>::value;
-    using GC = FftGeom<LOG2N, NLC>;
     const int dims = p.dims;
+    const SpecLayout L = spec_layout(dims, N);
     const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
+    const int lines = dims == 3 ? N * Nh : Nh;
     const T2 *tw2 = (const T2 *)a.twiddles;
     T *rho = (T *)a.rho;
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
@@ -408,55 +456,46 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     const size_t bins = (size_t)rows * N;
     if(init) {
         hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, dims == 1 ? RH : W,
-                           tw2, p, rows);
-        if(dims == 2) {
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, RH, RH,
-                               tw2, p, Nh, (long)N * Nh, (long)Nh, 1);
-        }
-        else if(dims == 3) {
-            const int lines = N * Nh, bc = col_blocks(lines, NLC);
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, RH, RH, tw2, p, lines,
-                               (long)Nh, (long)N * Nh, 2);
+                           tw2, p, rows, L.rs, L.ks);
+        if(dims == 2) pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH);
+        if(dims == 3) {
+            pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s);
+            pass_z<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH);
         }
         return hipGetLastError();
     }
     if(nsteps < 1) return hipSuccess;
     if(landau)
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows);
+        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows, L.rs,
+                           L.ks);
     else
         hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p,
-                           rows);
+                           rows, L.rs, L.ks);
     for(int64_t t = 0; t < nsteps; t++) {
         if(dims == 1) {
             hipLaunchKernelGGL((k_update1d<T>), dim3((Nh + 255) / 256), dim3(256), 0, s, W, RH, p, Nh);
         }
         else if(dims == 2) {
-            hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, W, RH,
-                               tw2, p, Nh, (long)N * Nh, (long)Nh, 1);
+            pass_y<T, 2, LOG2N>(L, W, RH, tw2, p, lines, s);
         }
         else {
             // y forward (line l: plane l / Nh, column l % Nh), the fused z pass, y inverse
-            const int lines = N * Nh, bc = col_blocks(lines, NLC);
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
-            hipLaunchKernelGGL((k_col<T, 2, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                               (long)Nh, (long)N * Nh, 2);
-            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, RH, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
+            pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s);
+            pass_z<T, 2, LOG2N>(L, W, RH, tw2, p, lines, s);
+            pass_y<T, 1, LOG2N>(L, W, RH, tw2, p, lines, s);
         }
         if(t + 1 < nsteps) {
             T *ro = tape ? tape + (size_t)(t + 1) * bins : nullptr;
             if(landau)
-                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows, ro);
+                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows, L.rs,
+                                   L.ks, ro);
             else
                 hipLaunchKernelGGL((k_xrow<T, CH_SIMPLE_WERTHEIM, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows,
-                                   ro);
+                                   L.rs, L.ks, ro);
         }
     }
     hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tape ? tape + (size_t)nsteps * bins : rho, tw2,
-                       p, rows);
+                       p, rows, L.rs, L.ks);
     return hipGetLastError();
 }
 
@@ -503,40 +542,30 @@ static hipError_t spec_t(const SpectralArgs &a, const void *real_in, void *spec,
     const Params &p = a.p;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
     using GX = FftGeom<LX, PackedLines<LX>::value>;
-    constexpr int NLC = ColLines<LOG2N>::value;
-    using GC = FftGeom<LOG2N, NLC>;
     const int dims = p.dims;
+    const SpecLayout L = spec_layout(dims, N);
     const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
     const int bx = (rows + GX::NL - 1) / GX::NL;
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)spec;
     const int lines = dims == 3 ? N * Nh : Nh;
-    const int bc = col_blocks(lines, NLC);
     if(!inverse) {
         if(!fprime)
             hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
-                               tw2, p, rows);
+                               tw2, p, rows, L.rs, L.ks);
         else if(a.model == CH_LANDAU)
             hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
-                               tw2, p, rows);
+                               tw2, p, rows, L.rs, L.ks);
         else
             hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s,
-                               (const T *)real_in, W, tw2, p, rows);
-        if(dims >= 2)
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
-        if(dims == 3)
-            hipLaunchKernelGGL((k_col<T, 0, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
-                               (long)Nh, (long)N * Nh, 2);
+                               (const T *)real_in, W, tw2, p, rows, L.rs, L.ks);
+        if(dims >= 2) pass_y<T, 0, LOG2N>(L, W, W, tw2, p, lines, s);
+        if(dims == 3) pass_z<T, 0, LOG2N>(L, W, W, tw2, p, lines, s);
     }
     else {
-        if(dims == 3)
-            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
-                               (long)Nh, (long)N * Nh, 2);
-        if(dims >= 2)
-            hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(bc), dim3(GC::NT), 0, s, W, W, W, tw2, p, lines,
-                               (long)N * Nh, (long)Nh, 1);
-        hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, (T *)real_out, tw2, p, rows);
+        if(dims == 3) pass_z<T, 1, LOG2N>(L, W, W, tw2, p, lines, s);
+        if(dims >= 2) pass_y<T, 1, LOG2N>(L, W, W, tw2, p, lines, s);
+        hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, (T *)real_out, tw2, p, rows, L.rs, L.ks);
     }
     return hipGetLastError();
 }
@@ -576,13 +605,12 @@ static hipError_t fft2_inverse_t(const SpectralArgs &a, void *out, hipStream_t s
     using T2 = typename Complex<T>::type;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
     using GX = FftGeom<LX, PackedLines<LX>::value>;
-    constexpr int NLC = ColLines<LOG2N>::value;
-    using GC = FftGeom<LOG2N, NLC>;
+    const SpecLayout L = spec_layout(2, N);
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)a.work;
-    hipLaunchKernelGGL((k_col<T, 1, LOG2N, NLC>), dim3(col_blocks(Nh, NLC)), dim3(GC::NT), 0, s, W, W, W, tw2, a.p, Nh,
-                       (long)N * Nh, (long)Nh, 1);
-    hipLaunchKernelGGL((k_xinv<T, LX>), dim3((N + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, (T *)out, tw2, a.p, N);
+    pass_y<T, 1, LOG2N>(L, W, W, tw2, a.p, Nh, s);
+    hipLaunchKernelGGL((k_xinv<T, LX>), dim3((N + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, (T *)out, tw2, a.p, N,
+                       L.rs, L.ks);
     return hipGetLastError();
 }
 

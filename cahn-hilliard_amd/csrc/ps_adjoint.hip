@@ -21,7 +21,8 @@
 
 namespace chd {
 
-// half-spectrum entry q -> (half-axis index ix, iy, iz) as mode_q2 takes them
+// half-spectrum entry q = ((z N) + y) Nh + kx -> (half-axis index ix, iy, iz)
+// as mode_q2 takes them
 __device__ __forceinline__ void mode_index(const Params &p, int64_t q, int &ix, int &iy, int &iz) {
     const int Nh = p.N / 2 + 1;
     ix = (int)(q % Nh);
