@@ -182,6 +182,10 @@ static chd::Params make_params(const ch_desc *d) {
     p.inv_dx2 = 1.0 / (d->dx * d->dx);
     p.dt = d->dt;
     p.Mdt = d->M * d->dt;
+    {
+        const double q = 2.0 * 3.14159265358979323846 / (d->N * d->dx);
+        p.q2_unit = q * q;
+    }
     p.wave_cfg = chd::wave_default_cfg(d->N);
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);

@@ -23,6 +23,7 @@ struct Params {
     double M, rho_min, rho_ref;     // mobility
     double k, two_k, dx, inv_dx, inv_dx2, dt, S;
     double Mdt;                     // M dt (constant-mobility Euler update)
+    double q2_unit;                 // (2 pi / (N dx))^2: q^2 of a unit wave vector
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
 };
@@ -165,35 +166,33 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 }
 
 // ---------------------------------------------------------------- spectra
-// q^2 and the dealias factor of a half-spectrum mode, exactly as the reference
-// builds its tables (PseudospectralCPU.cpp:34-83); ix is the half-axis index
+// q^2 and the dealias factor of a half-spectrum mode (ix: half-axis index).
+// The reference's tables (PseudospectralCPU.cpp:34-83) hold q = 2 pi |n| / (N dx)
+// for the integer wave vector n, and keep modes with q < k_cut (2D) or
+// q <= k_cut (1D, 3D), k_cut = (2/3) pi / dx, i.e. 9 |n|^2 < N^2 (<=).  The
+// cut is tested on the integers: N is a power of two, so 9 |n|^2 never equals
+// N^2 and |9 |n|^2 - N^2| >= 1 keeps every floating-point evaluation of the
+// reference's test on the same side.  q^2 = q2_unit |n|^2 with
+// q2_unit = (2 pi / (N dx))^2: no square root.
 __device__ __forceinline__ void mode_q2(const Params &p, int ix, int iy, int iz, double &k2, double &deal) {
-    const double PI = 3.14159265358979323846;
     const int N = p.N;
-    const double Ndx = N * p.dx;
-    const double k_cut = N * PI / (N * p.dx) * 2.0 / 3.0;
+    int64_t n2;
     if(p.dims == 1) {
-        double k = 2.0 * PI * ix / Ndx;
-        deal = (k <= k_cut) ? 1.0 : 0.0;
-        k2 = k * k;
+        n2 = (int64_t)ix * ix;
     }
     else if(p.dims == 2) {
-        int kx = (iy < N / 2) ? iy : N - iy;
-        double k = 2.0 * PI * sqrt((double)(kx * kx + ix * ix)) / Ndx;
-        deal = (k < k_cut) ? 1.0 : 0.0;
-        k2 = k * k;
+        const int ky = (iy < N / 2) ? iy : N - iy;
+        n2 = (int64_t)ky * ky + (int64_t)ix * ix;
     }
     else {
-        int kx_i = (iz < N / 2) ? iz : iz - N;
-        int ky_i = (iy < N / 2) ? iy : iy - N;
-        double kx = 2.0 * PI * kx_i / Ndx, ky = 2.0 * PI * ky_i / Ndx, kz = 2.0 * PI * ix / Ndx;
-        k2 = kx * kx + ky * ky + kz * kz;
-        double k = sqrt(k2);
-        deal = (k <= k_cut) ? 1.0 : 0.0;
+        const int kx = (iz < N / 2) ? iz : iz - N;
+        const int ky = (iy < N / 2) ? iy : iy - N;
+        n2 = (int64_t)kx * kx + (int64_t)ky * ky + (int64_t)ix * ix;
     }
+    k2 = p.q2_unit * (double)n2;
+    const int64_t lhs = 9 * n2, rhs = (int64_t)N * N;
+    deal = (p.dims == 2 ? lhs < rhs : lhs <= rhs) ? 1.0 : 0.0;
 }
-
-// rh: this mode of rho_hat, updated in place; returns rho_hat / N^d
 
 // number of 2D wave-kernel region shapes selectable through Params::wave_cfg
 constexpr int kWaveCfgs = 7;

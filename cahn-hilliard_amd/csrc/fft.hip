@@ -19,6 +19,7 @@
 // several per block.  Twiddles come from an FP64 table W_N^m = e^{-2 pi i m/N}
 // built on the host from exact integer indices (rounded once for FP32).
 #include "ch_internal.h"
+#include "wave_fft.h"
 
 namespace chd {
 
@@ -132,11 +133,11 @@ __device__ __forceinline__ T2 spectral_update(const Params &p, int ix, int iy, i
     if(p.use_dealias) { fh.x *= (T)deal; fh.y *= (T)deal; }
     const T k2 = (T)k2d, k4 = k2 * k2;
     const T dt = (T)p.dt, M = (T)p.M, S = (T)p.S;
-    const T denom = (T)1.0 + dt * M * (S * k2 + (T)2.0 * (T)p.k * k4);
+    const T inv_denom = (T)1.0 / ((T)1.0 + dt * M * (S * k2 + (T)2.0 * (T)p.k * k4));
     const T c = dt * M * k2;
     T2 nw;
-    nw.x = (rh.x - c * (fh.x - S * rh.x)) / denom;
-    nw.y = (rh.y - c * (fh.y - S * rh.y)) / denom;
+    nw.x = (rh.x - c * (fh.x - S * rh.x)) * inv_denom;
+    nw.y = (rh.y - c * (fh.y - S * rh.y)) * inv_denom;
     rh = nw;
     return T2{nw.x * inv_bins, nw.y * inv_bins};
 }
@@ -377,6 +378,231 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
     rho_hat[i] = rh;
 }
 
+// ---- one wave per line (wave_fft.h), for lines of 512, 1024 or 2048 points.
+// Fewer LDS passes than LineFFT (two exchanges per transform, wave-local) and
+// the line's entries stay in registers between the forward transform, the
+// spectral update and the inverse transform.
+
+template <typename T2, int LOG2N, bool INV, int E>
+__device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, const T2 *tw, int tstride) {
+    static_assert(E == WaveFFT<T2, LOG2N>::E, "entries per lane");
+    using F = WaveFFT<T2, LOG2N>;
+    __syncthreads();
+    F::template pass1_write<INV>(lane, v, lds);
+    __syncthreads();
+    F::pass_read(lane, v, lds);
+    __syncthreads();
+    F::template pass2_write<INV>(lane, v, lds, tw, tstride);
+    __syncthreads();
+    F::pass_read(lane, v, lds);
+    F::template pass3_final<INV>(lane, v, tw, tstride);
+}
+
+// axis pass (as k_col), one line per 64-thread block
+template <typename T, int MODE, int LOG2N>
+__global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__restrict__ src,
+                                             typename Complex<T>::type *__restrict__ dst,
+                                             typename Complex<T>::type *__restrict__ rho_hat,
+                                             const typename Complex<T>::type *__restrict__ tw, Params p, int lines,
+                                             long P, long ls, long stride, int axis) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E;
+    __shared__ T2 lds[F::LDS];
+    const int l = xcd_tile(blockIdx.x, gridDim.x);
+    if(l >= lines) return;
+    const int lane = threadIdx.x;
+    const int Nh = p.N / 2 + 1;
+    const size_t base = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls;
+    const int tstride = p.N >> LOG2N;
+    T2 v[E], rh[MODE == 2 ? E : 1];
+#pragma unroll
+    for(int m = 0; m < E; m++) v[m] = src[base + (size_t)(lane + 64 * m) * stride];
+    if constexpr(MODE == 2) {
+#pragma unroll
+        for(int m = 0; m < E; m++) rh[m] = rho_hat[base + (size_t)(lane + 64 * m) * stride];
+    }
+    wave_transform<T2, LOG2N, MODE == 1>(lane, v, lds, tw, tstride);
+    if constexpr(MODE == 2) {
+        const T inv_bins = (T)p.inv_bins;
+        const int ix = l % Nh;
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const int j = lane + 64 * m;
+            int iy, iz;
+            if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
+            else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
+            v[m] = spectral_update<T, T2>(p, ix, iy, iz, v[m], rh[m], inv_bins);
+            rho_hat[base + (size_t)j * stride] = rh[m];
+        }
+        wave_transform<T2, LOG2N, true>(lane, v, lds, tw, tstride);
+    }
+#pragma unroll
+    for(int m = 0; m < E; m++) dst[base + (size_t)(lane + 64 * m) * stride] = v[m];
+}
+
+// c2r pre-processing of a row of n + 1 half-spectrum entries into the packed
+// n-point inverse transform input, in distribution D (k = lane + 64 m)
+template <typename T, int E>
+__device__ __forceinline__ void c2r_pre(const typename Complex<T>::type *__restrict__ in, long ks, int lane,
+                                        const typename Complex<T>::type *__restrict__ tw,
+                                        typename Complex<T>::type (&z)[E]) {
+    using T2 = typename Complex<T>::type;
+    constexpr int n = 64 * E;
+#pragma unroll
+    for(int m = 0; m < E; m++) {
+        const int k = lane + 64 * m;
+        T2 xk = in[(size_t)k * ks];
+        T2 xnk = in[(size_t)(n - k) * ks];
+        if(k == 0) { xk.y = 0; xnk.y = 0; }
+        // E = X_k + conj X_{n-k}; O = (X_k - conj X_{n-k}) conj(W_N^k); Z = E + i O
+        const T2 Ev = T2{xk.x + xnk.x, xk.y - xnk.y};
+        const T2 Ov = cmul(T2{xk.x - xnk.x, xk.y + xnk.y}, conj_(tw[k]));
+        z[m] = T2{Ev.x - Ov.y, Ev.y + Ov.x};
+    }
+}
+
+// r2c post-processing: Z (distribution D) -> n + 1 half-spectrum entries
+template <typename T, int LOG2N, typename T2>
+__device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T2 *__restrict__ tw,
+                                         T2 *__restrict__ out, long ks) {
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E, n = F::n;
+    __syncthreads();
+#pragma unroll
+    for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
+    __syncthreads();
+    auto emit = [&](int k, T2 zk, T2 znk) {
+        // E = (Z_k + conj Z_{n-k}) / 2, O = (Z_k - conj Z_{n-k}) / (2i), X_k = E + W_N^k O
+        const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
+        const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
+        const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
+        out[(size_t)k * ks] = cadd(Ev, cmul(tw[k], Ov));
+    };
+#pragma unroll
+    for(int m = 0; m < E; m++) {
+        const int k = lane + 64 * m;
+        emit(k, z[m], lds[F::pad(k == 0 ? 0 : n - k)]);
+    }
+    if(lane == 0) emit(n, z[0], z[0]);
+}
+
+// x pass, forward (as k_xfwd), one row per 64-thread block; n = 2^LOG2N = N / 2
+template <typename T, int MODEL, bool FPRIME, int LOG2N>
+__global__ __launch_bounds__(64) void k_xfwdw(const T *__restrict__ in, typename Complex<T>::type *__restrict__ out,
+                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              long rs, long ks) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E, n = F::n;
+    __shared__ T2 lds[F::LDS];
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    if(row >= rows) return;
+    const int lane = threadIdx.x;
+    const T2 *rin = reinterpret_cast<const T2 *>(in + (size_t)row * 2 * n);
+    T2 v[E];
+#pragma unroll
+    for(int m = 0; m < E; m++) {
+        T2 x = rin[lane + 64 * m];
+        if constexpr(FPRIME) x = T2{fprime<T, MODEL>(p, x.x), fprime<T, MODEL>(p, x.y)};
+        v[m] = x;
+    }
+    wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
+    r2c_post<T, LOG2N>(v, lds, lane, tw, out + (size_t)row * rs, ks);
+}
+
+// x pass, inverse (as k_xinv)
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(64) void k_xinvw(const typename Complex<T>::type *__restrict__ in, T *__restrict__ out,
+                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              long rs, long ks) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E, n = F::n;
+    __shared__ T2 lds[F::LDS];
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    if(row >= rows) return;
+    const int lane = threadIdx.x;
+    T2 v[E];
+    c2r_pre<T, E>(in + (size_t)row * rs, ks, lane, tw, v);
+    wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
+    T2 *rout = reinterpret_cast<T2 *>(out + (size_t)row * 2 * n);
+#pragma unroll
+    for(int m = 0; m < E; m++) rout[lane + 64 * m] = v[m];
+}
+
+// x pass between two steps (as k_xrow): c2r, f', r2c, in place on W
+template <typename T, int MODEL, int LOG2N>
+__global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restrict__ W,
+                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              long rs, long ks, T *__restrict__ rho_out) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E, n = F::n;
+    __shared__ T2 lds[F::LDS];
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    if(row >= rows) return;
+    const int lane = threadIdx.x;
+    T2 *w = W + (size_t)row * rs;
+    T2 v[E];
+    c2r_pre<T, E>(w, ks, lane, tw, v);
+    wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
+    if(rho_out) {
+        T2 *ro = reinterpret_cast<T2 *>(rho_out + (size_t)row * 2 * n);
+#pragma unroll
+        for(int m = 0; m < E; m++) ro[lane + 64 * m] = v[m];
+    }
+#pragma unroll
+    for(int m = 0; m < E; m++) v[m] = T2{fprime<T, MODEL>(p, v[m].x), fprime<T, MODEL>(p, v[m].y)};
+    wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
+    r2c_post<T, LOG2N>(v, lds, lane, tw, w, ks);
+}
+
+// launch wrappers: the wave kernels where the line length allows, else the LDS ones
+template <int LOG2N>
+constexpr bool wave_len() { return LOG2N >= 9 && LOG2N <= 11; }
+
+template <typename T, int MODEL, bool FPRIME, int LX>
+static void launch_xfwd(const T *in, typename Complex<T>::type *out, const typename Complex<T>::type *tw, const Params &p,
+                        int rows, long rs, long ks, hipStream_t s) {
+    if constexpr(wave_len<LX>()) {
+        hipLaunchKernelGGL((k_xfwdw<T, MODEL, FPRIME, LX>), dim3((rows + 7) / 8 * 8), dim3(64), 0, s, in, out, tw, p, rows,
+                           rs, ks);
+    }
+    else {
+        using GX = FftGeom<LX, PackedLines<LX>::value>;
+        hipLaunchKernelGGL((k_xfwd<T, MODEL, FPRIME, LX>), dim3((rows + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, in, out,
+                           tw, p, rows, rs, ks);
+    }
+}
+
+template <typename T, int LX>
+static void launch_xinv(const typename Complex<T>::type *in, T *out, const typename Complex<T>::type *tw, const Params &p,
+                        int rows, long rs, long ks, hipStream_t s) {
+    if constexpr(wave_len<LX>()) {
+        hipLaunchKernelGGL((k_xinvw<T, LX>), dim3((rows + 7) / 8 * 8), dim3(64), 0, s, in, out, tw, p, rows, rs, ks);
+    }
+    else {
+        using GX = FftGeom<LX, PackedLines<LX>::value>;
+        hipLaunchKernelGGL((k_xinv<T, LX>), dim3((rows + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, in, out, tw, p, rows,
+                           rs, ks);
+    }
+}
+
+template <typename T, int MODEL, int LX>
+static void launch_xrow(typename Complex<T>::type *W, const typename Complex<T>::type *tw, const Params &p, int rows,
+                        long rs, long ks, T *rho_out, hipStream_t s) {
+    if constexpr(wave_len<LX>()) {
+        hipLaunchKernelGGL((k_xroww<T, MODEL, LX>), dim3((rows + 7) / 8 * 8), dim3(64), 0, s, W, tw, p, rows, rs, ks,
+                           rho_out);
+    }
+    else {
+        using GX = FftGeom<LX, PackedLines<LX>::value>;
+        hipLaunchKernelGGL((k_xrow<T, MODEL, LX>), dim3((rows + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, tw, p, rows,
+                           rs, ks, rho_out);
+    }
+}
+
 // ------------------------------------------------------------------ host side
 // columns per block of the strided passes: NLC lines of n = N, NT = NLC n / 4
 // threads (<= 1024)
@@ -407,6 +633,12 @@ template <typename T, int MODE, int LOG2N>
 static void launch_col(const typename Complex<T>::type *src, typename Complex<T>::type *dst,
                        typename Complex<T>::type *rh, const typename Complex<T>::type *tw, const Params &p, int lines,
                        long P, long ls, long stride, int axis, bool contig, hipStream_t s) {
+    if constexpr(wave_len<LOG2N>()) {
+        (void)contig;
+        hipLaunchKernelGGL((k_colw<T, MODE, LOG2N>), dim3((lines + 7) / 8 * 8), dim3(64), 0, s, src, dst, rh, tw, p, lines,
+                           P, ls, stride, axis);
+        return;
+    }
     constexpr int NLC = ColLines<LOG2N>::value;
     using GC = FftGeom<LOG2N, NLC>;
     const int bc = col_blocks(lines, NLC);
@@ -455,8 +687,7 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     const bool landau = a.model == CH_LANDAU;
     const size_t bins = (size_t)rows * N;
     if(init) {
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, dims == 1 ? RH : W,
-                           tw2, p, rows, L.rs, L.ks);
+        launch_xfwd<T, CH_LANDAU, false, LX>(rho, dims == 1 ? RH : W, tw2, p, rows, L.rs, L.ks, s);
         if(dims == 2) pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH);
         if(dims == 3) {
             pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s);
@@ -466,11 +697,9 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     }
     if(nsteps < 1) return hipSuccess;
     if(landau)
-        hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p, rows, L.rs,
-                           L.ks);
+        launch_xfwd<T, CH_LANDAU, true, LX>(rho, W, tw2, p, rows, L.rs, L.ks, s);
     else
-        hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s, rho, W, tw2, p,
-                           rows, L.rs, L.ks);
+        launch_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>(rho, W, tw2, p, rows, L.rs, L.ks, s);
     for(int64_t t = 0; t < nsteps; t++) {
         if(dims == 1) {
             hipLaunchKernelGGL((k_update1d<T>), dim3((Nh + 255) / 256), dim3(256), 0, s, W, RH, p, Nh);
@@ -487,15 +716,12 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
         if(t + 1 < nsteps) {
             T *ro = tape ? tape + (size_t)(t + 1) * bins : nullptr;
             if(landau)
-                hipLaunchKernelGGL((k_xrow<T, CH_LANDAU, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows, L.rs,
-                                   L.ks, ro);
+                launch_xrow<T, CH_LANDAU, LX>(W, tw2, p, rows, L.rs, L.ks, ro, s);
             else
-                hipLaunchKernelGGL((k_xrow<T, CH_SIMPLE_WERTHEIM, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tw2, p, rows,
-                                   L.rs, L.ks, ro);
+                launch_xrow<T, CH_SIMPLE_WERTHEIM, LX>(W, tw2, p, rows, L.rs, L.ks, ro, s);
         }
     }
-    hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, tape ? tape + (size_t)nsteps * bins : rho, tw2,
-                       p, rows, L.rs, L.ks);
+    launch_xinv<T, LX>(W, tape ? tape + (size_t)nsteps * bins : rho, tw2, p, rows, L.rs, L.ks, s);
     return hipGetLastError();
 }
 
@@ -551,21 +777,18 @@ static hipError_t spec_t(const SpectralArgs &a, const void *real_in, void *spec,
     const int lines = dims == 3 ? N * Nh : Nh;
     if(!inverse) {
         if(!fprime)
-            hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, false, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
-                               tw2, p, rows, L.rs, L.ks);
+            launch_xfwd<T, CH_LANDAU, false, LX>((const T *)real_in, W, tw2, p, rows, L.rs, L.ks, s);
         else if(a.model == CH_LANDAU)
-            hipLaunchKernelGGL((k_xfwd<T, CH_LANDAU, true, LX>), dim3(bx), dim3(GX::NT), 0, s, (const T *)real_in, W,
-                               tw2, p, rows, L.rs, L.ks);
+            launch_xfwd<T, CH_LANDAU, true, LX>((const T *)real_in, W, tw2, p, rows, L.rs, L.ks, s);
         else
-            hipLaunchKernelGGL((k_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>), dim3(bx), dim3(GX::NT), 0, s,
-                               (const T *)real_in, W, tw2, p, rows, L.rs, L.ks);
+            launch_xfwd<T, CH_SIMPLE_WERTHEIM, true, LX>((const T *)real_in, W, tw2, p, rows, L.rs, L.ks, s);
         if(dims >= 2) pass_y<T, 0, LOG2N>(L, W, W, tw2, p, lines, s);
         if(dims == 3) pass_z<T, 0, LOG2N>(L, W, W, tw2, p, lines, s);
     }
     else {
         if(dims == 3) pass_z<T, 1, LOG2N>(L, W, W, tw2, p, lines, s);
         if(dims >= 2) pass_y<T, 1, LOG2N>(L, W, W, tw2, p, lines, s);
-        hipLaunchKernelGGL((k_xinv<T, LX>), dim3(bx), dim3(GX::NT), 0, s, W, (T *)real_out, tw2, p, rows, L.rs, L.ks);
+        launch_xinv<T, LX>(W, (T *)real_out, tw2, p, rows, L.rs, L.ks, s);
     }
     return hipGetLastError();
 }
@@ -609,8 +832,7 @@ static hipError_t fft2_inverse_t(const SpectralArgs &a, void *out, hipStream_t s
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)a.work;
     pass_y<T, 1, LOG2N>(L, W, W, tw2, a.p, Nh, s);
-    hipLaunchKernelGGL((k_xinv<T, LX>), dim3((N + GX::NL - 1) / GX::NL), dim3(GX::NT), 0, s, W, (T *)out, tw2, a.p, N,
-                       L.rs, L.ks);
+    launch_xinv<T, LX>(W, (T *)out, tw2, a.p, N, L.rs, L.ks, s);
     return hipGetLastError();
 }
 
