@@ -39,7 +39,8 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 // Landau: f' = -eps psi + psi^3 (src/models/Landau.cpp:46-52), evaluated in T.
 // SimpleWertheim: f' = 2 B2 rho + ln rho (or the linear branch below
 // regularisation_delta) + valence ln X(rho) (src/models/SimpleWertheim.cpp:40-53);
-// X(rho) and its logs are evaluated in FP64 in both precisions.
+// X(rho) and its logs are evaluated in FP64 in both precisions, above
+// regularisation_delta as the single logarithm ln(rho X^valence).
 __device__ __forceinline__ double wertheim_X(const Params &p, double r) {
     return (-1.0 + sqrt(1.0 + 2.0 * p.tvd * r)) / (p.tvd * r);
 }
@@ -51,12 +52,23 @@ __device__ __forceinline__ T fprime(const Params &p, T r) {
         return fma_t(r * r, r, -(T)p.eps * r);
     }
     else {
-        double rd = (double)r;
+        const double rd = (double)r;
         double d = 2 * p.B2 * rd;
-        if(rd < p.reg) d += p.log_reg + (rd - p.reg) / p.reg;
-        else d += log(rd);
-        double db = (rd > 0.) ? p.valence * log(wertheim_X(p, rd)) : 0.0;
-        return (T)(d + db);
+        if(rd < p.reg) {
+            d += p.log_reg + (rd - p.reg) / p.reg;
+            if(rd > 0.) d += p.valence * log(wertheim_X(p, rd));
+        }
+        else if(rd > 0.) {
+            // ln rho + valence ln X = ln(rho X^valence): one logarithm
+            const double X = wertheim_X(p, rd);
+            double xv = rd;
+            for(int i = 0; i < p.valence; i++) xv *= X;
+            d += log(xv);
+        }
+        else {
+            d += log(rd);
+        }
+        return (T)d;
     }
 }
 
