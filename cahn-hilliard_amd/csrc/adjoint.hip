@@ -391,8 +391,12 @@ static hipError_t adj_wave_cfg(const Params &p, int KA, const void *li, void *lo
 template <typename T, int MODEL>
 static hipError_t adj_wave(const Params &p, int KA, const void *li, void *lo, const void *tape, int s_top,
                            double *gacc, hipStream_t s) {
-    if(wave_default_cfg(p.N) == 5) return adj_wave_cfg<T, MODEL, WaveCfg5>(p, KA, li, lo, tape, s_top, gacc, s);
-    return adj_wave_cfg<T, MODEL, WaveCfg4>(p, KA, li, lo, tape, s_top, gacc, s);
+    switch(p.adj_cfg) {
+    case 0: return adj_wave_cfg<T, MODEL, WaveCfg0>(p, KA, li, lo, tape, s_top, gacc, s);
+    case 3: return adj_wave_cfg<T, MODEL, WaveCfg3>(p, KA, li, lo, tape, s_top, gacc, s);
+    case 5: return adj_wave_cfg<T, MODEL, WaveCfg5>(p, KA, li, lo, tape, s_top, gacc, s);
+    default: return adj_wave_cfg<T, MODEL, WaveCfg4>(p, KA, li, lo, tape, s_top, gacc, s);
+    }
 }
 
 static bool adj_is_wave(int dims, int N, bool em) { return dims == 2 && !em && N >= 64; }
@@ -423,12 +427,21 @@ int tape_depth(int dims, int N, bool em, int64_t remaining) {
 
 // number of tiles (gradient accumulator slots) of the adjoint launches: the
 // most any launch depth of this shape uses
-int adjoint_tiles(int dims, int N, bool em) {
+int adjoint_tiles(const Params &p, bool em) {
+    const int dims = p.dims, N = p.N;
     if(dims == 1) return N >= 64 ? N / 64 : (N >= 16 ? N / 16 : N / 4);
     if(adj_is_wave(dims, N, em)) {
         int m = 0;
-        for(int k = 1; k <= 8; k *= 2)
-            m = std::max(m, wave_default_cfg(N) == 5 ? wave_adj_tiles<WaveCfg5>(N, k) : wave_adj_tiles<WaveCfg4>(N, k));
+        for(int k = 1; k <= 8; k *= 2) {
+            int t;
+            switch(p.adj_cfg) {
+            case 0: t = wave_adj_tiles<WaveCfg0>(N, k); break;
+            case 3: t = wave_adj_tiles<WaveCfg3>(N, k); break;
+            case 5: t = wave_adj_tiles<WaveCfg5>(N, k); break;
+            default: t = wave_adj_tiles<WaveCfg4>(N, k); break;
+            }
+            m = std::max(m, t);
+        }
         return m;
     }
     if(dims == 2) return N >= 64 ? (N / 64) * (N / 16) : (N >= 16 ? (N / 16) * (N / 16) : (N / 4) * (N / 4));

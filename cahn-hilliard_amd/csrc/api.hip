@@ -187,6 +187,7 @@ static chd::Params make_params(const ch_desc *d) {
         p.q2_unit = q * q;
     }
     p.wave_cfg = chd::wave_default_cfg(d->N);
+    p.adj_cfg = chd::wave_default_cfg(d->N);
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
@@ -265,6 +266,10 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     s->tb = 1;
     while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
     if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
+    if(const char *r = std::getenv("CH_ADJ_CFG")) {
+        const int v = std::atoi(r);
+        if(v == 0 || v == 3 || v == 4 || v == 5) s->p.adj_cfg = v;
+    }
     // graph chunk: a multiple of 2 tb launches' worth of steps (parity restored)
     const char *g = std::getenv("CH_GRAPH_STEPS");
     s->graph_steps = g ? std::atoi(g) : 64;
@@ -1001,7 +1006,7 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
     const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;
-    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->d.dims, s->d.N, em);
+    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->p, em);
     const size_t hb = (size_t)s->hat * 2 * s->elt;
     const int64_t W = s->rec_window, T = s->rec_steps;
     const int64_t nck = (T + W - 1) / W;
@@ -1101,7 +1106,7 @@ static ch_status adjoint_prepare(ch_sys *s) {
         }
     }
     const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;
-    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->d.dims, s->d.N, s->d.integrator == CH_EULER_MOBILITY);
+    const int ntiles = ps ? chd::kPsGradBlocks : chd::adjoint_tiles(s->p, s->d.integrator == CH_EULER_MOBILITY);
     int64_t cap0 = s->lam[0] ? 1 : 0, cap1 = s->lam[1] ? 1 : 0, capg = s->gacc ? 1 : 0;
     if(!grow(&s->lam[0], cap0, 1, fb) || !grow(&s->lam[1], cap1, 1, fb) ||
        !grow((void **)&s->gacc, capg, 1, chd::kGradSlots * sizeof(double) * ntiles))

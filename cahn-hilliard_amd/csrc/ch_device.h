@@ -17,7 +17,8 @@ struct Params {
     int N, mask, log2N, dims;
     int valence;
     int use_dealias;
-    int wave_cfg;          // region shape of the 2D wave kernel (0, 1, 2)
+    int wave_cfg;          // region shape of the 2D wave kernel (0 .. kWaveCfgs - 1)
+    int adj_cfg;           // region shape of the 2D wave adjoint (0, 3, 4 or 5)
     double eps;            // Landau epsilon
     double B2, reg, log_reg, tvd;   // SimpleWertheim: tvd = 2 valence delta
     double M, rho_min, rho_ref;     // mobility

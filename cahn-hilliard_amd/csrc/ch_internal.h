@@ -76,7 +76,7 @@ constexpr int kGradSlots = 4;
 // depth (steps) of the next adjoint / recompute launch with `remaining` steps left
 int adjoint_depth(int dims, int N, bool em, int64_t remaining);
 int tape_depth(int dims, int N, bool em, int64_t remaining);
-int adjoint_tiles(int dims, int N, bool em);
+int adjoint_tiles(const Params &p, bool em);
 hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bool em, int KA, const void *lam_in,
                           void *lam_out, const void *tape, int s_top, double *gacc, hipStream_t s);
 // recompute: K states from `in` into K consecutive tape fields (K from tape_depth)
