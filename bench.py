@@ -286,6 +286,44 @@ def w_wertheim(ch, ctx, args):
             "fwd_plus_adjoint_over_fwd": round(tot_grad / tot, 3)}
 
 
+def w_noise(ch, ctx, args):
+    """euler_mobility with conserved noise (mobility.with_noise, Philox normals),
+    FP64, one launch per time step: Landau 2D N=1024 (constant M = 1, dt = 0.01,
+    noise_rescale_factor = 0.01) and the C5 shape SimpleWertheim 3D N=128
+    (M = rho / 0.05, dt = 1e-4, noise_rescale_factor = 1e-3).  Algorithmic HBM
+    bytes: one read and one write of the field per time step (16 B / cell)."""
+    from cahn_hilliard_amd.init import initial_density
+    out = {}
+    shapes = {
+        "landau2d_N1024": (dict(dims=2, N=1024, integrator="euler_mobility", free_energy="landau", epsilon=0.9,
+                                mobility="constant", M=1.0, dt=0.01, dx=1.0, k=1.0, with_noise=True,
+                                noise_rescale_factor=0.01, seed=123456), initial_density(2, 1024, 0.0, 0.01, 0, 123456)),
+        "wertheim3d_N128": (dict(dims=3, N=128, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0,
+                                 delta=1e4, valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0,
+                                 with_noise=True, noise_rescale_factor=1e-3, seed=123456),
+                            initial_density(3, 128, 0.05, 0.05, 0, 123456)),
+    }
+    nst = 640 if args.steps >= 100 else 128
+    for name, (kw, rho0) in shapes.items():
+        s = ch.System(ctx, ch.make_desc(**kw))
+        s.upload(rho0)
+        s.evolve(128)  # graph capture + warm-up
+        sync(ctx)
+        t0 = time.perf_counter()
+        s.evolve(nst)
+        sync(ctx)
+        dt = time.perf_counter() - t0
+        mass0 = float(np.sum(rho0))
+        rho = s.download()
+        s.close()
+        cells = kw["N"] ** kw["dims"]
+        out[name] = {"Gcell-updates/s": round(cells * nst / dt / 1e9, 3), "us_per_time_step": round(dt / nst * 1e6, 3),
+                     "effective_GB/s": round(16.0 * cells * nst / dt / 1e9, 1), "time_steps": nst,
+                     "mass_rel_drift": float(f"{abs(float(np.sum(rho)) - mass0) / max(abs(mass0), 1.0):.3e}"),
+                     "finite": bool(np.isfinite(rho).all())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,7 +335,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--workload", default=None,
                     choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
-                             "wertheim", "copy"])
+                             "wertheim", "noise", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -311,7 +349,8 @@ def main():
              "pseudospectral": lambda: w_pseudospectral(ch, ctx, args, "fp64"),
              "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
              "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
-             "wertheim": lambda: w_wertheim(ch, ctx, args), "copy": lambda: w_copy(ch, ctx, args)}
+             "wertheim": lambda: w_wertheim(ch, ctx, args), "noise": lambda: w_noise(ch, ctx, args),
+             "copy": lambda: w_copy(ch, ctx, args)}
     if args.workload and args.workload != "euler":
         r = table[args.workload]()
         print(json.dumps({"workload": args.workload, "result": {k: v for k, v in r.items() if k != "wall"}}), flush=True)
@@ -353,7 +392,7 @@ def main():
                                "us_per_time_step": round(e32["wall"] / (e32["steps"] * args.inner) * 1e6, 3),
                                "fp32_rel_linf_vs_fp64": fp32_error(ch, ctx, "euler", 0.01, N, 1000),
                                "error_after_steps": 1000}
-        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim"):
+        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise"):
             extra[name] = table[name]()
         extra["pseudospectral_fp32"]["fp32_rel_linf_vs_fp64"] = fp32_error(ch, ctx, "pseudospectral", 0.1, N, 1000)
         extra["pseudospectral_fp32"]["error_after_steps"] = 1000

@@ -32,7 +32,8 @@ class ChDesc(ctypes.Structure):
                 ("epsilon", ctypes.c_double), ("B2", ctypes.c_double), ("delta", ctypes.c_double),
                 ("regularisation_delta", ctypes.c_double), ("M", ctypes.c_double), ("rho_min", ctypes.c_double),
                 ("rho_ref", ctypes.c_double), ("dt", ctypes.c_double), ("dx", ctypes.c_double),
-                ("k", ctypes.c_double), ("S", ctypes.c_double)]
+                ("k", ctypes.c_double), ("S", ctypes.c_double), ("with_noise", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("noise_rescale_factor", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
 class CHError(RuntimeError):

@@ -54,6 +54,7 @@ struct ch_sys {
     int64_t ckpt_hat_cap = 0;
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
+    int64_t *noise_step = nullptr;  // euler_mobility noise: index of the next step (device)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
@@ -155,8 +156,12 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     int64_t bins = 1;
     for(int i = 0; i < d->dims; i++) bins *= d->N;
     if(bins > ((int64_t)1 << 31)) return fail(ctx, CH_EUNSUPPORTED, "grid too large (%lld cells)", (long long)bins);
+    if(d->integrator == CH_EULER_MOBILITY && d->with_noise && !std::isfinite(d->noise_rescale_factor))
+        return fail(ctx, CH_EINVAL, "noise_rescale_factor must be finite");
     return CH_OK;
 }
+
+static bool has_noise(const ch_sys *s) { return s->d.integrator == CH_EULER_MOBILITY && s->d.with_noise; }
 
 static chd::Params make_params(const ch_desc *d) {
     chd::Params p{};
@@ -192,6 +197,11 @@ static chd::Params make_params(const ch_desc *d) {
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
     p.inv_bins = 1.0 / bins;
+    // EulerMobilityCPU.cpp:14
+    p.noise = (d->integrator == CH_EULER_MOBILITY && d->with_noise)
+                  ? std::sqrt(2.0 / (std::pow(d->dx, d->dims) * d->dt)) * d->noise_rescale_factor
+                  : 0.0;
+    p.seed = d->seed;
     return p;
 }
 
@@ -235,7 +245,7 @@ static void free_sys(ch_sys *s) {
         if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
-    void *bufs[] = {s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
+    void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
                     s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc, s->ckpt_hat, s->ps_rh, s->ps_w,
                     s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1]};
     for(void *b : bufs)
@@ -260,7 +270,9 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     s->hat = s->bins / N * (N / 2 + 1);
     // temporal blocking depth: kDefaultTB, or CH_TB_STEPS (environment) when
     // set, rounded down to a depth the shape has (1 for the pseudospectral path)
-    const int max_k = desc->integrator == CH_PSEUDOSPECTRAL ? 1 : chd::stencil_max_k(desc->dims, desc->N);
+    // (1 with noise: one launch per step)
+    const bool noisy = desc->integrator == CH_EULER_MOBILITY && desc->with_noise;
+    const int max_k = (desc->integrator == CH_PSEUDOSPECTRAL || noisy) ? 1 : chd::stencil_max_k(desc->dims, desc->N);
     int want = kDefaultTB;
     if(const char *tb = std::getenv("CH_TB_STEPS")) want = std::atoi(tb);
     s->tb = 1;
@@ -281,6 +293,8 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
               alloc(&s->stage2, s->bins * s->elt) && alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
               alloc((void **)&s->result, 2 * sizeof(double));
     if(ok && desc->integrator != CH_PSEUDOSPECTRAL) ok = alloc(&s->psi[1], s->bins * s->elt);
+    if(ok && noisy) ok = alloc((void **)&s->noise_step, sizeof(int64_t)) &&
+                         hipMemset(s->noise_step, 0, sizeof(int64_t)) == hipSuccess;
     if(ok && desc->integrator == CH_PSEUDOSPECTRAL) {
         ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->rho_hat, s->hat * 2 * s->elt) &&
              alloc(&s->twiddles, (size_t)N * 2 * s->elt);
@@ -399,7 +413,8 @@ extern "C" ch_status ch_sys_download_mobility(ch_sys *s, double *mob_host) {
 }
 
 // K time steps (one launch) queued on the stream
-static hipError_t launch_steps(ch_sys *s, int &cur, int K) {
+// (with noise: K = 1, step index *noise_step + noise_off)
+static hipError_t launch_steps(ch_sys *s, int &cur, int K, int noise_off = 0) {
     hipStream_t st = s->ctx->stream;
     if(s->d.integrator == CH_PSEUDOSPECTRAL) {
         chd::SpectralArgs a = spectral_args(s);
@@ -409,6 +424,12 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K) {
     void *out = s->psi[cur ^ 1];
     const ch_desc &d = s->d;
     hipError_t e;
+    if(has_noise(s)) {
+        // K = 1; the caller advances the device step index after the launches
+        e = chd::em_noise_step(s->p, d.precision, d.model, d.mobility, in, out, s->noise_step, noise_off, st);
+        cur ^= 1;
+        return e;
+    }
     switch(d.dims) {
     case 1: e = chd::stencil_step<1>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     case 2: e = chd::stencil_step<2>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
@@ -444,11 +465,45 @@ static ch_status evolve_spectral(ch_sys *s, int64_t nsteps) {
     return CH_OK;
 }
 
+// euler_mobility with noise: one launch per step, the step index read from
+// the device counter plus the launch's offset in its chain; the counter is
+// advanced by one small kernel after each chain (G steps captured once)
+static ch_status evolve_noise(ch_sys *s, int64_t nsteps) {
+    ch_ctx *c = s->ctx;
+    const int G = s->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        const int par = s->cur;
+        if(!s->gexec[par]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int cur = s->cur;
+            hipError_t e = hipSuccess;
+            for(int i = 0; i < G && e == hipSuccess; i++) e = launch_steps(s, cur, 1, i);
+            if(e == hipSuccess) e = chd::noise_step_advance(s->noise_step, G, c->stream);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&s->gexec[par], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[par], c->stream));
+        nsteps -= reps * G;  // G is even: parity unchanged
+    }
+    if(nsteps > 0) {
+        for(int64_t i = 0; i < nsteps; i++) HIPCHK(c, launch_steps(s, s->cur, 1, (int)i));
+        HIPCHK(c, chd::noise_step_advance(s->noise_step, (int)nsteps, c->stream));
+    }
+    return CH_OK;
+}
+
 extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
     if(!s || nsteps < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     if(s->d.integrator == CH_PSEUDOSPECTRAL) return evolve_spectral(s, nsteps);
+    if(has_noise(s)) return evolve_noise(s, nsteps);
     const int G = s->graph_steps;
     if(G > 0 && nsteps >= G) {
         const int par = s->cur;
@@ -940,6 +995,7 @@ static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count,
 extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every) {
     if(!s || nsteps < 1 || ckpt_every < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
+    if(has_noise(s)) return fail(c, CH_EUNSUPPORTED, "reverse mode does not cover euler_mobility with noise");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
     const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;

@@ -27,6 +27,8 @@ struct Params {
     double q2_unit;                 // (2 pi / (N dx))^2: q^2 of a unit wave vector
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
+    double noise;                   // sqrt(2 / (dx^dims dt)) noise_rescale_factor (0: no noise)
+    uint64_t seed;                  // key of the noise generator
 };
 
 // Every per-cell expression below is written with contraction off and

@@ -11,6 +11,11 @@ template <int DIMS>
 hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, int K, const void *in,
                         void *out, hipStream_t s);
 inline int stencil_max_k(int dims, int N) { return (dims == 2 && N >= 64) ? 8 : 1; }
+// one euler_mobility step with conserved noise (noise.hip), in -> out; the
+// step index is *step_base + step_off.  noise_step_advance adds n to *step_base.
+hipError_t em_noise_step(const Params &p, int precision, int model, int mob, const void *in, void *out,
+                         const int64_t *step_base, int step_off, hipStream_t s);
+hipError_t noise_step_advance(int64_t *step_base, int n, hipStream_t s);
 
 // pseudospectral pipeline
 struct SpectralArgs {

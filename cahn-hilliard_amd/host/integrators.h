@@ -14,6 +14,7 @@
 #pragma once
 
 #include <cmath>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -141,9 +142,21 @@ class EulerMobilityHIP : public HIPIntegrator<dims> {
 public:
     EulerMobilityHIP(SimulationState<dims> &s, FreeEnergyModel *m, const chtoml::Table &c)
         : HIPIntegrator<dims>(s, m, c, CH_EULER_MOBILITY) {
-        if(this->template _config_optional_value<bool>(c, "mobility.with_noise", false))
-            this->critical("mobility.with_noise is not part of this build");
-        this->info("Integrating the Cahn-Hilliard equation with non-constant mobility and without noise");
+        // src/integrators/EulerMobilityCPU.cpp:10-22 (keys, defaults, noise factor)
+        if(this->template _config_optional_value<bool>(c, "mobility.with_noise", false)) {
+            this->_desc.with_noise = 1;
+            this->_desc.noise_rescale_factor =
+                this->template _config_optional_value<double>(c, "mobility.noise_rescale_factor", 1.0);
+            this->_desc.seed =
+                (uint64_t)this->template _config_optional_value<long long>(c, "seed", (long long)std::time(NULL));
+            double dxd = 1.0;
+            for(int d = 0; d < dims; d++) dxd *= this->_dx;
+            this->info("Integrating the Cahn-Hilliard equation with non-constant mobility and noise (noise_factor = {})",
+                       std::sqrt(2.0 / (dxd * this->_dt)) * this->_desc.noise_rescale_factor);
+        }
+        else {
+            this->info("Integrating the Cahn-Hilliard equation with non-constant mobility and without noise");
+        }
     }
     std::string_view name() const override { return "EulerMobilityHIP"; }
 

@@ -10,6 +10,8 @@ SimpleWertheim.cpp:16-33, IMobility.cpp:18-57, PseudospectralCPU.cpp:23-25).
 """
 import ctypes
 import math
+import time
+import weakref
 
 import numpy as np
 
@@ -29,6 +31,8 @@ class Context:
         if st != L.CH_OK:
             raise L.CHError(st, f"ch_ctx_create(device={device}) failed (is a GPU visible?)")
         self.handle = h
+        # systems and batches created on this context: closed before it
+        self._children = weakref.WeakSet()
 
     def synchronize(self):
         _check(self.handle, L.lib().ch_ctx_synchronize(self.handle))
@@ -39,6 +43,8 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             L.lib().ch_ctx_destroy(self.handle)
             self.handle = None
 
@@ -51,7 +57,8 @@ class Context:
 
 def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64", dt=0.01, dx=1.0, k=1.0,
               epsilon=0.9, B2=1.0, delta=1e4, valence=4, regularisation_delta=0.0, mobility="constant", M=1.0,
-              rho_min=0.0, rho_ref=1.0, S=0.0, use_dealias=False):
+              rho_min=0.0, rho_ref=1.0, S=0.0, use_dealias=False, with_noise=False, noise_rescale_factor=1.0,
+              seed=0):
     d = L.ChDesc()
     d.dims, d.N = dims, N
     d.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision] if isinstance(precision, str) else precision
@@ -62,6 +69,7 @@ def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64
     d.epsilon, d.B2, d.delta, d.regularisation_delta = epsilon, B2, delta, regularisation_delta
     d.M, d.rho_min, d.rho_ref = M, rho_min, rho_ref
     d.dt, d.dx, d.k, d.S = dt, dx, k, S
+    d.with_noise, d.noise_rescale_factor, d.seed = int(bool(with_noise)), noise_rescale_factor, seed
     return d
 
 
@@ -114,7 +122,9 @@ def desc_from_config(cfg, dims, precision="fp64"):
                      dx=get("dx", 1.0) * u2i, k=k * u2i ** 5, mobility=mob, M=get("mobility.M", 1.0) / u2i,
                      rho_min=get("mobility.rho_min", 0.0) / u2i, rho_ref=get("mobility.rho_ref", 1.0),
                      S=get("pseudospectral.S", 0.0) * u2i ** 3, use_dealias=get("pseudospectral.use_dealias", False),
-                     **kw)
+                     with_noise=get("mobility.with_noise", False),
+                     noise_rescale_factor=get("mobility.noise_rescale_factor", 1.0),
+                     seed=int(get("seed", int(time.time()))) & 0xFFFFFFFFFFFFFFFF, **kw)
 
 
 class System:
@@ -125,6 +135,7 @@ class System:
         h = ctypes.c_void_p()
         _check(ctx.handle, L.lib().ch_sys_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
+        ctx._children.add(self)
         self.bins = desc.N ** desc.dims
 
     def _chk(self, st):
@@ -239,6 +250,7 @@ class Batch:
         h = ctypes.c_void_p()
         _check(ctx.handle, L.lib().ch_batch_create(ctx.handle, len(descs), arr, ctypes.byref(h)))
         self.handle, self.n = h, len(descs)
+        ctx._children.add(self)
         self.bins = [d.N ** d.dims for d in descs]
 
     def _chk(self, st):

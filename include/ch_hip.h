@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 1
+#define CH_ABI_VERSION 2
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -67,6 +67,17 @@ typedef struct ch_desc {
     double rho_ref;        /* mobility.rho_ref (linear)                           */
     double dt, dx, k;      /* dt, dx, k (the step uses mu = f' - 2 k lap)         */
     double S;              /* pseudospectral.S                                    */
+    /* conserved noise of euler_mobility (EulerMobilityCPU.cpp:10-18, 65-68):
+     * each face flux gains sqrt(max(M_face, 0)) sqrt(2 / (dx^dims dt))
+     * noise_rescale_factor eta with eta a standard normal.  eta comes from a
+     * counter-based generator, Philox4x32-10 keyed by `seed` with the counter
+     * (cell index, step index), so a run is reproducible for any launch shape;
+     * the reference's mt19937 / curand streams are not reproduced.  Ignored
+     * by the other integrators, as in the reference. */
+    int32_t with_noise;            /* mobility.with_noise (0 / 1)                 */
+    int32_t reserved0;             /* 0                                           */
+    double noise_rescale_factor;   /* mobility.noise_rescale_factor (default 1)   */
+    uint64_t seed;                 /* seed                                        */
 } ch_desc;
 
 typedef struct ch_ctx ch_ctx;
@@ -79,6 +90,7 @@ const char *ch_status_string(ch_status s);
 /* device + stream; replaces the implicit CUDA context and legacy default
  * stream of src/CUDA/integrators/CUDAIntegrator.cu:13-33 */
 ch_status ch_ctx_create(int device, ch_ctx **out);
+/* every system and batch created on ctx must be destroyed first */
 void ch_ctx_destroy(ch_ctx *ctx);
 const char *ch_ctx_last_error(const ch_ctx *ctx);
 ch_status ch_ctx_synchronize(ch_ctx *ctx);

@@ -881,3 +881,81 @@ void och_ps_adjoint(const och_params *p, const och_model *m, const double *rho0,
     for(int64_t t = steps - 1; t >= 0; t--) och_ps_adjoint_step(p, m, tape + t * bins, dL_drho0, grad);
     free(tape);
 }
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:  */
+/* as easy as 1, 2, 3", SC 2011): 10 rounds of                          */
+/*   (hi0, lo0) = M0 * c0, (hi1, lo1) = M1 * c2,                         */
+/*   c = (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0),                       */
+/* with the key bumped by the Weyl constants (W0, W1) between rounds.   */
+/* ------------------------------------------------------------------ */
+void och_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for(int r = 0; r < 10; r++) {
+        if(r > 0) { k0 += W0; k1 += W1; }
+        const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* The four standard normals of cell `cell` at step `step`: Philox of the
+ * counter (cell lo, cell hi, step lo, step hi) under the key (seed lo, seed
+ * hi), uniforms u = (x + 1/2) 2^-32 in (0, 1), Box-Muller on (u0, u1) and
+ * (u2, u3).  Normal d drives the stochastic flux of the cell's +e_d face. */
+void och_cell_normals(uint64_t seed, int64_t cell, int64_t step, double nrm[4]) {
+    const uint32_t ctr[4] = {(uint32_t)cell, (uint32_t)((uint64_t)cell >> 32), (uint32_t)step,
+                             (uint32_t)((uint64_t)step >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t x[4];
+    och_philox4x32_10(ctr, key, x);
+    for(int h = 0; h < 2; h++) {
+        const double u1 = ((double)x[2 * h] + 0.5) * (1.0 / 4294967296.0);
+        const double u2 = ((double)x[2 * h + 1] + 0.5) * (1.0 / 4294967296.0);
+        const double r = sqrt(-2.0 * log(u1)), th = 2.0 * OCH_PI * u2;
+        nrm[2 * h] = r * cos(th);
+        nrm[2 * h + 1] = r * sin(th);
+    }
+}
+
+/* euler_mobility with conserved noise (src/integrators/EulerMobilityCPU.cpp:
+ * 10-18, 55-83): the +e_d face flux of cell i gains
+ * sqrt(max(M_face, 0)) * sqrt(2 / (dx^d dt)) * rescale * eta_d(i, step),
+ * and rho += dt (div J_det + div J_sto). */
+void och_euler_mobility_noise_step(const och_params *p, const och_model *m, double noise_rescale, uint64_t seed,
+                                   int64_t step, double *rho) {
+    int dims = p->dims, N = p->N;
+    int64_t bins = ipow(N, dims);
+    double *mob = (double *)malloc(sizeof(double) * bins);
+    double *mu = (double *)malloc(sizeof(double) * bins);
+    double *flux = (double *)malloc(sizeof(double) * bins * dims);
+    double *sto = (double *)malloc(sizeof(double) * bins * dims);
+    const double nf = sqrt(2.0 / (pow(p->dx, dims) * p->dt)) * noise_rescale;
+    och_update_mobility(p, m, rho, mob, bins);
+    for(int64_t i = 0; i < bins; i++) mu[i] = och_der_bulk_free_energy(m, rho[i]) - 2.0 * p->k * cell_laplacian(dims, N, p->dx, rho, i);
+    for(int64_t i = 0; i < bins; i++) {
+        double nrm[4];
+        och_cell_normals(seed, i, step, nrm);
+        for(int d = 0; d < dims; d++) {
+            int64_t ip = neighbour(dims, N, i, d, 1);
+            double M_face = 0.5 * (mob[i] + mob[ip]);
+            flux[i * dims + d] = M_face * (mu[ip] - mu[i]) / p->dx;
+            sto[i * dims + d] = sqrt(M_face > 0.0 ? M_face : 0.0) * nf * nrm[d];
+        }
+    }
+    for(int64_t i = 0; i < bins; i++) {
+        double det = 0, st = 0;
+        for(int d = 0; d < dims; d++) {
+            int64_t im = neighbour(dims, N, i, d, -1);
+            det += (flux[i * dims + d] - flux[im * dims + d]) / p->dx;
+            st += (sto[i * dims + d] - sto[im * dims + d]) / p->dx;
+        }
+        rho[i] += (det + st) * p->dt;
+    }
+    free(mob); free(mu); free(flux); free(sto);
+}

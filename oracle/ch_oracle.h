@@ -120,6 +120,13 @@ void och_adjoint(int integrator, const och_params *p, const och_model *m, const 
 void och_ps_adjoint_step(const och_params *p, const och_model *m, const double *rho, double *lambda, double *grad);
 void och_ps_adjoint(const och_params *p, const och_model *m, const double *rho0, int64_t steps,
                     const double *dL_drhoT, double *dL_drho0, double *grad);
+/* ---- conserved noise of euler_mobility (counter-based Philox4x32-10) ---- */
+void och_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void och_cell_normals(uint64_t seed, int64_t cell, int64_t step, double nrm[4]);
+/* one euler_mobility step with noise; `step` is the step's index since the
+ * system was created */
+void och_euler_mobility_noise_step(const och_params *p, const och_model *m, double noise_rescale, uint64_t seed,
+                                   int64_t step, double *rho);
 /* F = average_free_energy(rho) with dF/drho (may be NULL) and the explicit
  * dF/dparams[OCH_NGRADS] (may be NULL) */
 double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,

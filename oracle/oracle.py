@@ -105,6 +105,9 @@ def lib(fast=False):
         "och_free_energy_grad": (D, [PP, MP, P, P, P]),
         "och_ps_adjoint_step": (None, [PP, MP, P, P, P]),
         "och_ps_adjoint": (None, [PP, MP, P, I64, P, P, P]),
+        "och_philox4x32_10": (None, [P, P, P]),
+        "och_cell_normals": (None, [ctypes.c_uint64, I64, I64, P]),
+        "och_euler_mobility_noise_step": (None, [PP, MP, D, ctypes.c_uint64, I64, P]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(L, n)
@@ -313,3 +316,26 @@ def free_energy_grad(params, model, rho):
     dp = np.zeros(NGRADS)
     F = _L().och_free_energy_grad(ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(rho), _ptr(g), _ptr(dp))
     return F, g, dp
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    _L().och_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def cell_normals(seed, cell, step):
+    out = np.zeros(4)
+    _L().och_cell_normals(seed, cell, step, _ptr(out))
+    return out
+
+
+def run_noise(params, model, rho, steps, noise_rescale=1.0, seed=123456, step0=0):
+    """`steps` euler_mobility steps with conserved noise (step indices step0, ...)."""
+    rho = np.array(rho, dtype=np.float64, copy=True)
+    for t in range(steps):
+        _L().och_euler_mobility_noise_step(ctypes.byref(params.c()), ctypes.byref(model.c()), noise_rescale, seed,
+                                           step0 + t, _ptr(rho))
+    return rho

@@ -28,7 +28,7 @@ def test_every_header_in_include_is_exported(ch):
 
 def test_abi_version_and_status_strings_without_gpu(ch):
     L = ch._lib.lib()
-    assert L.ch_abi_version() == 1
+    assert L.ch_abi_version() == 2
     assert L.ch_status_string(3) == b"NaN encountered"
 
 
@@ -54,3 +54,31 @@ def test_desc_from_reference_config(ch):
     assert d.mobility == 1 and d.valence == 4 and d.delta > 0
     with pytest.raises(KeyError):
         ch.desc_from_config({"N": 8, "dt": 0.1, "free_energy": "landau", "landau": {}}, dims=2)
+
+
+def test_noise_keys_from_reference_config(ch):
+    # src/integrators/EulerMobilityCPU.cpp:10-18: mobility.with_noise,
+    # mobility.noise_rescale_factor (default 1.0), seed
+    cfg = {"N": 64, "dt": 0.01, "k": 1.0, "free_energy": "landau", "landau": {"epsilon": 0.9},
+           "integrator": "euler_mobility", "seed": 1234,
+           "mobility": {"type": "constant", "with_noise": True, "noise_rescale_factor": 0.25}}
+    d = ch.desc_from_config(cfg, dims=2)
+    assert (d.with_noise, d.noise_rescale_factor, d.seed) == (1, 0.25, 1234)
+    del cfg["mobility"]["noise_rescale_factor"]
+    assert ch.desc_from_config(cfg, dims=2).noise_rescale_factor == 1.0
+    assert ch.desc_from_config({**cfg, "mobility": {"type": "constant"}}, dims=2).with_noise == 0
+
+
+def test_desc_layout_matches_header(ch, tmp_path):
+    """sizeof / offsetof of every ch_desc field, from the C header, equal the ctypes mirror."""
+    fields = [f for f, _ in ch._lib.ChDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ch_hip.h"\nint main(void) {\n'
+                   '    printf("%zu\\n", sizeof(ch_desc));\n' +
+                   "".join(f'    printf("%zu\\n", offsetof(ch_desc, {f}));\n' for f in fields) + "    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(ch._lib.ChDesc)] + [getattr(ch._lib.ChDesc, f).offset for f in fields]
+    assert got == want
