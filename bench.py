@@ -251,9 +251,12 @@ def w_wertheim(ch, ctx, args):
     delta = 1e4, k = 1, dt = 1e-4, M = rho / 0.05, FP64; the 8 replicas run one
     after the other (seeds 123456..123463).  Then, per replica, dF(psi_T)/d delta:
     record (checkpoint every 50 steps), the free-energy gradient as the seed, the
-    reverse sweep, plus the explicit dF/d delta."""
+    reverse sweep, plus the explicit dF/d delta.  The configuration's 2000 steps
+    drive rho below 0 near step 450 (f' = ln rho then fails: the oracle's FP64
+    restatement of EulerMobilityCPU gives the same values and the same NaN),
+    so the workload runs 400 steps."""
     from cahn_hilliard_amd.init import initial_density
-    N, nst = 128, 2000 if args.steps >= 100 else 200
+    N, nst = 128, 400 if args.steps >= 100 else 200
     d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
                      valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
     tot = tot_grad = 0.0
@@ -262,6 +265,7 @@ def w_wertheim(ch, ctx, args):
     # untimed set-up: graphs, checkpoints, tapes and the recompute stream
     s.upload(initial_density(3, N, 0.05, 0.05, 0, 123456))
     s.evolve(nst)
+    s.upload(initial_density(3, N, 0.05, 0.05, 0, 123456))
     s.record(nst, 50)
     s.adjoint(s.free_energy_grad()[1])
     for r in range(8):
@@ -303,11 +307,12 @@ def w_noise(ch, ctx, args):
                                  with_noise=True, noise_rescale_factor=1e-3, seed=123456),
                             initial_density(3, 128, 0.05, 0.05, 0, 123456)),
     }
-    nst = 640 if args.steps >= 100 else 128
     for name, (kw, rho0) in shapes.items():
+        # the 3D shape stays below the step (~450) where its rho turns negative
+        warm, nst = (128, 640 if args.steps >= 100 else 128) if kw["dims"] == 2 else (64, 256)
         s = ch.System(ctx, ch.make_desc(**kw))
         s.upload(rho0)
-        s.evolve(128)  # graph capture + warm-up
+        s.evolve(warm)  # graph capture + warm-up
         sync(ctx)
         t0 = time.perf_counter()
         s.evolve(nst)
@@ -324,6 +329,50 @@ def w_noise(ch, ctx, args):
     return out
 
 
+def w_psm(ch, ctx, args):
+    """pseudospectral_mobility, Landau 2D N=1024 FP64 (eps = 0.9, k = 1, dx = 1,
+    psi0 = 0.5 + U[-0.15, 0.15], regularised mobility M psi / (psi + 0.2)):
+    the splitting scheme (dt = 0.02, S = 0.5, dealiased; one captured chain per
+    64 steps) and the GMRES scheme (dt = 0.05, gmres_tol = 1e-10)."""
+    N = 1024
+    rng = np.random.default_rng(123456)
+    rho0 = 0.5 + 0.15 * (2 * rng.random(N * N) - 1)
+    base = dict(dims=2, N=N, integrator="pseudospectral_mobility", free_energy="landau", epsilon=0.9, k=1.0, dx=1.0,
+                mobility="regularised", M=1.0, rho_min=0.2, S=0.5, use_dealias=True)
+    out = {}
+    s = ch.System(ctx, ch.make_desc(dt=0.02, **base))
+    s.upload(rho0)
+    s.evolve(64)
+    sync(ctx)
+    nst = 256 if args.steps >= 100 else 64
+    t0 = time.perf_counter()
+    s.evolve(nst)
+    sync(ctx)
+    dt = time.perf_counter() - t0
+    rho = s.download()
+    s.close()
+    out["splitting"] = {"us_per_time_step": round(dt / nst * 1e6, 2), "Gcell-updates/s": round(N * N * nst / dt / 1e9, 3),
+                        "time_steps": nst, "finite": bool(np.isfinite(rho).all()),
+                        "mass_rel_drift": float(f"{abs(rho.sum() - rho0.sum()) / rho0.sum():.3e}")}
+    s = ch.System(ctx, ch.make_desc(dt=0.05, use_gmres=True, gmres_tol=1e-10, **base))
+    s.upload(rho0)
+    s.evolve(1)
+    sync(ctx)
+    nst = 8 if args.steps >= 100 else 2
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(nst):
+        s.evolve(1)
+        its.append(s.gmres_iterations)
+    sync(ctx)
+    dt = time.perf_counter() - t0
+    rho = s.download()
+    s.close()
+    out["gmres"] = {"ms_per_time_step": round(dt / nst * 1e3, 3), "iterations_per_step": its,
+                    "ms_per_iteration": round(dt / max(1, sum(its)) * 1e3, 3), "finite": bool(np.isfinite(rho).all())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,7 +384,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--workload", default=None,
                     choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
-                             "wertheim", "noise", "copy"])
+                             "wertheim", "noise", "psm", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -350,6 +399,7 @@ def main():
              "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
              "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
              "wertheim": lambda: w_wertheim(ch, ctx, args), "noise": lambda: w_noise(ch, ctx, args),
+             "psm": lambda: w_psm(ch, ctx, args),
              "copy": lambda: w_copy(ch, ctx, args)}
     if args.workload and args.workload != "euler":
         r = table[args.workload]()
@@ -392,11 +442,18 @@ def main():
                                "us_per_time_step": round(e32["wall"] / (e32["steps"] * args.inner) * 1e6, 3),
                                "fp32_rel_linf_vs_fp64": fp32_error(ch, ctx, "euler", 0.01, N, 1000),
                                "error_after_steps": 1000}
-        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise"):
-            extra[name] = table[name]()
-        extra["pseudospectral_fp32"]["fp32_rel_linf_vs_fp64"] = fp32_error(ch, ctx, "pseudospectral", 0.1, N, 1000)
-        extra["pseudospectral_fp32"]["error_after_steps"] = 1000
-        extra["batch"]["vs_fp64_single"] = round(extra["batch"]["Gcell-updates/s"] / r["gcell"], 3)
+        # a failing extra workload is reported, never allowed to drop the headline line
+        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise", "psm"):
+            try:
+                extra[name] = table[name]()
+            except Exception as e:  # noqa: BLE001
+                extra[name] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            extra["pseudospectral_fp32"]["fp32_rel_linf_vs_fp64"] = fp32_error(ch, ctx, "pseudospectral", 0.1, N, 1000)
+            extra["pseudospectral_fp32"]["error_after_steps"] = 1000
+            extra["batch"]["vs_fp64_single"] = round(extra["batch"]["Gcell-updates/s"] / r["gcell"], 3)
+        except Exception as e:  # noqa: BLE001
+            extra["error"] = f"{type(e).__name__}: {e}"
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N)

@@ -13,13 +13,14 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ch_hip.h")
 
 CH_OK, CH_EINVAL, CH_EHIP, CH_ENAN, CH_EUNSUPPORTED, CH_ENOMEM, CH_ESTATE = range(7)
 FP64, FP32 = 0, 1
-EULER, EULER_MOBILITY, PSEUDOSPECTRAL = 0, 1, 2
+EULER, EULER_MOBILITY, PSEUDOSPECTRAL, PSEUDOSPECTRAL_MOBILITY = 0, 1, 2, 3
 LANDAU, SIMPLE_WERTHEIM = 0, 1
 MOB_CONSTANT, MOB_FREE_ENERGY, MOB_REGULARISED, MOB_LINEAR = 0, 1, 2, 3
 # parameter-gradient slots of ch_sys_adjoint / ch_sys_free_energy_grad
 GRAD_EPSILON, GRAD_K, GRAD_M, GRAD_DELTA, GRAD_B2, NGRADS = 0, 1, 2, 3, 4, 5
 
-INTEGRATORS = {"euler": EULER, "euler_mobility": EULER_MOBILITY, "pseudospectral": PSEUDOSPECTRAL}
+INTEGRATORS = {"euler": EULER, "euler_mobility": EULER_MOBILITY, "pseudospectral": PSEUDOSPECTRAL,
+               "pseudospectral_mobility": PSEUDOSPECTRAL_MOBILITY}
 MODELS = {"landau": LANDAU, "simple_wertheim": SIMPLE_WERTHEIM}
 MOBILITIES = {"constant": MOB_CONSTANT, "free_energy": MOB_FREE_ENERGY, "regularised": MOB_REGULARISED,
               "linear": MOB_LINEAR}
@@ -33,7 +34,9 @@ class ChDesc(ctypes.Structure):
                 ("regularisation_delta", ctypes.c_double), ("M", ctypes.c_double), ("rho_min", ctypes.c_double),
                 ("rho_ref", ctypes.c_double), ("dt", ctypes.c_double), ("dx", ctypes.c_double),
                 ("k", ctypes.c_double), ("S", ctypes.c_double), ("with_noise", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("noise_rescale_factor", ctypes.c_double), ("seed", ctypes.c_uint64)]
+                ("reserved0", ctypes.c_int32), ("noise_rescale_factor", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("use_gmres", ctypes.c_int32), ("gmres_restart", ctypes.c_int32), ("gmres_max_iter", ctypes.c_int32),
+                ("reserved1", ctypes.c_int32), ("gmres_tol", ctypes.c_double)]
 
 
 class CHError(RuntimeError):
@@ -89,6 +92,7 @@ def lib():
         "ch_sys_download_mobility": (ctypes.c_int, [P, P]),
         "ch_sys_evolve": (ctypes.c_int, [P, I64]),
         "ch_sys_steps_per_launch": (ctypes.c_int32, [P]),
+        "ch_sys_gmres_iterations": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
         "ch_sq_nshells": (I32, [I32]),
         "ch_sys_structure_factor": (ctypes.c_int, [P, P, I32]),

@@ -55,6 +55,13 @@ struct ch_sys {
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
     int64_t *noise_step = nullptr;  // euler_mobility noise: index of the next step (device)
+    // pseudospectral_mobility: spectra, fields, reductions and the GMRES workspace
+    void *psm_spec[4] = {nullptr, nullptr, nullptr, nullptr};
+    void *psm_field[3] = {nullptr, nullptr, nullptr};
+    void *psm_vec[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // V, w, b, x, rhs, r, av
+    double *psm_dev = nullptr;   // kPsmParts partials, 4 scalars, Hcol, y
+    double *psm_host = nullptr;  // pinned, kPsmParts doubles
+    int gmres_iters = 0;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
@@ -137,12 +144,12 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     if(d->N < 4) return fail(ctx, CH_EUNSUPPORTED, "N must be at least 4 (got %d)", d->N);
     if(d->precision != CH_FP64 && d->precision != CH_FP32) return fail(ctx, CH_EINVAL, "bad precision %d", d->precision);
     if(d->model != CH_LANDAU && d->model != CH_SIMPLE_WERTHEIM) return fail(ctx, CH_EINVAL, "bad model %d", d->model);
-    if(d->integrator < CH_EULER || d->integrator > CH_PSEUDOSPECTRAL)
+    if(d->integrator < CH_EULER || d->integrator > CH_PSEUDOSPECTRAL_MOBILITY)
         return fail(ctx, CH_EINVAL, "bad integrator %d", d->integrator);
     if(d->mobility < CH_MOB_CONSTANT || d->mobility > CH_MOB_LINEAR)
         return fail(ctx, CH_EINVAL, "bad mobility type %d", d->mobility);
     // Integrator::validate (src/integrators/Integrator.cpp:42-46)
-    if(d->integrator != CH_EULER_MOBILITY && d->mobility != CH_MOB_CONSTANT)
+    if(d->integrator != CH_EULER_MOBILITY && d->integrator != CH_PSEUDOSPECTRAL_MOBILITY && d->mobility != CH_MOB_CONSTANT)
         return fail(ctx, CH_EINVAL, "The selected integrator only supports constant mobility");
     // FreeEnergyModel::set_mobility default (src/models/FreeEnergyModel.h:51-53)
     if(d->mobility == CH_MOB_FREE_ENERGY && d->model != CH_SIMPLE_WERTHEIM)
@@ -151,7 +158,10 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     if(d->model == CH_SIMPLE_WERTHEIM && (d->valence <= 0 || !(d->delta > 0.0)))
         return fail(ctx, CH_EINVAL, "simple_wertheim needs valence > 0 and delta > 0");
     if(!(d->dt > 0.0) || !(d->dx > 0.0)) return fail(ctx, CH_EINVAL, "dt and dx must be positive");
-    if(d->integrator == CH_PSEUDOSPECTRAL && d->N > 4096)
+    if(d->integrator == CH_PSEUDOSPECTRAL_MOBILITY && d->use_gmres &&
+       (d->gmres_restart < 0 || d->gmres_max_iter < 0 || !(d->gmres_tol >= 0.0)))
+        return fail(ctx, CH_EINVAL, "gmres_restart, gmres_max_iter and gmres_tol must not be negative");
+    if((d->integrator == CH_PSEUDOSPECTRAL || d->integrator == CH_PSEUDOSPECTRAL_MOBILITY) && d->N > 4096)
         return fail(ctx, CH_EUNSUPPORTED, "pseudospectral supports N <= 4096 (got %d)", d->N);
     int64_t bins = 1;
     for(int i = 0; i < d->dims; i++) bins *= d->N;
@@ -162,6 +172,9 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
 }
 
 static bool has_noise(const ch_sys *s) { return s->d.integrator == CH_EULER_MOBILITY && s->d.with_noise; }
+static bool is_spectral(int integrator) {
+    return integrator == CH_PSEUDOSPECTRAL || integrator == CH_PSEUDOSPECTRAL_MOBILITY;
+}
 
 static chd::Params make_params(const ch_desc *d) {
     chd::Params p{};
@@ -190,6 +203,7 @@ static chd::Params make_params(const ch_desc *d) {
     {
         const double q = 2.0 * 3.14159265358979323846 / (d->N * d->dx);
         p.q2_unit = q * q;
+        p.k_unit = q;
     }
     p.wave_cfg = chd::wave_default_cfg(d->N);
     p.adj_cfg = chd::wave_default_cfg(d->N);
@@ -250,6 +264,14 @@ static void free_sys(ch_sys *s) {
                     s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1]};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
+    for(void *b : s->psm_spec)
+        if(b) (void)hipFree(b);
+    for(void *b : s->psm_field)
+        if(b) (void)hipFree(b);
+    for(void *b : s->psm_vec)
+        if(b) (void)hipFree(b);
+    if(s->psm_dev) (void)hipFree(s->psm_dev);
+    if(s->psm_host) (void)hipHostFree(s->psm_host);
     delete s;
 }
 
@@ -272,7 +294,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     // set, rounded down to a depth the shape has (1 for the pseudospectral path)
     // (1 with noise: one launch per step)
     const bool noisy = desc->integrator == CH_EULER_MOBILITY && desc->with_noise;
-    const int max_k = (desc->integrator == CH_PSEUDOSPECTRAL || noisy) ? 1 : chd::stencil_max_k(desc->dims, desc->N);
+    const int max_k = (is_spectral(desc->integrator) || noisy) ? 1 : chd::stencil_max_k(desc->dims, desc->N);
     int want = kDefaultTB;
     if(const char *tb = std::getenv("CH_TB_STEPS")) want = std::atoi(tb);
     s->tb = 1;
@@ -292,10 +314,21 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     bool ok = alloc(&s->psi[0], s->bins * s->elt) && alloc(&s->stage, s->bins * 8) &&
               alloc(&s->stage2, s->bins * s->elt) && alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
               alloc((void **)&s->result, 2 * sizeof(double));
-    if(ok && desc->integrator != CH_PSEUDOSPECTRAL) ok = alloc(&s->psi[1], s->bins * s->elt);
+    if(ok && !is_spectral(desc->integrator)) ok = alloc(&s->psi[1], s->bins * s->elt);
     if(ok && noisy) ok = alloc((void **)&s->noise_step, sizeof(int64_t)) &&
                          hipMemset(s->noise_step, 0, sizeof(int64_t)) == hipSuccess;
-    if(ok && desc->integrator == CH_PSEUDOSPECTRAL) {
+    if(ok && desc->integrator == CH_PSEUDOSPECTRAL_MOBILITY) {
+        const int m = std::max(2, desc->gmres_restart > 0 ? desc->gmres_restart : 30);
+        for(void *&b : s->psm_spec) ok = ok && alloc(&b, s->hat * 2 * s->elt);
+        for(void *&b : s->psm_field) ok = ok && alloc(&b, s->bins * s->elt);
+        if(desc->use_gmres) {
+            ok = ok && alloc(&s->psm_vec[0], (size_t)(m + 1) * s->bins * s->elt);
+            for(int i = 1; i < 7; i++) ok = ok && alloc(&s->psm_vec[i], s->bins * s->elt);
+        }
+        ok = ok && alloc((void **)&s->psm_dev, sizeof(double) * (chd::kPsmParts + 4 + 2 * (m + 2))) &&
+             hipHostMalloc((void **)&s->psm_host, sizeof(double) * std::max(chd::kPsmParts, m + 2)) == hipSuccess;
+    }
+    if(ok && is_spectral(desc->integrator)) {
         ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->rho_hat, s->hat * 2 * s->elt) &&
              alloc(&s->twiddles, (size_t)N * 2 * s->elt);
         if(ok) {
@@ -341,7 +374,7 @@ static chd::SpectralArgs spectral_args(ch_sys *s) {
 static ch_status after_upload(ch_sys *s) {
     ch_ctx *c = s->ctx;
     s->cur = 0;
-    if(s->d.integrator == CH_PSEUDOSPECTRAL) {
+    if(is_spectral(s->d.integrator)) {
         chd::SpectralArgs a = spectral_args(s);
         HIPCHK(c, chd::ps_run(a, c->stream, true));
     }
@@ -465,6 +498,75 @@ static ch_status evolve_spectral(ch_sys *s, int64_t nsteps) {
     return CH_OK;
 }
 
+static chd::PsmArgs psm_args(ch_sys *s) {
+    chd::PsmArgs B{};
+    B.a = spectral_args(s);
+    B.mob_rule = s->d.mobility;
+    B.fh = s->psm_spec[0];
+    B.muh = s->psm_spec[1];
+    B.gh = s->psm_spec[2];
+    B.divh = s->psm_spec[3];
+    B.g = s->psm_field[0];
+    B.mob = s->psm_field[1];
+    B.tr = s->psm_field[2];
+    B.restart = s->d.gmres_restart > 0 ? s->d.gmres_restart : 30;
+    B.max_iter = s->d.gmres_max_iter > 0 ? s->d.gmres_max_iter : 200;
+    B.tol = s->d.gmres_tol > 0.0 ? s->d.gmres_tol : 1e-10;
+    if(s->d.precision == CH_FP32) B.tol = std::max(B.tol, 1e-6);
+    const int m = std::max(2, B.restart);
+    B.partials = s->psm_dev;
+    B.scal = s->psm_dev + chd::kPsmParts;
+    B.Hcol = B.scal + 4;
+    B.ydev = B.Hcol + (m + 2);
+    B.host_partials = s->psm_host;
+    B.V = s->psm_vec[0];
+    B.w = s->psm_vec[1];
+    B.b = s->psm_vec[2];
+    B.x = s->psm_vec[3];
+    B.rhs = s->psm_vec[4];
+    B.r = s->psm_vec[5];
+    B.av = s->psm_vec[6];
+    return B;
+}
+
+// pseudospectral_mobility: the splitting scheme captures a chain of G steps
+// once (no parity: one real buffer) and replays it; the GMRES scheme runs
+// step by step (its restarts are decided on the host)
+static ch_status evolve_psm(ch_sys *s, int64_t nsteps) {
+    ch_ctx *c = s->ctx;
+    const chd::PsmArgs B = psm_args(s);
+    if(s->d.use_gmres) {
+        for(int64_t i = 0; i < nsteps; i++) {
+            int it = 0;
+            HIPCHK(c, chd::psm_step_full(B, c->stream, it));
+            s->gmres_iters = it;
+        }
+        return CH_OK;
+    }
+    const int G = s->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        if(!s->gexec[0]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for(int i = 0; i < G && e == hipSuccess; i++) e = chd::psm_step_simple(B, c->stream);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&s->gexec[0], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[0], c->stream));
+        nsteps -= reps * G;
+    }
+    for(int64_t i = 0; i < nsteps; i++) HIPCHK(c, chd::psm_step_simple(B, c->stream));
+    return CH_OK;
+}
+
+extern "C" int32_t ch_sys_gmres_iterations(const ch_sys *s) { return s ? s->gmres_iters : 0; }
+
 // euler_mobility with noise: one launch per step, the step index read from
 // the device counter plus the launch's offset in its chain; the counter is
 // advanced by one small kernel after each chain (G steps captured once)
@@ -503,6 +605,7 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     if(s->d.integrator == CH_PSEUDOSPECTRAL) return evolve_spectral(s, nsteps);
+    if(s->d.integrator == CH_PSEUDOSPECTRAL_MOBILITY) return evolve_psm(s, nsteps);
     if(has_noise(s)) return evolve_noise(s, nsteps);
     const int G = s->graph_steps;
     if(G > 0 && nsteps >= G) {
@@ -996,6 +1099,8 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     if(!s || nsteps < 1 || ckpt_every < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     if(has_noise(s)) return fail(c, CH_EUNSUPPORTED, "reverse mode does not cover euler_mobility with noise");
+    if(s->d.integrator == CH_PSEUDOSPECTRAL_MOBILITY)
+        return fail(c, CH_EUNSUPPORTED, "reverse mode does not cover pseudospectral_mobility");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
     const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;

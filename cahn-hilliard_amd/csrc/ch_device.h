@@ -27,6 +27,7 @@ struct Params {
     double q2_unit;                 // (2 pi / (N dx))^2: q^2 of a unit wave vector
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
+    double k_unit;                  // 2 pi / (N dx): wave number of a unit index
     double noise;                   // sqrt(2 / (dx^dims dt)) noise_rescale_factor (0: no noise)
     uint64_t seed;                  // key of the noise generator
 };
@@ -215,6 +216,32 @@ constexpr int kWaveCfgs = 7;
 // 64 x 64 regions of 16 waves below, where more, smaller blocks fill the CUs
 // (measured on MI355X over shapes 0-6 at 8 steps per launch)
 __host__ __device__ constexpr int wave_default_cfg(int N) { return N >= 512 ? 5 : 4; }
+
+// half-spectrum entry q = ((z N) + y) Nh + kx -> (half-axis index ix, iy, iz)
+// as mode_q2 takes them
+__device__ __forceinline__ void mode_index(const Params &p, int64_t q, int &ix, int &iy, int &iz) {
+    const int Nh = p.N / 2 + 1;
+    ix = (int)(q % Nh);
+    const int64_t rest = q / Nh;
+    if(p.dims == 3) {
+        iy = (int)(rest % p.N);
+        iz = (int)(rest / p.N);
+    }
+    else {
+        iy = (int)rest;
+        iz = 0;
+    }
+}
+
+// signed wave number of `axis` (0 = slowest) of a half-spectrum entry, in
+// units of 2 pi / (N dx): the x (half) axis runs 0 .. N/2, a full axis uses
+// i < N/2 ? i : i - N, so its Nyquist entry is -N/2
+// (PseudospectralMobilityCPU.cpp:56-125)
+__device__ __forceinline__ int axis_wavenumber(const Params &p, int ix, int iy, int iz, int axis) {
+    if(axis == p.dims - 1) return ix;
+    const int i = (p.dims == 3 && axis == 0) ? iz : iy;
+    return i < p.N / 2 ? i : i - p.N;
+}
 
 template <typename T> struct Complex;
 template <> struct Complex<double> { using type = double2; };

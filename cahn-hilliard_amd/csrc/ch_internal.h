@@ -53,6 +53,28 @@ hipError_t sq_shells(int precision, const void *W, int N, int nshells, double *s
 hipError_t sq_scale(int precision, void *W, int N, const double *sq, const double *target, const double *cnt,
                     hipStream_t s);
 
+// pseudospectral_mobility (psm.hip): buffers of one system
+constexpr int kPsmParts = 1024;
+struct PsmArgs {
+    SpectralArgs a;               // rho = the real field, rho_hat = the spectral state, twiddles
+    int mob_rule;                 // CH_MOB_*
+    void *fh, *muh, *gh, *divh;   // half spectra (T2)
+    void *g, *mob, *tr;           // fields (T)
+    double *partials;             // kPsmParts reduction partials
+    double *scal;                 // scal[0] = M0 = max(0, max M)
+    // full scheme (GMRES)
+    int restart, max_iter;
+    double tol;
+    void *V;                      // restart + 1 Krylov vectors (T)
+    void *w, *b, *x, *rhs, *r, *av;
+    double *Hcol, *ydev;          // restart + 2 doubles each (device)
+    double *host_partials;        // pinned host memory, kPsmParts doubles
+};
+// one splitting step, asynchronous (capturable)
+hipError_t psm_step_simple(const PsmArgs &B, hipStream_t s);
+// one GMRES step, synchronous; iters = GMRES iterations
+hipError_t psm_step_full(const PsmArgs &B, hipStream_t s, int &iters);
+
 // FP64 observables: partials -> result[0] = sum rho, result[1] = sum f-density
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
                        double *result, hipStream_t s);

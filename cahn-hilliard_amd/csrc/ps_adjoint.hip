@@ -21,22 +21,6 @@
 
 namespace chd {
 
-// half-spectrum entry q = ((z N) + y) Nh + kx -> (half-axis index ix, iy, iz)
-// as mode_q2 takes them
-__device__ __forceinline__ void mode_index(const Params &p, int64_t q, int &ix, int &iy, int &iz) {
-    const int Nh = p.N / 2 + 1;
-    ix = (int)(q % Nh);
-    const int64_t rest = q / Nh;
-    if(p.dims == 3) {
-        iy = (int)(rest % p.N);
-        iz = (int)(rest / p.N);
-    }
-    else {
-        iy = (int)rest;
-        iz = 0;
-    }
-}
-
 // accumulates the block's sums of two per-thread values into slots s0, s1
 __device__ __forceinline__ void add_block_pair(double a, double b, double *__restrict__ slots, int s0, int s1) {
     __shared__ double red[2][4];

@@ -88,7 +88,9 @@ public:
             integrator = std::make_unique<EulerMobilityHIP<dims>>(_sim_state, m, config);
         else if(user_integrator == "pseudospectral")
             integrator = std::make_unique<PseudospectralHIP<dims>>(_sim_state, m, config);
-        else if(user_integrator == "bailo" || user_integrator == "pseudospectral_mobility")
+        else if(user_integrator == "pseudospectral_mobility")
+            integrator = std::make_unique<PseudospectralMobilityHIP<dims>>(_sim_state, m, config);
+        else if(user_integrator == "bailo")
             critical("Unsupported CUDA integrator {}", user_integrator);
         else critical("Unsupported integrator {}", user_integrator);
         integrator->validate();

@@ -178,4 +178,25 @@ public:
     std::string_view name() const override { return "PseudospectralHIP"; }
 };
 
+template <int dims>
+class PseudospectralMobilityHIP : public HIPIntegrator<dims> {
+public:
+    PseudospectralMobilityHIP(SimulationState<dims> &s, FreeEnergyModel *m, const chtoml::Table &c)
+        : HIPIntegrator<dims>(s, m, c, CH_PSEUDOSPECTRAL_MOBILITY) {
+        // src/integrators/PseudospectralMobilityCPU.cpp:22-23, 203-208 (S is not rescaled here)
+        this->_desc.S = this->template _config_optional_value<double>(c, "pseudospectral.S", 0.0);
+        this->_desc.use_dealias = this->template _config_optional_value<bool>(c, "pseudospectral.use_dealias", false);
+        this->_desc.gmres_restart = this->template _config_optional_value<int>(c, "pseudospectral.gmres_restart", 30);
+        this->_desc.gmres_max_iter = this->template _config_optional_value<int>(c, "pseudospectral.gmres_max_iter", 200);
+        this->_desc.gmres_tol = this->template _config_optional_value<double>(c, "pseudospectral.gmres_tol", 1e-10);
+        this->_desc.use_gmres = this->template _config_optional_value<bool>(c, "pseudospectral.use_gmres", false);
+        this->info("S = {}, use_dealias = {}, use_gmres = {}", this->_desc.S, this->_desc.use_dealias,
+                   this->_desc.use_gmres);
+    }
+    std::string_view name() const override { return "PseudospectralMobilityHIP"; }
+
+protected:
+    bool _supports_nonconstant_mobility() const override { return true; }
+};
+
 }  // namespace ch

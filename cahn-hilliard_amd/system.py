@@ -58,7 +58,7 @@ class Context:
 def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64", dt=0.01, dx=1.0, k=1.0,
               epsilon=0.9, B2=1.0, delta=1e4, valence=4, regularisation_delta=0.0, mobility="constant", M=1.0,
               rho_min=0.0, rho_ref=1.0, S=0.0, use_dealias=False, with_noise=False, noise_rescale_factor=1.0,
-              seed=0):
+              seed=0, use_gmres=False, gmres_restart=30, gmres_max_iter=200, gmres_tol=1e-10):
     d = L.ChDesc()
     d.dims, d.N = dims, N
     d.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision] if isinstance(precision, str) else precision
@@ -70,6 +70,8 @@ def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64
     d.M, d.rho_min, d.rho_ref = M, rho_min, rho_ref
     d.dt, d.dx, d.k, d.S = dt, dx, k, S
     d.with_noise, d.noise_rescale_factor, d.seed = int(bool(with_noise)), noise_rescale_factor, seed
+    d.use_gmres, d.gmres_restart, d.gmres_max_iter, d.gmres_tol = int(bool(use_gmres)), gmres_restart, gmres_max_iter, \
+        gmres_tol
     return d
 
 
@@ -121,10 +123,18 @@ def desc_from_config(cfg, dims, precision="fp64"):
                      free_energy=model, precision=precision, dt=get("dt", required=True),
                      dx=get("dx", 1.0) * u2i, k=k * u2i ** 5, mobility=mob, M=get("mobility.M", 1.0) / u2i,
                      rho_min=get("mobility.rho_min", 0.0) / u2i, rho_ref=get("mobility.rho_ref", 1.0),
-                     S=get("pseudospectral.S", 0.0) * u2i ** 3, use_dealias=get("pseudospectral.use_dealias", False),
+                     # pseudospectral rescales S (PseudospectralCPU.cpp:22-23); pseudospectral_mobility
+                     # does not (PseudospectralMobilityCPU.cpp:22)
+                     S=get("pseudospectral.S", 0.0) * (u2i ** 3 if get("integrator", "euler") == "pseudospectral"
+                                                       else 1.0),
+                     use_dealias=get("pseudospectral.use_dealias", False),
                      with_noise=get("mobility.with_noise", False),
                      noise_rescale_factor=get("mobility.noise_rescale_factor", 1.0),
-                     seed=int(get("seed", int(time.time()))) & 0xFFFFFFFFFFFFFFFF, **kw)
+                     seed=int(get("seed", int(time.time()))) & 0xFFFFFFFFFFFFFFFF,
+                     use_gmres=get("pseudospectral.use_gmres", False),
+                     gmres_restart=int(get("pseudospectral.gmres_restart", 30)),
+                     gmres_max_iter=int(get("pseudospectral.gmres_max_iter", 200)),
+                     gmres_tol=get("pseudospectral.gmres_tol", 1e-10), **kw)
 
 
 class System:
@@ -171,6 +181,11 @@ class System:
     @property
     def steps_per_launch(self):
         return int(L.lib().ch_sys_steps_per_launch(self.handle))
+
+    @property
+    def gmres_iterations(self):
+        """GMRES iterations of the last pseudospectral_mobility step (use_gmres)."""
+        return int(L.lib().ch_sys_gmres_iterations(self.handle))
 
     def observables(self):
         m, f = ctypes.c_double(), ctypes.c_double()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 2
+#define CH_ABI_VERSION 3
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -41,7 +41,7 @@ typedef enum ch_status {
 
 enum { CH_FP64 = 0, CH_FP32 = 1 };
 /* `integrator` key: src/CahnHilliard.cpp:169-220 */
-enum { CH_EULER = 0, CH_EULER_MOBILITY = 1, CH_PSEUDOSPECTRAL = 2 };
+enum { CH_EULER = 0, CH_EULER_MOBILITY = 1, CH_PSEUDOSPECTRAL = 2, CH_PSEUDOSPECTRAL_MOBILITY = 3 };
 /* `free_energy` key: src/main.cpp:44-62 */
 enum { CH_LANDAU = 0, CH_SIMPLE_WERTHEIM = 1 };
 /* `mobility.type` key: src/mobility/IMobility.cpp:18-57; LINEAR (M rho / rho_ref) is new */
@@ -53,7 +53,8 @@ typedef struct ch_desc {
     int32_t dims;          /* 1, 2 or 3                                           */
     int32_t N;             /* cells per side, a power of two >= 4                 */
     int32_t precision;     /* CH_FP64 or CH_FP32 (field arithmetic)              */
-    int32_t integrator;    /* CH_EULER, CH_EULER_MOBILITY, CH_PSEUDOSPECTRAL     */
+    int32_t integrator;    /* CH_EULER, CH_EULER_MOBILITY, CH_PSEUDOSPECTRAL,    
+                              CH_PSEUDOSPECTRAL_MOBILITY                         */
     int32_t model;         /* CH_LANDAU, CH_SIMPLE_WERTHEIM                      */
     int32_t mobility;      /* CH_MOB_*                                            */
     int32_t valence;       /* wertheim.valence                                    */
@@ -78,6 +79,15 @@ typedef struct ch_desc {
     int32_t reserved0;             /* 0                                           */
     double noise_rescale_factor;   /* mobility.noise_rescale_factor (default 1)   */
     uint64_t seed;                 /* seed                                        */
+    /* pseudospectral_mobility (PseudospectralMobilityCPU.cpp:203-208): 0 =
+     * mobility splitting, 1 = restarted GMRES; 0 in the other three fields
+     * selects the reference's defaults (30, 200, 1e-10).  FP32 systems solve
+     * to max(gmres_tol, 1e-6). */
+    int32_t use_gmres;             /* pseudospectral.use_gmres                    */
+    int32_t gmres_restart;         /* pseudospectral.gmres_restart                */
+    int32_t gmres_max_iter;        /* pseudospectral.gmres_max_iter               */
+    int32_t reserved1;             /* 0                                           */
+    double gmres_tol;              /* pseudospectral.gmres_tol                    */
 } ch_desc;
 
 typedef struct ch_ctx ch_ctx;
@@ -127,6 +137,10 @@ ch_status ch_sys_evolve(ch_sys *sys, int64_t nsteps);
  * measurement (the reference always does one step per launch,
  * EulerCUDA.cu:60-69).  Override with the CH_TB_STEPS environment variable. */
 int32_t ch_sys_steps_per_launch(const ch_sys *sys);
+/* GMRES iterations of the last pseudospectral_mobility step with use_gmres
+ * (0 otherwise); the reference computes the count and discards it
+ * (PseudospectralMobilityCPU.cpp:375-376). */
+int32_t ch_sys_gmres_iterations(const ch_sys *sys);
 
 /* FP64 reductions of the current field: average_mass and average_free_energy
  * (src/CahnHilliard.cpp:285-319, V_bin = dx^3 as :163).  Synchronous.

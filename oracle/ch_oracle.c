@@ -959,3 +959,307 @@ void och_euler_mobility_noise_step(const och_params *p, const och_model *m, doub
     }
     free(mob); free(mu); free(flux); free(sto);
 }
+
+/* ------------------------------------------------------------------ */
+/* pseudospectral_mobility, src/integrators/PseudospectralMobilityCPU.cpp */
+/* ------------------------------------------------------------------ */
+struct och_psm {
+    och_params p;
+    och_model m;
+    int use_gmres, restart, max_iter, last_iters;
+    double tol, M0;
+    int64_t bins, hat;
+    double *rho_hat;                   /* state, 2 * hat                          */
+    double *k2, *deal, *kc[3];         /* per half-spectrum entry; kc[d] in the   */
+                                       /* reference's axis order (slowest first)  */
+    double *fh, *muh, *gh, *divh, *th; /* spectra, 2 * hat each                   */
+    double *mob, *g, *tr;              /* real fields                             */
+};
+
+/* wave-vector tables, PseudospectralMobilityCPU.cpp:56-125: full axes use the
+ * signed index (i < N/2 ? i : i - N), the x (half) axis 0 .. N/2; the
+ * dealiaser keeps |k| <= k_cut in every dimension */
+och_psm *och_psm_create(const och_params *p, const och_model *m, const double *rho, int use_gmres, int restart,
+                        int max_iter, double tol) {
+    och_psm *s = (och_psm *)calloc(1, sizeof(och_psm));
+    s->p = *p; s->m = *m;
+    s->use_gmres = use_gmres;
+    s->restart = restart; s->max_iter = max_iter; s->tol = tol;
+    int N = p->N, dims = p->dims, Nh = N / 2 + 1;
+    s->bins = ipow(N, dims);
+    s->hat = ipow(N, dims - 1) * Nh;
+    size_t hb = sizeof(double) * 2 * s->hat, fb = sizeof(double) * s->bins;
+    s->rho_hat = (double *)malloc(hb);
+    s->fh = (double *)malloc(hb); s->muh = (double *)malloc(hb); s->gh = (double *)malloc(hb);
+    s->divh = (double *)malloc(hb); s->th = (double *)malloc(hb);
+    s->mob = (double *)malloc(fb); s->g = (double *)malloc(fb); s->tr = (double *)malloc(fb);
+    s->k2 = (double *)malloc(sizeof(double) * s->hat);
+    s->deal = (double *)malloc(sizeof(double) * s->hat);
+    for(int d = 0; d < dims; d++) s->kc[d] = (double *)malloc(sizeof(double) * s->hat);
+    const double k_cut = N * OCH_PI / (N * p->dx) * 2.0 / 3.0;
+    for(int64_t q = 0; q < s->hat; q++) {
+        int ix = (int)(q % Nh);
+        int64_t rest = q / Nh;
+        int iy = dims == 3 ? (int)(rest % N) : (int)rest, iz = dims == 3 ? (int)(rest / N) : 0;
+        int idx[3], nd = 0;  /* slowest axis first, the half axis last */
+        if(dims == 3) idx[nd++] = iz;
+        if(dims >= 2) idx[nd++] = iy;
+        double k2 = 0;
+        for(int d = 0; d < dims; d++) {
+            double kd;
+            if(d == dims - 1) kd = 2.0 * OCH_PI * ix / (N * p->dx);
+            else {
+                int ki = (idx[d] < N / 2) ? idx[d] : (idx[d] - N);
+                kd = 2.0 * OCH_PI * ki / (N * p->dx);
+            }
+            s->kc[d][q] = kd;
+            k2 += SQR(kd);
+        }
+        s->k2[q] = k2;
+        s->deal[q] = (sqrt(k2) <= k_cut) ? 1.0 : 0.0;
+    }
+    r2c(dims, N, rho, s->rho_hat);
+    return s;
+}
+
+void och_psm_destroy(och_psm *s) {
+    if(!s) return;
+    free(s->rho_hat); free(s->fh); free(s->muh); free(s->gh); free(s->divh); free(s->th);
+    free(s->mob); free(s->g); free(s->tr); free(s->k2); free(s->deal);
+    for(int d = 0; d < s->p.dims; d++) free(s->kc[d]);
+    free(s);
+}
+
+/* (i k) z */
+static inline void ik_mul(double k, const double *z, double *out) {
+    double re = -k * z[1], im = k * z[0];
+    out[0] = re; out[1] = im;
+}
+
+/* M0 = max(0, max_i M_i), PseudospectralMobilityCPU.cpp:406-414 */
+static double psm_M0(const och_psm *s) {
+    double M0 = 0.0;
+    for(int64_t i = 0; i < s->bins; i++) M0 = s->mob[i] > M0 ? s->mob[i] : M0;
+    return M0;
+}
+
+/* divh = sum_d (i k_d) deal r2c(w(x) c2r(deal (i k_d) src)/bins); `dM` is
+ * subtracted from w (the simple splitting uses M - M0, the full scheme M).
+ * :257-291 (simple) and :329-357, :495-526 (full) */
+static void psm_div_w_grad(och_psm *s, const double *src, const double *w, double dM) {
+    const och_params *p = &s->p;
+    int dims = p->dims, N = p->N;
+    for(int64_t q = 0; q < s->hat; q++) { s->divh[2 * q] = 0.0; s->divh[2 * q + 1] = 0.0; }
+    for(int d = 0; d < dims; d++) {
+        for(int64_t q = 0; q < s->hat; q++) {
+            ik_mul(s->kc[d][q], src + 2 * q, s->gh + 2 * q);
+            if(p->use_dealias) { s->gh[2 * q] *= s->deal[q]; s->gh[2 * q + 1] *= s->deal[q]; }
+        }
+        c2r(dims, N, s->gh, s->g);
+        for(int64_t i = 0; i < s->bins; i++) s->g[i] /= s->bins;
+        for(int64_t i = 0; i < s->bins; i++) s->g[i] = (w[i] - dM) * s->g[i];
+        r2c(dims, N, s->g, s->gh);
+        for(int64_t q = 0; q < s->hat; q++) {
+            double *f = s->gh + 2 * q, t[2];
+            if(p->use_dealias) { f[0] *= s->deal[q]; f[1] *= s->deal[q]; }
+            ik_mul(s->kc[d][q], f, t);
+            s->divh[2 * q] += t[0];
+            s->divh[2 * q + 1] += t[1];
+        }
+    }
+}
+
+void och_psm_div_mob_grad(const och_params *p, const double *mob, const double *g, double *out) {
+    och_model m = {OCH_LANDAU, 0.9, 0, 0, 0, 0};
+    och_psm *s = och_psm_create(p, &m, g, 0, 30, 200, 1e-10);
+    psm_div_w_grad(s, s->rho_hat, mob, 0.0);  /* rho_hat = r2c(g) */
+    c2r(p->dims, p->N, s->divh, out);
+    for(int64_t i = 0; i < s->bins; i++) out[i] /= s->bins;
+    och_psm_destroy(s);
+}
+
+/* _evolve_simple, :234-301 */
+static void psm_step_simple(och_psm *s, double *rho) {
+    const och_params *p = &s->p;
+    int dims = p->dims, N = p->N;
+    s->M0 = psm_M0(s);
+    for(int64_t i = 0; i < s->bins; i++) s->g[i] = och_der_bulk_free_energy(&s->m, rho[i]);
+    r2c(dims, N, s->g, s->fh);
+    for(int64_t q = 0; q < s->hat; q++) {
+        for(int c = 0; c < 2; c++) {
+            double v = s->fh[2 * q + c] + 2.0 * p->k * s->k2[q] * s->rho_hat[2 * q + c];
+            if(p->use_dealias) v *= s->deal[q];
+            s->muh[2 * q + c] = v;
+        }
+    }
+    psm_div_w_grad(s, s->muh, s->mob, s->M0);
+    for(int64_t q = 0; q < s->hat; q++) {
+        double k2 = s->k2[q], k4 = SQR(k2);
+        double denom = 1.0 + p->dt * s->M0 * (p->S * k2 + 2.0 * p->k * k4);
+        for(int c = 0; c < 2; c++) {
+            double *rh = s->rho_hat + 2 * q + c;
+            double v = (*rh + p->dt * s->divh[2 * q + c] -
+                        p->dt * s->M0 * k2 * (s->fh[2 * q + c] - p->S * *rh)) / denom;
+            *rh = v;
+            s->th[2 * q + c] = v / s->bins;
+        }
+    }
+    c2r(dims, N, s->th, rho);
+}
+
+/* A x = x - dt div(M grad((S - 2k lap) x)), :466-527 */
+static void psm_apply_A(och_psm *s, const double *x, double *Ax) {
+    const och_params *p = &s->p;
+    r2c(p->dims, p->N, x, s->th);
+    for(int64_t q = 0; q < s->hat; q++) {
+        for(int c = 0; c < 2; c++) {
+            double v = (p->S + 2.0 * p->k * s->k2[q]) * s->th[2 * q + c];
+            if(p->use_dealias) v *= s->deal[q];
+            s->muh[2 * q + c] = v;
+        }
+    }
+    psm_div_w_grad(s, s->muh, s->mob, 0.0);
+    c2r(p->dims, p->N, s->divh, s->tr);
+    for(int64_t i = 0; i < s->bins; i++) Ax[i] = x[i] - p->dt * (s->tr[i] / s->bins);
+}
+
+/* z = P^-1 r, P = 1 + dt M0 (S k^2 + 2k k^4) in Fourier space, :431-464 */
+static void psm_apply_Pinv(och_psm *s, const double *r, double *z) {
+    const och_params *p = &s->p;
+    r2c(p->dims, p->N, r, s->th);
+    for(int64_t q = 0; q < s->hat; q++) {
+        double k2 = s->k2[q], k4 = k2 * k2;
+        double denom = 1.0 + p->dt * s->M0 * (p->S * k2 + 2.0 * p->k * k4);
+        s->th[2 * q] /= denom;
+        s->th[2 * q + 1] /= denom;
+    }
+    c2r(p->dims, p->N, s->th, s->tr);
+    for(int64_t i = 0; i < s->bins; i++) z[i] = s->tr[i] / s->bins;
+}
+
+static double psm_dot(const double *a, const double *b, int64_t n) {
+    double sum = 0.0;
+    for(int64_t i = 0; i < n; i++) sum += a[i] * b[i];
+    return sum;
+}
+
+/* left-preconditioned restarted GMRES (modified Gram-Schmidt, Givens), :529-663 */
+static int psm_gmres(och_psm *s, const double *b, double *x) {
+    const int64_t n = s->bins;
+    const int m = s->restart > 2 ? s->restart : 2;
+    double *rhs = (double *)malloc(sizeof(double) * n), *Mx = (double *)malloc(sizeof(double) * n);
+    double *r = (double *)malloc(sizeof(double) * n), *w = (double *)malloc(sizeof(double) * n);
+    double *Av = (double *)malloc(sizeof(double) * n);
+    double *V = (double *)malloc(sizeof(double) * n * (m + 1));
+    double *H = (double *)malloc(sizeof(double) * (m + 1) * m);
+    double *cs = (double *)malloc(sizeof(double) * m), *sn = (double *)malloc(sizeof(double) * m);
+    double *g = (double *)malloc(sizeof(double) * (m + 1)), *y = (double *)malloc(sizeof(double) * m);
+#define HH(i, j) H[(i) * m + (j)]
+    psm_apply_Pinv(s, b, rhs);
+    psm_apply_A(s, x, Av);
+    psm_apply_Pinv(s, Av, Mx);
+    for(int64_t i = 0; i < n; i++) r[i] = rhs[i] - Mx[i];
+    double rn = sqrt(psm_dot(rhs, rhs, n));
+    const double rhs_norm = rn > 1e-30 ? rn : 1e-30;
+    double beta = sqrt(psm_dot(r, r, n));
+    double rel = beta / rhs_norm;
+    int iter_total = 0;
+    if(rel >= s->tol) {
+        while(iter_total < s->max_iter) {
+            for(int64_t i = 0; i < n * (m + 1); i++) V[i] = 0.0;
+            for(int i = 0; i < (m + 1) * m; i++) H[i] = 0.0;
+            for(int i = 0; i < m; i++) { cs[i] = 0.0; sn[i] = 0.0; }
+            for(int i = 0; i <= m; i++) g[i] = 0.0;
+            for(int64_t i = 0; i < n; i++) V[i] = r[i] / beta;
+            g[0] = beta;
+            int j_end = 0;
+            for(int j = 0; j < m && iter_total < s->max_iter; j++, iter_total++) {
+                psm_apply_A(s, V + (int64_t)j * n, Av);
+                psm_apply_Pinv(s, Av, w);
+                for(int i = 0; i <= j; i++) {
+                    HH(i, j) = psm_dot(w, V + (int64_t)i * n, n);
+                    for(int64_t k = 0; k < n; k++) w[k] -= HH(i, j) * V[(int64_t)i * n + k];
+                }
+                HH(j + 1, j) = sqrt(psm_dot(w, w, n));
+                if(HH(j + 1, j) > 1e-30) {
+                    for(int64_t k = 0; k < n; k++) V[(int64_t)(j + 1) * n + k] = w[k] / HH(j + 1, j);
+                }
+                else {
+                    j_end = j + 1;
+                    break;
+                }
+                for(int i = 0; i < j; i++) {
+                    double t1 = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
+                    double t2 = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
+                    HH(i, j) = t1;
+                    HH(i + 1, j) = t2;
+                }
+                double a = HH(j, j), b2 = HH(j + 1, j);
+                double den = sqrt(a * a + b2 * b2);
+                cs[j] = (den > 0) ? (a / den) : 1.0;
+                sn[j] = (den > 0) ? (b2 / den) : 0.0;
+                HH(j, j) = cs[j] * a + sn[j] * b2;
+                HH(j + 1, j) = 0.0;
+                double gj = g[j];
+                g[j] = cs[j] * gj;
+                g[j + 1] = -sn[j] * gj;
+                rel = fabs(g[j + 1]) / rhs_norm;
+                if(rel < s->tol) {
+                    j_end = j + 1;
+                    break;
+                }
+                j_end = j + 1;
+            }
+            for(int i = j_end - 1; i >= 0; i--) {
+                double sum = g[i];
+                for(int k = i + 1; k < j_end; k++) sum -= HH(i, k) * y[k];
+                y[i] = sum / HH(i, i);
+            }
+            for(int i = 0; i < j_end; i++)
+                for(int64_t k = 0; k < n; k++) x[k] += V[(int64_t)i * n + k] * y[i];
+            psm_apply_A(s, x, Av);
+            psm_apply_Pinv(s, Av, Mx);
+            for(int64_t i = 0; i < n; i++) r[i] = rhs[i] - Mx[i];
+            beta = sqrt(psm_dot(r, r, n));
+            rel = beta / rhs_norm;
+            if(rel < s->tol) break;
+        }
+    }
+#undef HH
+    free(rhs); free(Mx); free(r); free(w); free(Av); free(V); free(H); free(cs); free(sn); free(g); free(y);
+    return iter_total;
+}
+
+/* _evolve_full, :303-404 */
+static void psm_step_full(och_psm *s, double *rho) {
+    const och_params *p = &s->p;
+    int dims = p->dims, N = p->N;
+    const int64_t n = s->bins;
+    s->M0 = psm_M0(s);
+    double *b = (double *)malloc(sizeof(double) * n), *x = (double *)malloc(sizeof(double) * n);
+    for(int64_t i = 0; i < n; i++) s->tr[i] = och_der_bulk_free_energy(&s->m, rho[i]) - p->S * rho[i];
+    r2c(dims, N, s->tr, s->fh);
+    psm_div_w_grad(s, s->fh, s->mob, 0.0);
+    c2r(dims, N, s->divh, s->tr);
+    for(int64_t i = 0; i < n; i++) {
+        b[i] = rho[i] + p->dt * (s->tr[i] / n);
+        x[i] = rho[i];
+    }
+    s->last_iters = psm_gmres(s, b, x);
+    memcpy(rho, x, sizeof(double) * n);
+    r2c(dims, N, rho, s->rho_hat);
+    free(b); free(x);
+}
+
+/* one step: IMobility::update_mobility() on rho (CahnHilliard.cpp:279-282), then evolve() */
+int och_psm_step(och_psm *s, double *rho) {
+    och_update_mobility(&s->p, &s->m, rho, s->mob, s->bins);
+    if(s->use_gmres) psm_step_full(s, rho);
+    else {
+        s->last_iters = 0;
+        psm_step_simple(s, rho);
+    }
+    return s->last_iters;
+}
+
+double och_psm_last_M0(const och_psm *s) { return s->M0; }

@@ -127,6 +127,18 @@ void och_cell_normals(uint64_t seed, int64_t cell, int64_t step, double nrm[4]);
  * system was created */
 void och_euler_mobility_noise_step(const och_params *p, const och_model *m, double noise_rescale, uint64_t seed,
                                    int64_t step, double *rho);
+/* ---- pseudospectral_mobility (src/integrators/PseudospectralMobilityCPU.cpp) ----
+ * use_gmres = 0: _evolve_simple (mobility splitting, M0 = max M); 1:
+ * _evolve_full (restarted GMRES with the FFT-diagonal preconditioner).  The
+ * state carries rho_hat; och_psm_step returns the GMRES iterations. */
+typedef struct och_psm och_psm;
+och_psm *och_psm_create(const och_params *p, const och_model *m, const double *rho, int use_gmres, int restart,
+                        int max_iter, double tol);
+int och_psm_step(och_psm *s, double *rho);
+double och_psm_last_M0(const och_psm *s);
+void och_psm_destroy(och_psm *s);
+/* the spectral div(mob grad g) of the integrator (c2r semantics included) */
+void och_psm_div_mob_grad(const och_params *p, const double *mob, const double *g, double *out);
 /* F = average_free_energy(rho) with dF/drho (may be NULL) and the explicit
  * dF/dparams[OCH_NGRADS] (may be NULL) */
 double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,

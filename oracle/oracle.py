@@ -105,6 +105,11 @@ def lib(fast=False):
         "och_free_energy_grad": (D, [PP, MP, P, P, P]),
         "och_ps_adjoint_step": (None, [PP, MP, P, P, P]),
         "och_ps_adjoint": (None, [PP, MP, P, I64, P, P, P]),
+        "och_psm_create": (ctypes.c_void_p, [PP, MP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, D]),
+        "och_psm_step": (ctypes.c_int, [ctypes.c_void_p, P]),
+        "och_psm_last_M0": (D, [ctypes.c_void_p]),
+        "och_psm_destroy": (None, [ctypes.c_void_p]),
+        "och_psm_div_mob_grad": (None, [PP, P, P, P]),
         "och_philox4x32_10": (None, [P, P, P]),
         "och_cell_normals": (None, [ctypes.c_uint64, I64, I64, P]),
         "och_euler_mobility_noise_step": (None, [PP, MP, D, ctypes.c_uint64, I64, P]),
@@ -339,3 +344,27 @@ def run_noise(params, model, rho, steps, noise_rescale=1.0, seed=123456, step0=0
         _L().och_euler_mobility_noise_step(ctypes.byref(params.c()), ctypes.byref(model.c()), noise_rescale, seed,
                                            step0 + t, _ptr(rho))
     return rho
+
+
+def run_psm(params, model, rho, steps, use_gmres=False, restart=30, max_iter=200, tol=1e-10, iters=None):
+    """`steps` pseudospectral_mobility steps; GMRES iteration counts appended to `iters`."""
+    rho = np.array(rho, dtype=np.float64, copy=True)
+    L = _L()
+    h = L.och_psm_create(ctypes.byref(params.c()), ctypes.byref(model.c()), _ptr(rho), int(use_gmres), restart,
+                         max_iter, tol)
+    try:
+        for _ in range(steps):
+            it = L.och_psm_step(h, _ptr(rho))
+            if iters is not None:
+                iters.append(it)
+    finally:
+        L.och_psm_destroy(h)
+    return rho
+
+
+def psm_div_mob_grad(params, mob, g):
+    mob = np.ascontiguousarray(mob, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.zeros(params.bins)
+    _L().och_psm_div_mob_grad(ctypes.byref(params.c()), _ptr(mob), _ptr(g), _ptr(out))
+    return out

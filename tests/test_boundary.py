@@ -28,7 +28,7 @@ def test_every_header_in_include_is_exported(ch):
 
 def test_abi_version_and_status_strings_without_gpu(ch):
     L = ch._lib.lib()
-    assert L.ch_abi_version() == 2
+    assert L.ch_abi_version() == 3
     assert L.ch_status_string(3) == b"NaN encountered"
 
 
@@ -82,3 +82,14 @@ def test_desc_layout_matches_header(ch, tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     want = [ctypes.sizeof(ch._lib.ChDesc)] + [getattr(ch._lib.ChDesc, f).offset for f in fields]
     assert got == want
+
+
+def test_pseudospectral_mobility_keys_from_reference_config(ch):
+    # PseudospectralMobilityCPU.cpp:22-23 (S read without the m^3 rescaling), :203-208 (GMRES keys)
+    cfg = {"N": 64, "dt": 0.01, "k": 1.0, "free_energy": "landau", "landau": {"epsilon": 0.9},
+           "integrator": "pseudospectral_mobility", "distance_scaling_factor": 1.0,
+           "mobility": {"type": "regularised", "rho_min": 0.2},
+           "pseudospectral": {"S": 0.5, "use_dealias": True, "use_gmres": True, "gmres_tol": 1e-8}}
+    d = ch.desc_from_config(cfg, dims=2)
+    assert d.integrator == ch.PSEUDOSPECTRAL_MOBILITY and d.S == 0.5 and d.use_dealias == 1
+    assert (d.use_gmres, d.gmres_restart, d.gmres_max_iter, d.gmres_tol) == (1, 30, 200, 1e-8)
