@@ -8,8 +8,11 @@ lives in host/.
 from . import _lib
 from ._lib import (CHError, EULER, EULER_MOBILITY, FP32, FP64, GRAD_B2, GRAD_DELTA, GRAD_EPSILON, GRAD_K, GRAD_M,
                    LANDAU, NGRADS, PSEUDOSPECTRAL, PSEUDOSPECTRAL_MOBILITY, SIMPLE_WERTHEIM, header_functions)
-from .system import Batch, Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility
+from ._lib import ChSpeciesModel, GENERIC_WERTHEIM, RICCI_WERTHEIM, SALEH_WERTHEIM
+from .system import (Batch, Context, System, der_bulk_free_energy, desc_from_config, make_desc, set_mobility,
+                     species_model_from_config)
 
-__all__ = ["CHError", "Batch", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
+__all__ = ["CHError", "ChSpeciesModel", "species_model_from_config", "SALEH_WERTHEIM", "RICCI_WERTHEIM",
+           "GENERIC_WERTHEIM", "Batch", "Context", "System", "make_desc", "desc_from_config", "der_bulk_free_energy",
            "set_mobility", "header_functions", "EULER", "EULER_MOBILITY", "PSEUDOSPECTRAL", "PSEUDOSPECTRAL_MOBILITY", "LANDAU",
            "SIMPLE_WERTHEIM", "FP32", "FP64", "GRAD_EPSILON", "GRAD_K", "GRAD_M", "GRAD_DELTA", "GRAD_B2", "NGRADS"]

@@ -19,6 +19,8 @@ MOB_CONSTANT, MOB_FREE_ENERGY, MOB_REGULARISED, MOB_LINEAR = 0, 1, 2, 3
 # parameter-gradient slots of ch_sys_adjoint / ch_sys_free_energy_grad
 GRAD_EPSILON, GRAD_K, GRAD_M, GRAD_DELTA, GRAD_B2, NGRADS = 0, 1, 2, 3, 4, 5
 
+SALEH_WERTHEIM, RICCI_WERTHEIM, GENERIC_WERTHEIM = 2, 3, 4
+MODELS_SPECIES = {"saleh": SALEH_WERTHEIM, "ricci": RICCI_WERTHEIM, "generic_wertheim": GENERIC_WERTHEIM}
 INTEGRATORS = {"euler": EULER, "euler_mobility": EULER_MOBILITY, "pseudospectral": PSEUDOSPECTRAL,
                "pseudospectral_mobility": PSEUDOSPECTRAL_MOBILITY}
 MODELS = {"landau": LANDAU, "simple_wertheim": SIMPLE_WERTHEIM}
@@ -36,7 +38,23 @@ class ChDesc(ctypes.Structure):
                 ("k", ctypes.c_double), ("S", ctypes.c_double), ("with_noise", ctypes.c_int32),
                 ("reserved0", ctypes.c_int32), ("noise_rescale_factor", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("use_gmres", ctypes.c_int32), ("gmres_restart", ctypes.c_int32), ("gmres_max_iter", ctypes.c_int32),
-                ("reserved1", ctypes.c_int32), ("gmres_tol", ctypes.c_double)]
+                ("reserved1", ctypes.c_int32), ("gmres_tol", ctypes.c_double),
+                ("density_conversion", ctypes.c_double)]
+
+
+MAX_SPECIES, MAX_PATCHES = 8, 16
+
+
+class ChSpeciesModel(ctypes.Structure):
+    """ch_species_model: a multi-species Wertheim model (include/ch_hip.h)."""
+    _fields_ = [("model", ctypes.c_int32), ("n_species", ctypes.c_int32), ("k", ctypes.c_double * MAX_SPECIES),
+                ("saleh_valence", ctypes.c_int32 * 3), ("reserved0", ctypes.c_int32),
+                ("saleh_B2", ctypes.c_double), ("saleh_B3", ctypes.c_double), ("saleh_delta_AA", ctypes.c_double),
+                ("saleh_delta_BB", ctypes.c_double), ("ricci_B2", ctypes.c_double), ("ricci_delta_00", ctypes.c_double),
+                ("ricci_delta_12", ctypes.c_double), ("n_patches", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+                ("patch_count", (ctypes.c_int32 * MAX_PATCHES) * MAX_SPECIES),
+                ("delta", (ctypes.c_double * MAX_PATCHES) * MAX_PATCHES),
+                ("B2", (ctypes.c_double * MAX_SPECIES) * MAX_SPECIES)]
 
 
 class CHError(RuntimeError):
@@ -84,6 +102,8 @@ def lib():
         "ch_ctx_synchronize": (ctypes.c_int, [P]),
         "ch_ctx_stream": (P, [P]),
         "ch_sys_create": (ctypes.c_int, [P, DP, PP]),
+        "ch_sys_create_species": (ctypes.c_int, [P, DP, P, PP]),
+        "ch_sys_species": (ctypes.c_int32, [P]),
         "ch_sys_destroy": (None, [P]),
         "ch_sys_upload": (ctypes.c_int, [P, P]),
         "ch_sys_download": (ctypes.c_int, [P, P]),

@@ -62,6 +62,12 @@ struct ch_sys {
     double *psm_dev = nullptr;   // kPsmParts partials, 4 scalars, Hcol, y
     double *psm_host = nullptr;  // pinned, kPsmParts doubles
     int gmres_iters = 0;
+    // multi-species models: host and device model, X (Generic), mu, mobility,
+    // per-species spectra and two scratch spectra
+    int S = 1;
+    chd::SpModel *spm_host = nullptr, *spm_dev = nullptr;
+    double *sp_X = nullptr;
+    void *sp_mu = nullptr, *sp_mob = nullptr, *sp_rho_hat = nullptr, *sp_spec[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
@@ -143,7 +149,8 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     if(!is_pow2(d->N)) return fail(ctx, CH_EINVAL, "N should be a power of 2 (got %d)", d->N);
     if(d->N < 4) return fail(ctx, CH_EUNSUPPORTED, "N must be at least 4 (got %d)", d->N);
     if(d->precision != CH_FP64 && d->precision != CH_FP32) return fail(ctx, CH_EINVAL, "bad precision %d", d->precision);
-    if(d->model != CH_LANDAU && d->model != CH_SIMPLE_WERTHEIM) return fail(ctx, CH_EINVAL, "bad model %d", d->model);
+    if(d->model < CH_LANDAU || d->model > CH_GENERIC_WERTHEIM) return fail(ctx, CH_EINVAL, "bad model %d", d->model);
+    if(!(d->density_conversion >= 0.0)) return fail(ctx, CH_EINVAL, "density_conversion must not be negative");
     if(d->integrator < CH_EULER || d->integrator > CH_PSEUDOSPECTRAL_MOBILITY)
         return fail(ctx, CH_EINVAL, "bad integrator %d", d->integrator);
     if(d->mobility < CH_MOB_CONSTANT || d->mobility > CH_MOB_LINEAR)
@@ -152,7 +159,7 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     if(d->integrator != CH_EULER_MOBILITY && d->integrator != CH_PSEUDOSPECTRAL_MOBILITY && d->mobility != CH_MOB_CONSTANT)
         return fail(ctx, CH_EINVAL, "The selected integrator only supports constant mobility");
     // FreeEnergyModel::set_mobility default (src/models/FreeEnergyModel.h:51-53)
-    if(d->mobility == CH_MOB_FREE_ENERGY && d->model != CH_SIMPLE_WERTHEIM)
+    if(d->mobility == CH_MOB_FREE_ENERGY && d->model != CH_SIMPLE_WERTHEIM && d->model != CH_GENERIC_WERTHEIM)
         return fail(ctx, CH_EUNSUPPORTED, "this model does not support mobility coupling");
     if(d->mobility == CH_MOB_LINEAR && d->rho_ref == 0.0) return fail(ctx, CH_EINVAL, "linear mobility needs rho_ref != 0");
     if(d->model == CH_SIMPLE_WERTHEIM && (d->valence <= 0 || !(d->delta > 0.0)))
@@ -204,6 +211,7 @@ static chd::Params make_params(const ch_desc *d) {
         const double q = 2.0 * 3.14159265358979323846 / (d->N * d->dx);
         p.q2_unit = q * q;
         p.k_unit = q;
+    p.dcf = d->density_conversion > 0.0 ? d->density_conversion : 1.0;
     }
     p.wave_cfg = chd::wave_default_cfg(d->N);
     p.adj_cfg = chd::wave_default_cfg(d->N);
@@ -248,6 +256,24 @@ static void twiddle(int m, int N, long double &c, long double &s) {
     s = -sign * ss;  // forward sign
 }
 
+// W_N^m, m = 0 .. N-1, in the system's precision
+static bool upload_twiddles(ch_sys *s) {
+    const int N = s->d.N;
+    if(hipMalloc(&s->twiddles, (size_t)N * 2 * s->elt) != hipSuccess) return false;
+    std::vector<double> twd(2 * N);
+    std::vector<float> twf(2 * N);
+    for(int m = 0; m < N; m++) {
+        long double c, sn;
+        twiddle(m, N, c, sn);
+        twd[2 * m] = (double)c;
+        twd[2 * m + 1] = (double)sn;
+        twf[2 * m] = (float)twd[2 * m];
+        twf[2 * m + 1] = (float)twd[2 * m + 1];
+    }
+    const void *src = s->d.precision == CH_FP64 ? (const void *)twd.data() : (const void *)twf.data();
+    return hipMemcpy(s->twiddles, src, (size_t)N * 2 * s->elt, hipMemcpyHostToDevice) == hipSuccess;
+}
+
 static void free_sys(ch_sys *s) {
     if(s->aux) {
         (void)hipStreamSynchronize(s->aux);
@@ -271,6 +297,9 @@ static void free_sys(ch_sys *s) {
     for(void *b : s->psm_vec)
         if(b) (void)hipFree(b);
     if(s->psm_dev) (void)hipFree(s->psm_dev);
+    for(void *b : {(void *)s->spm_dev, (void *)s->sp_X, s->sp_mu, s->sp_mob, s->sp_rho_hat, s->sp_spec[0], s->sp_spec[1]})
+        if(b) (void)hipFree(b);
+    delete s->spm_host;
     if(s->psm_host) (void)hipHostFree(s->psm_host);
     delete s;
 }
@@ -280,6 +309,8 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     *out = nullptr;
     ch_status st = validate(ctx, desc);
     if(st != CH_OK) return st;
+    if(desc->model > CH_SIMPLE_WERTHEIM)
+        return fail(ctx, CH_EINVAL, "multi-species model %d: use ch_sys_create_species", desc->model);
     HIPCHK(ctx, hipSetDevice(ctx->device));
     ch_sys *s = new ch_sys();
     s->ctx = ctx;
@@ -328,22 +359,8 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
         ok = ok && alloc((void **)&s->psm_dev, sizeof(double) * (chd::kPsmParts + 4 + 2 * (m + 2))) &&
              hipHostMalloc((void **)&s->psm_host, sizeof(double) * std::max(chd::kPsmParts, m + 2)) == hipSuccess;
     }
-    if(ok && is_spectral(desc->integrator)) {
-        ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->rho_hat, s->hat * 2 * s->elt) &&
-             alloc(&s->twiddles, (size_t)N * 2 * s->elt);
-        if(ok) {
-            std::vector<double> twd(2 * N);
-            std::vector<float> twf(2 * N);
-            for(int m = 0; m < N; m++) {
-                long double c, sn;
-                twiddle(m, N, c, sn);
-                twd[2 * m] = (double)c; twd[2 * m + 1] = (double)sn;
-                twf[2 * m] = (float)twd[2 * m]; twf[2 * m + 1] = (float)twd[2 * m + 1];
-            }
-            const void *src = desc->precision == CH_FP64 ? (const void *)twd.data() : (const void *)twf.data();
-            ok = hipMemcpy(s->twiddles, src, (size_t)N * 2 * s->elt, hipMemcpyHostToDevice) == hipSuccess;
-        }
-    }
+    if(ok && is_spectral(desc->integrator))
+        ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->rho_hat, s->hat * 2 * s->elt) && upload_twiddles(s);
     if(!ok) {
         free_sys(s);
         return fail(ctx, CH_ENOMEM, "device allocation failed for N=%d dims=%d", desc->N, desc->dims);
@@ -357,6 +374,151 @@ extern "C" void ch_sys_destroy(ch_sys *s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     free_sys(s);
+}
+
+// ---------------------------------------------------------------- species
+static chd::SpectralArgs spectral_args(ch_sys *s);
+static bool is_species(const ch_sys *s) { return s->spm_host != nullptr; }
+
+static chd::SpArgs sp_args(ch_sys *s) {
+    chd::SpArgs a{};
+    a.p = s->p;
+    a.precision = s->d.precision;
+    a.mob_rule = s->d.mobility;
+    a.m = s->spm_dev;
+    a.m_host = s->spm_host;
+    a.rho = s->psi[0];
+    a.X = s->sp_X;
+    a.mu = s->sp_mu;
+    a.mob = s->sp_mob;
+    a.rho_hat = s->sp_rho_hat;
+    a.spec[0] = s->sp_spec[0];
+    a.spec[1] = s->sp_spec[1];
+    a.bins = s->bins;
+    a.hat = s->hat;
+    return a;
+}
+
+static ch_status validate_species(ch_ctx *ctx, const ch_desc *d, const ch_species_model *m) {
+    if(m->model < CH_SALEH_WERTHEIM || m->model > CH_GENERIC_WERTHEIM)
+        return fail(ctx, CH_EINVAL, "bad multi-species model %d", m->model);
+    if(d->model != m->model) return fail(ctx, CH_EINVAL, "desc->model (%d) differs from model->model (%d)", d->model, m->model);
+    const int want = m->model == CH_SALEH_WERTHEIM ? 3 : (m->model == CH_RICCI_WERTHEIM ? 2 : 0);
+    if(m->n_species < 1 || m->n_species > CH_MAX_SPECIES || (want && m->n_species != want))
+        return fail(ctx, CH_EINVAL, "bad number of species %d for model %d", m->n_species, m->model);
+    if(d->integrator == CH_PSEUDOSPECTRAL_MOBILITY)
+        return fail(ctx, CH_EUNSUPPORTED, "pseudospectral_mobility supports single-species models only");
+    if(d->integrator == CH_EULER_MOBILITY && d->with_noise)
+        return fail(ctx, CH_EUNSUPPORTED, "noise supports single-species models only");
+    if(m->model == CH_SALEH_WERTHEIM && (m->saleh_valence[0] < 0 || m->saleh_valence[1] < 0 || m->saleh_valence[2] < 0))
+        return fail(ctx, CH_EINVAL, "saleh.valence must not be negative");
+    if(m->model == CH_GENERIC_WERTHEIM) {
+        if(m->n_patches < 1 || m->n_patches > CH_MAX_PATCHES)
+            return fail(ctx, CH_EINVAL, "generic_wertheim needs 1 .. %d patch types (got %d)", CH_MAX_PATCHES, m->n_patches);
+        for(int a = 0; a < m->n_species; a++)
+            for(int q = 0; q < CH_MAX_PATCHES; q++)
+                if(m->patch_count[a][q] < 0 || (q >= m->n_patches && m->patch_count[a][q] != 0))
+                    return fail(ctx, CH_EINVAL, "bad patch count for species %d, patch %d", a, q);
+        for(int a = 0; a < m->n_patches; a++)
+            for(int b = 0; b < m->n_patches; b++)
+                if(!(m->delta[a][b] >= 0.0) || m->delta[a][b] != m->delta[b][a])
+                    return fail(ctx, CH_EINVAL, "generic_wertheim deltas must be symmetric and not negative");
+    }
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_create_species(ch_ctx *ctx, const ch_desc *desc, const ch_species_model *model,
+                                           ch_sys **out) {
+    if(!ctx || !out || !model) return CH_EINVAL;
+    *out = nullptr;
+    ch_status st = validate(ctx, desc);
+    if(st == CH_OK) st = validate_species(ctx, desc, model);
+    if(st != CH_OK) return st;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ch_sys *s = new ch_sys();
+    s->ctx = ctx;
+    s->d = *desc;
+    s->p = make_params(desc);
+    s->bins = 1;
+    for(int i = 0; i < desc->dims; i++) s->bins *= desc->N;
+    s->elt = desc->precision == CH_FP64 ? 8 : 4;
+    s->hat = s->bins / desc->N * (desc->N / 2 + 1);
+    s->tb = 1;
+    const char *g = std::getenv("CH_GRAPH_STEPS");
+    s->graph_steps = std::max(0, g ? std::atoi(g) : 64);
+    s->spm_host = new chd::SpModel;
+    chd::sp_build_model(*model, *s->spm_host);
+    s->S = model->n_species;
+    const size_t SB = (size_t)s->S * s->bins;
+    const int P = std::max(1, s->spm_host->P);
+    auto alloc = [&](void **p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
+    bool ok = alloc(&s->psi[0], SB * s->elt) && alloc(&s->stage, SB * 8) &&
+              alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
+              alloc((void **)&s->result, 2 * sizeof(double)) && alloc((void **)&s->spm_dev, sizeof(chd::SpModel)) &&
+              alloc((void **)&s->sp_X, (size_t)P * s->bins * sizeof(double)) && alloc(&s->sp_mu, SB * s->elt) &&
+              alloc(&s->sp_mob, SB * s->elt);
+    ok = ok && hipMemcpy(s->spm_dev, s->spm_host, sizeof(chd::SpModel), hipMemcpyHostToDevice) == hipSuccess;
+    if(ok && desc->integrator == CH_PSEUDOSPECTRAL) {
+        ok = alloc(&s->work, s->hat * 2 * s->elt) && alloc(&s->sp_rho_hat, (size_t)s->S * s->hat * 2 * s->elt) &&
+             alloc(&s->sp_spec[0], s->hat * 2 * s->elt) && alloc(&s->sp_spec[1], s->hat * 2 * s->elt) &&
+             upload_twiddles(s);
+    }
+    if(!ok) {
+        free_sys(s);
+        return fail(ctx, CH_ENOMEM, "device allocation failed for N=%d dims=%d species=%d", desc->N, desc->dims,
+                    model->n_species);
+    }
+    *out = s;
+    return CH_OK;
+}
+
+extern "C" int32_t ch_sys_species(const ch_sys *s) { return s ? s->S : 0; }
+
+// X = 1 (GenericWertheim.cpp:252-254), the set-up evaluation of the model
+// (CahnHilliard.cpp:226-228), rho_hat of every species for the spectral path
+static ch_status sp_after_upload(ch_sys *s) {
+    ch_ctx *c = s->ctx;
+    const int P = s->spm_host->P;
+    if(P > 0) {
+        std::vector<double> ones((size_t)P * s->bins, 1.0);
+        HIPCHK(c, hipMemcpyAsync(s->sp_X, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    const chd::SpArgs a = sp_args(s);
+    HIPCHK(c, chd::sp_warm(a, c->stream));
+    if(s->d.integrator == CH_PSEUDOSPECTRAL) HIPCHK(c, chd::sp_init_spectral(a, spectral_args(s), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CH_OK;
+}
+
+static ch_status sp_evolve(ch_sys *s, int64_t nsteps) {
+    ch_ctx *c = s->ctx;
+    const chd::SpArgs a = sp_args(s);
+    const chd::SpectralArgs sa = spectral_args(s);
+    auto step = [&]() -> hipError_t {
+        if(s->d.integrator == CH_PSEUDOSPECTRAL) return chd::sp_step_spectral(a, sa, c->stream);
+        return chd::sp_step_stencil(a, s->d.integrator, c->stream);
+    };
+    const int G = s->graph_steps;
+    if(G > 0 && nsteps >= G) {
+        if(!s->gexec[0]) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for(int i = 0; i < G && e == hipSuccess; i++) e = step();
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+            if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+            e = hipGraphInstantiate(&s->gexec[0], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        const int64_t reps = nsteps / G;
+        for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[0], c->stream));
+        nsteps -= reps * G;
+    }
+    for(int64_t i = 0; i < nsteps; i++) HIPCHK(c, step());
+    return CH_OK;
 }
 
 static chd::SpectralArgs spectral_args(ch_sys *s) {
@@ -387,34 +549,36 @@ extern "C" ch_status ch_sys_upload(ch_sys *s, const double *rho_host) {
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t n = s->S * s->bins;
     if(s->d.precision == CH_FP64) {
-        HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_host, s->bins * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_host, n * 8, hipMemcpyHostToDevice, c->stream));
     }
     else {
-        HIPCHK(c, hipMemcpyAsync(s->stage, rho_host, s->bins * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, chd::convert_d2f((const double *)s->stage, (float *)s->psi[0], s->bins, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s->stage, rho_host, n * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, chd::convert_d2f((const double *)s->stage, (float *)s->psi[0], n, c->stream));
     }
-    return after_upload(s);
+    return is_species(s) ? sp_after_upload(s) : after_upload(s);
 }
 
 extern "C" ch_status ch_sys_upload_device(ch_sys *s, const void *rho_dev) {
     if(!s || !rho_dev) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_dev, s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
-    return after_upload(s);
+    HIPCHK(c, hipMemcpyAsync(s->psi[0], rho_dev, s->S * s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
+    return is_species(s) ? sp_after_upload(s) : after_upload(s);
 }
 
 extern "C" ch_status ch_sys_download(ch_sys *s, double *rho_host) {
     if(!s || !rho_host) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = s->S * s->bins;
     if(s->d.precision == CH_FP64) {
-        HIPCHK(c, hipMemcpyAsync(rho_host, s->psi[s->cur], s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rho_host, s->psi[s->cur], n * 8, hipMemcpyDeviceToHost, c->stream));
     }
     else {
-        HIPCHK(c, chd::convert_f2d((const float *)s->psi[s->cur], (double *)s->stage, s->bins, c->stream));
-        HIPCHK(c, hipMemcpyAsync(rho_host, s->stage, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, chd::convert_f2d((const float *)s->psi[s->cur], (double *)s->stage, n, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rho_host, s->stage, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CH_OK;
@@ -424,7 +588,7 @@ extern "C" ch_status ch_sys_download_device(ch_sys *s, void *rho_dev) {
     if(!s || !rho_dev) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpyAsync(rho_dev, s->psi[s->cur], s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rho_dev, s->psi[s->cur], s->S * s->bins * s->elt, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CH_OK;
 }
@@ -433,6 +597,12 @@ extern "C" ch_status ch_sys_download_mobility(ch_sys *s, double *mob_host) {
     if(!s || !mob_host) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if(is_species(s)) {
+        HIPCHK(c, chd::sp_mobility_field(sp_args(s), (double *)s->stage, c->stream));
+        HIPCHK(c, hipMemcpyAsync(mob_host, s->stage, s->S * s->bins * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return CH_OK;
+    }
     HIPCHK(c, chd::pointwise_mobility(s->p, s->d.precision, s->d.mobility, s->psi[s->cur], s->stage2, s->bins, c->stream));
     if(s->d.precision == CH_FP64) {
         HIPCHK(c, hipMemcpyAsync(mob_host, s->stage2, s->bins * 8, hipMemcpyDeviceToHost, c->stream));
@@ -604,6 +774,7 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
     if(!s || nsteps < 0) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if(is_species(s)) return sp_evolve(s, nsteps);
     if(s->d.integrator == CH_PSEUDOSPECTRAL) return evolve_spectral(s, nsteps);
     if(s->d.integrator == CH_PSEUDOSPECTRAL_MOBILITY) return evolve_psm(s, nsteps);
     if(has_noise(s)) return evolve_noise(s, nsteps);
@@ -643,8 +814,11 @@ extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_en
     if(!s) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts, s->result,
-                               c->stream));
+    if(is_species(s))
+        HIPCHK(c, chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, s->result, c->stream));
+    else
+        HIPCHK(c, chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts,
+                                   s->result, c->stream));
     double h[2];
     HIPCHK(c, hipMemcpyAsync(h, s->result, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -699,6 +873,7 @@ static ch_status sq_compute(ch_sys *s, int ns, double *sq_dev, double *cnt_dev) 
 extern "C" ch_status ch_sys_structure_factor(ch_sys *s, double *Sq, int32_t nshells) {
     if(!s || !Sq) return CH_EINVAL;
     ch_ctx *c = s->ctx;
+    if(is_species(s)) return fail(c, CH_EUNSUPPORTED, "S(q) covers single-species systems only");
     if(s->d.dims != 2) return fail(c, CH_EUNSUPPORTED, "the structure factor is defined for 2D fields");
     const int ns = ch_sq_nshells(s->d.N);
     if(nshells != ns) return fail(c, CH_EINVAL, "N = %d has %d shells (got %d)", s->d.N, ns, nshells);
@@ -720,6 +895,7 @@ extern "C" ch_status ch_sys_structure_factor(ch_sys *s, double *Sq, int32_t nshe
 extern "C" ch_status ch_sys_sq_loss(ch_sys *s, const double *target, int32_t nshells, double *loss, double *grad) {
     if(!s || !target) return CH_EINVAL;
     ch_ctx *c = s->ctx;
+    if(is_species(s)) return fail(c, CH_EUNSUPPORTED, "S(q) covers single-species systems only");
     if(s->d.dims != 2) return fail(c, CH_EUNSUPPORTED, "the structure factor is defined for 2D fields");
     const int ns = ch_sq_nshells(s->d.N);
     if(nshells != ns) return fail(c, CH_EINVAL, "N = %d has %d shells (got %d)", s->d.N, ns, nshells);
@@ -758,6 +934,7 @@ extern "C" ch_status ch_sys_sq_loss(ch_sys *s, const double *target, int32_t nsh
 extern "C" ch_status ch_sys_free_energy_grad(ch_sys *s, double *F, double *dF_dpsi, double *dF_dparams) {
     if(!s) return CH_EINVAL;
     ch_ctx *c = s->ctx;
+    if(is_species(s)) return fail(c, CH_EUNSUPPORTED, "the free-energy gradient covers single-species systems only");
     HIPCHK(c, hipSetDevice(c->device));
     constexpr int NS = chd::kFeSlots;
     const int max_parts = chd::kObsParts * 2 / NS;
@@ -1101,6 +1278,7 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     if(has_noise(s)) return fail(c, CH_EUNSUPPORTED, "reverse mode does not cover euler_mobility with noise");
     if(s->d.integrator == CH_PSEUDOSPECTRAL_MOBILITY)
         return fail(c, CH_EUNSUPPORTED, "reverse mode does not cover pseudospectral_mobility");
+    if(is_species(s)) return fail(c, CH_EUNSUPPORTED, "reverse mode covers single-species models only");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t fb = s->bins * s->elt;
     const bool ps = s->d.integrator == CH_PSEUDOSPECTRAL;

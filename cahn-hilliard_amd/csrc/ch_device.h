@@ -28,6 +28,7 @@ struct Params {
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
     double k_unit;                  // 2 pi / (N dx): wave number of a unit index
+    double dcf;                     // density conversion: ln(rho * dcf) in the Wertheim f
     double noise;                   // sqrt(2 / (dx^dims dt)) noise_rescale_factor (0: no noise)
     uint64_t seed;                  // key of the noise generator
 };
@@ -91,7 +92,7 @@ __device__ __forceinline__ double bulk(const Params &p, double r) {
             f_ref += a * r * r + b * r + c;
         }
         else {
-            f_ref += r * (log(r) - 1.0);
+            f_ref += r * (log(r * p.dcf) - 1.0);
         }
         double f_bond = 0.0;
         if(r > 0.) {

@@ -75,6 +75,43 @@ hipError_t psm_step_simple(const PsmArgs &B, hipStream_t s);
 // one GMRES step, synchronous; iters = GMRES iterations
 hipError_t psm_step_full(const PsmArgs &B, hipStream_t s, int &iters);
 
+// multi-species models (species.hip)
+constexpr int kSpMaxSpecies = CH_MAX_SPECIES, kSpMaxPatches = CH_MAX_PATCHES;
+constexpr int kSpMaxTerms = kSpMaxPatches * kSpMaxSpecies * kSpMaxPatches;
+struct SpModel {
+    int model, S, P;               // CH_*_WERTHEIM, species, patch types (Generic)
+    double k[kSpMaxSpecies];
+    int val[3], lh;                // Saleh valences, linker half valence
+    double sB2, sB3, dAA, dBB;     // Saleh
+    double rB2, d00, d12;          // Ricci
+    // Generic: unique patch types ascending, their mass-action terms
+    int n_unique, unique[kSpMaxPatches], term_start[kSpMaxPatches + 1];
+    int t_species[kSpMaxTerms], t_other[kSpMaxTerms];
+    double t_coeff[kSpMaxTerms];
+    int pc[kSpMaxSpecies][kSpMaxPatches];  // patch multiplicities per species
+    double B2[kSpMaxSpecies][kSpMaxSpecies];
+};
+void sp_build_model(const ch_species_model &in, SpModel &m);
+struct SpArgs {
+    Params p;
+    int precision, mob_rule;
+    const SpModel *m;       // device copy
+    const SpModel *m_host;  // host copy
+    void *rho;              // S * bins (T), species-major
+    double *X;              // P * bins, patch-major (Generic)
+    void *mu, *mob;         // S * bins (T) each
+    void *rho_hat;          // S * hat (T2), pseudospectral state
+    void *spec[2];          // hat (T2) scratch
+    int64_t bins, hat;
+};
+hipError_t sp_warm(const SpArgs &a, hipStream_t s);
+hipError_t sp_step_stencil(const SpArgs &a, int integrator, hipStream_t s);
+hipError_t sp_step_spectral(const SpArgs &a, const SpectralArgs &sa, hipStream_t s);
+hipError_t sp_init_spectral(const SpArgs &a, const SpectralArgs &sa, hipStream_t s);
+hipError_t sp_mobility_field(const SpArgs &a, double *out, hipStream_t s);
+hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double *result, hipStream_t s);
+hipError_t finalize_pairs(double *partials, int nparts, double *result, hipStream_t s);
+
 // FP64 observables: partials -> result[0] = sum rho, result[1] = sum f-density
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
                        double *result, hipStream_t s);

@@ -99,6 +99,11 @@ hipError_t observables(const Params &p, int precision, int model, const void *rh
     return hipGetLastError();
 }
 
+hipError_t finalize_pairs(double *partials, int nparts, double *result, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nparts, result);
+    return hipGetLastError();
+}
+
 // one block per replica of a batch
 template <typename T, int MODEL>
 __global__ __launch_bounds__(256) void k_batch_obs(const T *__restrict__ buf, const BatchRep *__restrict__ reps,

@@ -58,12 +58,13 @@ class Context:
 def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64", dt=0.01, dx=1.0, k=1.0,
               epsilon=0.9, B2=1.0, delta=1e4, valence=4, regularisation_delta=0.0, mobility="constant", M=1.0,
               rho_min=0.0, rho_ref=1.0, S=0.0, use_dealias=False, with_noise=False, noise_rescale_factor=1.0,
-              seed=0, use_gmres=False, gmres_restart=30, gmres_max_iter=200, gmres_tol=1e-10):
+              seed=0, use_gmres=False, gmres_restart=30, gmres_max_iter=200, gmres_tol=1e-10,
+              density_conversion=1.0):
     d = L.ChDesc()
     d.dims, d.N = dims, N
     d.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision] if isinstance(precision, str) else precision
     d.integrator = L.INTEGRATORS[integrator] if isinstance(integrator, str) else integrator
-    d.model = L.MODELS[free_energy] if isinstance(free_energy, str) else free_energy
+    d.model = ({**L.MODELS, **L.MODELS_SPECIES}[free_energy] if isinstance(free_energy, str) else free_energy)
     d.mobility = L.MOBILITIES[mobility] if isinstance(mobility, str) else mobility
     d.valence, d.use_dealias = valence, int(bool(use_dealias))
     d.epsilon, d.B2, d.delta, d.regularisation_delta = epsilon, B2, delta, regularisation_delta
@@ -72,11 +73,11 @@ def make_desc(dims, N, integrator="euler", free_energy="landau", precision="fp64
     d.with_noise, d.noise_rescale_factor, d.seed = int(bool(with_noise)), noise_rescale_factor, seed
     d.use_gmres, d.gmres_restart, d.gmres_max_iter, d.gmres_tol = int(bool(use_gmres)), gmres_restart, gmres_max_iter, \
         gmres_tol
+    d.density_conversion = density_conversion
     return d
 
 
-def desc_from_config(cfg, dims, precision="fp64"):
-    """ch_desc from a reference TOML table (already parsed to nested dicts)."""
+def _getter(cfg):
     def get(path, default=None, required=False):
         node = cfg
         for part in path.split("."):
@@ -86,6 +87,86 @@ def desc_from_config(cfg, dims, precision="fp64"):
                 return default
             node = node[part]
         return node
+    return get
+
+
+def delta_value(delta):
+    """A bonding volume given as a number or as the {T, deltaH, deltaS, salt,
+    sticky_size} / {value} table of src/utils/Delta.cpp:15-42."""
+    if isinstance(delta, dict):
+        value = delta.get("value", 0.0)
+        if value == 0.0:
+            T, dH, dS = delta["T"], delta["deltaH"], delta["deltaS"]
+            salt, L_DNA = delta.get("salt", 1.0), delta.get("sticky_size", 6)
+            dS_salt = 0.368 * (L_DNA - 1.0) * math.log(salt)
+            dG = dH - T * (dS + dS_salt)
+            value = 1.6606 * math.exp(-dG / (1.9872036 * T))
+        return value
+    return delta
+
+
+def species_model_from_config(cfg):
+    """ChSpeciesModel of a multi-species free energy (saleh, ricci,
+    generic_wertheim), or None for the single-species models.  Keys and unit
+    scaling as SalehWertheim.cpp:18-37, RicciWertheim.cpp:19-36,
+    GenericWertheim.cpp:22-121 and Integrator.cpp:24-27 (k per species ~ m^5)."""
+    get = _getter(cfg)
+    u2i = 1.0 / get("distance_scaling_factor", 1.0)
+    model = get("free_energy", required=True)
+    if model not in L.MODELS_SPECIES:
+        return None
+    m = L.ChSpeciesModel()
+    m.model = L.MODELS_SPECIES[model]
+    if model == "saleh":
+        m.n_species = 3
+        val = get("saleh.valence", required=True)
+        val = [val] * 3 if not isinstance(val, list) else val
+        for i in range(3):
+            m.saleh_valence[i] = int(val[i])
+        m.saleh_B2 = get("saleh.B2", required=True) * u2i ** 3
+        m.saleh_B3 = get("saleh.B3", 0.0) * u2i ** 6
+        m.saleh_delta_AA = delta_value(get("saleh.delta_AA", required=True)) * u2i ** 3
+        m.saleh_delta_BB = delta_value(get("saleh.delta_BB", required=True)) * u2i ** 3
+    elif model == "ricci":
+        m.n_species = 2
+        m.ricci_B2 = get("ricci.B2", required=True) * u2i ** 3
+        m.ricci_delta_00 = delta_value(get("ricci.delta_00", required=True)) * u2i ** 3
+        m.ricci_delta_12 = delta_value(get("ricci.delta_12", required=True)) * u2i ** 3
+    else:
+        species = get("generic_wertheim.species", required=True)
+        m.n_species = len(species)
+        if m.n_species > L.MAX_SPECIES:
+            raise ValueError(f"at most {L.MAX_SPECIES} species are supported")
+        top = 0
+        for s_, sp in enumerate(species):
+            for q in sp["patches"]:
+                m.patch_count[s_][int(q)] += 1
+                top = max(top, int(q))
+        m.n_patches = top + 1
+
+        def pair(text):
+            a, b = text.split("-")
+            return int(a), int(b)
+        for d in get("generic_wertheim.deltas", required=True):
+            a, b = pair(d["interaction"])
+            m.delta[a][b] = m.delta[b][a] = delta_value(d) * u2i ** 3
+        B2s = get("generic_wertheim.B2s", required=True)
+        if len(B2s) != m.n_species * (m.n_species + 1) // 2 and not get("generic_wertheim.allow_unspecified_B2s", False):
+            raise ValueError("The number of B2 coefficients differs from N (N + 1) / 2; set "
+                             "generic_wertheim.allow_unspecified_B2s = true to run regardless")
+        for b2 in B2s:
+            a, b = pair(b2["interaction"])
+            m.B2[a][b] = m.B2[b][a] = b2["value"] * u2i ** 3
+    k = get("k", required=True)
+    ks = k if isinstance(k, list) else [k] * m.n_species
+    for i in range(m.n_species):
+        m.k[i] = ks[i] * u2i ** 5
+    return m
+
+
+def desc_from_config(cfg, dims, precision="fp64"):
+    """ch_desc from a reference TOML table (already parsed to nested dicts)."""
+    get = _getter(cfg)
 
     u2i = 1.0 / get("distance_scaling_factor", 1.0)
     model = get("free_energy", required=True)
@@ -101,19 +182,10 @@ def desc_from_config(cfg, dims, precision="fp64"):
             raise ValueError("The Landau free energy does not support distance_scaling_factor values different from 1.0")
         kw["epsilon"] = eps
     elif model == "simple_wertheim":
-        delta = get("wertheim.delta", required=True)
-        if isinstance(delta, dict):  # src/utils/Delta.cpp:15-42
-            value = delta.get("value", 0.0)
-            if value == 0.0:
-                T, dH, dS = delta["T"], delta["deltaH"], delta["deltaS"]
-                salt, L_DNA = delta.get("salt", 1.0), delta.get("sticky_size", 6)
-                dS_salt = 0.368 * (L_DNA - 1.0) * math.log(salt)
-                dG = dH - T * (dS + dS_salt)
-                value = 1.6606 * math.exp(-dG / (1.9872036 * T))
-            delta = value
+        delta = delta_value(get("wertheim.delta", required=True))
         kw.update(valence=int(get("wertheim.valence", required=True)), B2=get("wertheim.B2", required=True) * u2i ** 3,
                   delta=delta * u2i ** 3, regularisation_delta=get("wertheim.regularisation_delta", 0.0) / u2i ** 3)
-    else:
+    elif model not in L.MODELS_SPECIES:
         raise ValueError(f"Unsupported free energy model '{model}'")
     k = get("k", required=True)
     if isinstance(k, list):
@@ -134,19 +206,27 @@ def desc_from_config(cfg, dims, precision="fp64"):
                      use_gmres=get("pseudospectral.use_gmres", False),
                      gmres_restart=int(get("pseudospectral.gmres_restart", 30)),
                      gmres_max_iter=int(get("pseudospectral.gmres_max_iter", 200)),
-                     gmres_tol=get("pseudospectral.gmres_tol", 1e-10), **kw)
+                     gmres_tol=get("pseudospectral.gmres_tol", 1e-10), density_conversion=u2i ** 3, **kw)
 
 
 class System:
     """One Cahn-Hilliard system resident on the GPU."""
 
-    def __init__(self, ctx, desc):
+    def __init__(self, ctx, desc, species_model=None):
+        """species_model: a ChSpeciesModel for the multi-species Wertheim models;
+        fields are then species-major arrays of n_species * N^dims values."""
         self.ctx, self.desc = ctx, desc
         h = ctypes.c_void_p()
-        _check(ctx.handle, L.lib().ch_sys_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
+        if species_model is None:
+            st = L.lib().ch_sys_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h))
+        else:
+            st = L.lib().ch_sys_create_species(ctx.handle, ctypes.byref(desc), ctypes.byref(species_model),
+                                               ctypes.byref(h))
+        _check(ctx.handle, st)
         self.handle = h
         ctx._children.add(self)
-        self.bins = desc.N ** desc.dims
+        self.species = int(L.lib().ch_sys_species(h))
+        self.bins = desc.N ** desc.dims * self.species
 
     def _chk(self, st):
         _check(self.ctx.handle, st)

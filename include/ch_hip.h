@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 3
+#define CH_ABI_VERSION 4
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -43,7 +43,7 @@ enum { CH_FP64 = 0, CH_FP32 = 1 };
 /* `integrator` key: src/CahnHilliard.cpp:169-220 */
 enum { CH_EULER = 0, CH_EULER_MOBILITY = 1, CH_PSEUDOSPECTRAL = 2, CH_PSEUDOSPECTRAL_MOBILITY = 3 };
 /* `free_energy` key: src/main.cpp:44-62 */
-enum { CH_LANDAU = 0, CH_SIMPLE_WERTHEIM = 1 };
+enum { CH_LANDAU = 0, CH_SIMPLE_WERTHEIM = 1, CH_SALEH_WERTHEIM = 2, CH_RICCI_WERTHEIM = 3, CH_GENERIC_WERTHEIM = 4 };
 /* `mobility.type` key: src/mobility/IMobility.cpp:18-57; LINEAR (M rho / rho_ref) is new */
 enum { CH_MOB_CONSTANT = 0, CH_MOB_FREE_ENERGY = 1, CH_MOB_REGULARISED = 2, CH_MOB_LINEAR = 3 };
 
@@ -88,7 +88,35 @@ typedef struct ch_desc {
     int32_t gmres_max_iter;        /* pseudospectral.gmres_max_iter               */
     int32_t reserved1;             /* 0                                           */
     double gmres_tol;              /* pseudospectral.gmres_tol                    */
+    /* distance_scaling_factor^-3: the Wertheim models' ideal term is
+     * rho (ln(rho * density_conversion) - 1) (FreeEnergyModel.h:20-21); 0 means 1 */
+    double density_conversion;
 } ch_desc;
+
+/* ---- multi-species models (src/models/SalehWertheim.cpp, RicciWertheim.cpp,
+ * GenericWertheim.cpp).  Fields of such a system are species-major: species s
+ * occupies elements [s N^dims, (s + 1) N^dims) of every host buffer. */
+#define CH_MAX_SPECIES 8
+#define CH_MAX_PATCHES 16
+typedef struct ch_species_model {
+    int32_t model;                 /* CH_SALEH_WERTHEIM (3 species), CH_RICCI_WERTHEIM (2), CH_GENERIC_WERTHEIM */
+    int32_t n_species;             /* 3, 2, or the number of generic_wertheim.species (<= CH_MAX_SPECIES) */
+    double k[CH_MAX_SPECIES];      /* the `k` array of the integrator, per species (after unit scaling)   */
+    /* saleh.*: valence[3], B2, B3, delta_AA, delta_BB (internal units) */
+    int32_t saleh_valence[3];
+    int32_t reserved0;
+    double saleh_B2, saleh_B3, saleh_delta_AA, saleh_delta_BB;
+    /* ricci.*: B2, delta_00, delta_12 */
+    double ricci_B2, ricci_delta_00, ricci_delta_12;
+    /* generic_wertheim.*: patch_count[s][p] = how many patches of type p species s carries
+     * (the `patches` lists), n_patches = largest patch type + 1, delta[p][q] and B2[s][t]
+     * symmetric (the `deltas` and `B2s` tables) */
+    int32_t n_patches;
+    int32_t reserved1;
+    int32_t patch_count[CH_MAX_SPECIES][CH_MAX_PATCHES];
+    double delta[CH_MAX_PATCHES][CH_MAX_PATCHES];
+    double B2[CH_MAX_SPECIES][CH_MAX_SPECIES];
+} ch_species_model;
 
 typedef struct ch_ctx ch_ctx;
 typedef struct ch_sys ch_sys;
@@ -112,6 +140,15 @@ void *ch_ctx_stream(ch_ctx *ctx);
  * EulerCUDA.cu:37-55, PseudospectralCUDA.cu:190-230 */
 ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **out);
 void ch_sys_destroy(ch_sys *sys);
+/* == the same for a multi-species model (CahnHilliard.cpp:169-228 with
+ * N_species() > 1): desc->model equals model->model; the euler (constant
+ * mobility), euler_mobility (constant, regularised, linear; free_energy with
+ * CH_GENERIC_WERTHEIM, GenericWertheim.cpp:276-291) and pseudospectral
+ * integrators.  Upload / download / observables use n_species N^dims values;
+ * reverse mode, noise, S(q) and batches are single-species only. */
+ch_status ch_sys_create_species(ch_ctx *ctx, const ch_desc *desc, const ch_species_model *model, ch_sys **out);
+/* number of species of a system (1 for the single-species models) */
+int32_t ch_sys_species(const ch_sys *sys);
 
 /* host -> device copy of rho (N^dims doubles); == CUDAIntegrator::_CPU_GPU
  * (CUDAIntegrator.cu:43-53).  For the pseudospectral integrator it also sets

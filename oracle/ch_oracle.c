@@ -1263,3 +1263,337 @@ int och_psm_step(och_psm *s, double *rho) {
 }
 
 double och_psm_last_M0(const och_psm *s) { return s->M0; }
+
+/* ------------------------------------------------------------------ */
+/* multi-species Wertheim models                                       */
+/* ------------------------------------------------------------------ */
+/* Generic tables, GenericWertheim.cpp:22-150: the unique patch types in
+ * ascending order; per unique patch the terms (species, other patch,
+ * multiplicity * delta) over the species in order and, within a species, its
+ * patch types in ascending order; per species its patch types with
+ * multiplicities */
+typedef struct {
+    int n_unique, unique[OCH_MAX_PATCHES];
+    int n_terms[OCH_MAX_PATCHES];
+    int t_species[OCH_MAX_PATCHES][OCH_MAX_SPECIES * OCH_MAX_PATCHES];
+    int t_other[OCH_MAX_PATCHES][OCH_MAX_SPECIES * OCH_MAX_PATCHES];
+    double t_coeff[OCH_MAX_PATCHES][OCH_MAX_SPECIES * OCH_MAX_PATCHES];
+} generic_tables;
+
+static void generic_build(const och_species_model *m, generic_tables *g) {
+    memset(g, 0, sizeof(*g));
+    int used[OCH_MAX_PATCHES] = {0};
+    for(int s = 0; s < m->n_species; s++)
+        for(int q = 0; q < m->n_patches; q++)
+            if(m->patch_count[s][q] > 0) used[q] = 1;
+    for(int q = 0; q < m->n_patches; q++)
+        if(used[q]) g->unique[g->n_unique++] = q;
+    for(int u = 0; u < g->n_unique; u++) {
+        int patch = g->unique[u], n = 0;
+        for(int s = 0; s < m->n_species; s++)
+            for(int q = 0; q < m->n_patches; q++)
+                if(m->patch_count[s][q] > 0 && m->delta[patch][q] > 0.0) {
+                    g->t_species[u][n] = s;
+                    g->t_other[u][n] = q;
+                    g->t_coeff[u][n] = (double)m->patch_count[s][q] * m->delta[patch][q];
+                    n++;
+                }
+        g->n_terms[u] = n;
+    }
+}
+
+void och_ms_update_X(const och_species_model *m, const double *rhos, double *X) {
+    generic_tables g;
+    generic_build(m, &g);
+    const double tolerance = 1e-8;
+    for(int iter = 0; iter < 10000; iter++) {
+        double max_delta = 0.0;
+        for(int u = 0; u < g.n_unique; u++) {
+            int patch = g.unique[u];
+            double sum = 0.0;
+            for(int t = 0; t < g.n_terms[u]; t++) {
+                double rho = rhos[g.t_species[u][t]];
+                if(rho != 0.0) sum += rho * X[g.t_other[u][t]] * g.t_coeff[u][t];
+            }
+            double new_X = 1.0 / (1.0 + sum);
+            double d = fabs(new_X - X[patch]);
+            max_delta = (max_delta < d) ? d : max_delta;
+            X[patch] = new_X;
+        }
+        if(max_delta < tolerance) return;
+    }
+}
+
+/* SalehWertheim.cpp:80-85 */
+static double saleh_contribution(const och_species_model *m, const double *rhos, int species) {
+    double delta = (species == 0) ? m->saleh_delta_AA : m->saleh_delta_BB;
+    int lh = m->saleh_valence[2] / 2;
+    double rho_factor = delta * (m->saleh_valence[species] * rhos[species] + lh * rhos[2]);
+    double X = (-1.0 + sqrt(1.0 + 4.0 * rho_factor)) / (2.0 * rho_factor);
+    return (rho_factor >= 0) ? log(X) : 0.0;
+}
+
+/* SalehWertheim.cpp:87-110, RicciWertheim.cpp:72-98, GenericWertheim.cpp:250-274 */
+void och_ms_der(const och_species_model *m, const double *rhos, double *X, double *out) {
+    const int S = m->n_species;
+    double logX[OCH_MAX_PATCHES];
+    if(m->model == OCH_GENERIC) {
+        och_ms_update_X(m, rhos, X);
+        for(int i = 0; i < m->n_patches; i++) logX[i] = log(X[i]);
+    }
+    for(int s = 0; s < S; s++) {
+        out[s] = 0.0;
+        if(rhos[s] == 0.) continue;
+        double rho_tot = 0.;
+        for(int t = 0; t < S; t++) rho_tot += rhos[t];
+        double ref, bond = 0.0;
+        if(m->model == OCH_SALEH) {
+            ref = log(rhos[s]) + 2.0 * m->saleh_B2 * rho_tot + 3.0 * m->saleh_B3 * SQR(rho_tot);
+            int lh = m->saleh_valence[2] / 2;
+            if(s == 0) bond = m->saleh_valence[0] * saleh_contribution(m, rhos, 0);
+            else if(s == 1) bond = m->saleh_valence[1] * saleh_contribution(m, rhos, 1);
+            else bond = lh * (saleh_contribution(m, rhos, 0) + saleh_contribution(m, rhos, 1));
+        }
+        else if(m->model == OCH_RICCI) {
+            ref = log(rhos[s]) + 2.0 * m->ricci_B2 * rho_tot;
+            double A = rhos[1] * m->ricci_delta_12, B = rhos[0] * m->ricci_delta_12;
+            if(s == 0) {
+                double d00 = m->ricci_delta_00;
+                double X0 = (-1 + sqrt((1 + 12 * rhos[0] * d00))) / (6 * rhos[0] * d00);
+                double X1 = (B - 1 - 2 * A + sqrt(SQR(1 + 2 * A - B) + 4 * B)) / (2 * B);
+                bond = 3 * log(X0) + log(X1);
+            }
+            else {
+                double X2 = (-B + 2 * A - 1 + sqrt(SQR(B - 2 * A + 1) + 8 * A)) / (4 * A);
+                bond = 2 * log(X2);
+            }
+        }
+        else {
+            double B2c = 0.;
+            for(int o = 0; o < S; o++) B2c += 2.0 * rhos[o] * m->B2[s][o];
+            ref = log(rhos[s]) + B2c;
+            for(int q = 0; q < m->n_patches; q++)
+                if(m->patch_count[s][q] > 0) bond += m->patch_count[s][q] * logX[q];
+        }
+        out[s] = ref + bond;
+    }
+}
+
+static double wertheim_term(double X) { return log(X) - X / 2.0 + 0.5; }
+
+/* SalehWertheim.cpp:47-78, RicciWertheim.cpp:43-70, GenericWertheim.cpp:210-248 */
+double och_ms_bulk(const och_species_model *m, double dcf, const double *rhos) {
+    const int S = m->n_species;
+    double rho = 0.;
+    for(int i = 0; i < S; i++) rho += rhos[i];
+    double mixing_S = 0., B2c = 0.;
+    for(int i = 0; i < S; i++) {
+        double x_i = rhos[i] / rho;
+        mixing_S += (x_i > 0) ? x_i * log(x_i) : 0;
+        if(m->model == OCH_GENERIC)
+            for(int j = 0; j < S; j++) B2c += rhos[i] * rhos[j] * m->B2[i][j];
+    }
+    if(m->model == OCH_SALEH) {
+        int lh = m->saleh_valence[2] / 2;
+        double f_ref = rho * (log(rho * dcf) - 1.0 + mixing_S + m->saleh_B2 * rho + 0.5 * m->saleh_B3 * SQR(rho));
+        double rf = m->saleh_delta_AA * (m->saleh_valence[0] * rhos[0] + lh * rhos[2]);
+        double X1A = (-1.0 + sqrt(1.0 + 4.0 * rf)) / (2.0 * rf);
+        double p1 = log(X1A) - X1A / 2.0 + 0.5;
+        rf = m->saleh_delta_BB * (m->saleh_valence[1] * rhos[1] + lh * rhos[2]);
+        double X2B = (-1.0 + sqrt(1.0 + 4.0 * rf)) / (2.0 * rf);
+        double p2 = log(X2B) - X2B / 2.0 + 0.5;
+        double bond = rhos[0] * m->saleh_valence[0] * p1 + rhos[1] * m->saleh_valence[1] * p2 + rhos[2] * lh * (p1 + p2);
+        return f_ref + bond;
+    }
+    if(m->model == OCH_RICCI) {
+        double f_ref = rho * (log(rho * dcf) - 1.0 + mixing_S + m->ricci_B2 * rho);
+        double d00 = m->ricci_delta_00, d12 = m->ricci_delta_12;
+        double X0 = (-1 + sqrt((1 + 12 * rhos[0] * d00))) / (6 * rhos[0] * d00);
+        double A = rhos[1] * d12, B = rhos[0] * d12;
+        double X1 = (B - 1 - 2 * A + sqrt(SQR(1 + 2 * A - B) + 4 * B)) / (2 * B);
+        double X2 = (-B + 2 * A - 1 + sqrt(SQR(B - 2 * A + 1) + 8 * A)) / (4 * A);
+        double bond = rhos[0] * (3 * (log(X0) - X0 / 2) + log(X1) - X1 / 2 + 2);
+        bond += rhos[1] * (2 * log(X2) - X2 + 1);
+        return f_ref + bond;
+    }
+    double f_ref = rho * (log(rho * dcf) - 1.0 + mixing_S) + B2c;
+    double X[OCH_MAX_PATCHES];
+    for(int q = 0; q < OCH_MAX_PATCHES; q++) X[q] = 0.0;
+    och_ms_update_X(m, rhos, X);
+    double bond = 0;
+    for(int s = 0; s < S; s++) {
+        double sf = 0;
+        for(int q = 0; q < m->n_patches; q++)
+            if(m->patch_count[s][q] > 0) sf += m->patch_count[s][q] * wertheim_term(X[q]);
+        bond += rhos[s] * sf;
+    }
+    return f_ref + bond;
+}
+
+struct och_ms {
+    int integrator;
+    och_params p;
+    och_species_model m;
+    int S;
+    int64_t bins, hat;
+    double *X;                 /* n_patches per cell (Generic), patch-major X[q * bins + i] */
+    double *fder, *mob, *mu;   /* S * bins */
+    double *rho_hat, *fh, *th; /* S * 2 * hat (pseudospectral) */
+    och_ps *tab;               /* q^2 and dealias tables */
+};
+
+/* f' of every cell (X updated), the model's der_bulk_free_energy over the field */
+static void ms_der_field(och_ms *s, const double *rho) {
+    const int S = s->S, P = s->m.n_patches;
+    for(int64_t i = 0; i < s->bins; i++) {
+        double rhos[OCH_MAX_SPECIES], X[OCH_MAX_PATCHES], out[OCH_MAX_SPECIES];
+        for(int t = 0; t < S; t++) rhos[t] = rho[t * s->bins + i];
+        for(int q = 0; q < P; q++) X[q] = s->X[q * s->bins + i];
+        och_ms_der(&s->m, rhos, X, out);
+        for(int q = 0; q < P; q++) s->X[q * s->bins + i] = X[q];
+        for(int t = 0; t < S; t++) s->fder[t * s->bins + i] = out[t];
+    }
+}
+
+/* IMobility::update_mobility for every species (RegularisedMobility.h:26-33,
+ * GenericWertheim.cpp:276-291 with X of the last f' evaluation) */
+static void ms_mobility(och_ms *s, const double *rho) {
+    const och_params *p = &s->p;
+    for(int t = 0; t < s->S; t++)
+        for(int64_t i = 0; i < s->bins; i++) {
+            double r = rho[t * s->bins + i], v;
+            switch(p->mobility) {
+            case OCH_MOB_CONSTANT: v = p->M; break;
+            case OCH_MOB_REGULARISED: v = p->M * r / (r + p->rho_min); break;
+            case OCH_MOB_LINEAR: v = p->M * r / p->rho_ref; break;
+            default: {
+                double D = p->M;
+                for(int q = 0; q < s->m.n_patches; q++)
+                    if(s->m.patch_count[t][q] > 0) D *= pow(s->X[q * s->bins + i], (double)s->m.patch_count[t][q]);
+                v = r * D;
+            }
+            }
+            s->mob[t * s->bins + i] = v;
+        }
+}
+
+och_ms *och_ms_create(int integrator, const och_params *p, const och_species_model *m, const double *rho) {
+    och_ms *s = (och_ms *)calloc(1, sizeof(och_ms));
+    s->integrator = integrator;
+    s->p = *p;
+    s->m = *m;
+    s->S = m->n_species;
+    s->bins = ipow(p->N, p->dims);
+    s->hat = ipow(p->N, p->dims - 1) * (p->N / 2 + 1);
+    int64_t SB = s->S * s->bins;
+    s->X = (double *)malloc(sizeof(double) * (m->n_patches > 0 ? m->n_patches : 1) * s->bins);
+    for(int64_t i = 0; i < (int64_t)(m->n_patches > 0 ? m->n_patches : 1) * s->bins; i++) s->X[i] = 1.0;
+    s->fder = (double *)calloc(SB, sizeof(double));
+    s->mob = (double *)calloc(SB, sizeof(double));
+    s->mu = (double *)calloc(SB, sizeof(double));
+    /* CahnHilliard.cpp:226-228: one f' evaluation at set-up initialises X */
+    ms_der_field(s, rho);
+    if(integrator == 2) {
+        s->tab = och_ps_create(p, &(och_model){OCH_LANDAU, 0.9, 0, 0, 0, 0}, rho);
+        s->rho_hat = (double *)malloc(sizeof(double) * 2 * s->hat * s->S);
+        s->fh = (double *)malloc(sizeof(double) * 2 * s->hat * s->S);
+        s->th = (double *)malloc(sizeof(double) * 2 * s->hat);
+        for(int t = 0; t < s->S; t++) r2c(p->dims, p->N, rho + t * s->bins, s->rho_hat + 2 * t * s->hat);
+    }
+    return s;
+}
+
+void och_ms_destroy(och_ms *s) {
+    if(!s) return;
+    free(s->X); free(s->fder); free(s->mob); free(s->mu);
+    free(s->rho_hat); free(s->fh); free(s->th);
+    if(s->tab) och_ps_destroy(s->tab);
+    free(s);
+}
+
+const double *och_ms_X(const och_ms *s) { return s->X; }
+
+/* one CahnHilliard::evolve (:279-282): update_mobility, then EulerCPU::evolve
+ * (:18-36), EulerMobilityCPU::evolve (:31-83) or PseudospectralCPU::evolve
+ * (:112-136), every species with its own k */
+void och_ms_step(och_ms *s, double *rho) {
+    const och_params *p = &s->p;
+    const int S = s->S, dims = p->dims, N = p->N;
+    const int64_t bins = s->bins;
+    ms_mobility(s, rho);
+    ms_der_field(s, rho);
+    if(s->integrator == 2) {
+        double M = s->mob[0];
+        for(int t = 0; t < S; t++) {
+            double *fh = s->fh + 2 * t * s->hat, *rh = s->rho_hat + 2 * t * s->hat;
+            r2c(dims, N, s->fder + t * bins, fh);
+            for(int64_t k = 0; k < s->hat; k++) {
+                if(p->use_dealias) { fh[2 * k] *= s->tab->dealiaser[k]; fh[2 * k + 1] *= s->tab->dealiaser[k]; }
+                double k2 = s->tab->k2[k], k4 = SQR(k2);
+                double denom = 1.0 + p->dt * M * (p->S * k2 + 2.0 * s->m.k[t] * k4);
+                for(int c = 0; c < 2; c++) {
+                    double v = (rh[2 * k + c] - p->dt * M * k2 * (fh[2 * k + c] - p->S * rh[2 * k + c])) / denom;
+                    rh[2 * k + c] = v;
+                    s->th[2 * k + c] = v / bins;
+                }
+            }
+            c2r(dims, N, s->th, rho + t * bins);
+        }
+        return;
+    }
+    for(int t = 0; t < S; t++)
+        for(int64_t i = 0; i < bins; i++)
+            s->mu[t * bins + i] = s->fder[t * bins + i] - 2 * s->m.k[t] * cell_laplacian(dims, N, p->dx, rho + t * bins, i);
+    if(s->integrator == 0) {
+        double M = s->mob[0];
+        for(int t = 0; t < S; t++)
+            for(int64_t i = 0; i < bins; i++)
+                s->fder[t * bins + i] = M * cell_laplacian(dims, N, p->dx, s->mu + t * bins, i);
+        for(int64_t i = 0; i < S * bins; i++) rho[i] += s->fder[i] * p->dt;
+        return;
+    }
+    for(int t = 0; t < S; t++) {
+        const double *mu = s->mu + t * bins, *mob = s->mob + t * bins;
+        double *r = rho + t * bins, *upd = s->fder + t * bins;
+        for(int64_t i = 0; i < bins; i++) {
+            double res = 0;
+            for(int d = 0; d < dims; d++) {
+                int64_t ip = neighbour(dims, N, i, d, 1), im = neighbour(dims, N, i, d, -1);
+                double Jq = 0.5 * (mob[i] + mob[ip]) * (mu[ip] - mu[i]) / p->dx;
+                double Jm = 0.5 * (mob[im] + mob[i]) * (mu[i] - mu[im]) / p->dx;
+                res += (Jq - Jm) / p->dx;
+            }
+            upd[i] = res;
+        }
+        for(int64_t i = 0; i < bins; i++) r[i] += upd[i] * p->dt;
+    }
+}
+
+/* CahnHilliard.cpp:285-319 over every species */
+void och_ms_observables(const och_ms *s, double dcf, const double *rho, double *mass, double *fe) {
+    const och_params *p = &s->p;
+    const int S = s->S, dims = p->dims, N = p->N;
+    double ms = 0., f = 0.;
+    for(int64_t i = 0; i < s->bins; i++) {
+        double rhos[OCH_MAX_SPECIES], cell = 0.;
+        for(int t = 0; t < S; t++) {
+            rhos[t] = rho[t * s->bins + i];
+            cell += rhos[t];
+        }
+        ms += cell;
+        double inter = 0.;
+        int c[3], cp[3];
+        fill_coords(dims, N, i, c);
+        for(int t = 0; t < S; t++)
+            for(int d = 0; d < dims; d++) {
+                memcpy(cp, c, sizeof(c));
+                cp[d] = (c[d] + 1) & (N - 1);
+                double g = (rho[t * s->bins + cell_idx(dims, N, cp)] - rhos[t]) / p->dx;
+                inter += s->m.k[t] * g * g;
+            }
+        f += och_ms_bulk(&s->m, dcf, rhos) + inter;
+    }
+    double V_bin = p->dx * p->dx * p->dx;
+    *mass = ms * V_bin / s->bins;
+    *fe = f * V_bin / s->bins;
+}

@@ -139,6 +139,39 @@ double och_psm_last_M0(const och_psm *s);
 void och_psm_destroy(och_psm *s);
 /* the spectral div(mob grad g) of the integrator (c2r semantics included) */
 void och_psm_div_mob_grad(const och_params *p, const double *mob, const double *g, double *out);
+/* ---- multi-species Wertheim models (src/models/SalehWertheim.cpp,
+ * RicciWertheim.cpp, GenericWertheim.cpp) and the stencil / pseudospectral
+ * integrators over species-major fields rho[s * bins + i].  The struct has
+ * the layout of ch_species_model. */
+enum { OCH_SALEH = 2, OCH_RICCI = 3, OCH_GENERIC = 4 };
+#define OCH_MAX_SPECIES 8
+#define OCH_MAX_PATCHES 16
+typedef struct och_species_model {
+    int32_t model, n_species;
+    double k[OCH_MAX_SPECIES];
+    int32_t saleh_valence[3], reserved0;
+    double saleh_B2, saleh_B3, saleh_delta_AA, saleh_delta_BB;
+    double ricci_B2, ricci_delta_00, ricci_delta_12;
+    int32_t n_patches, reserved1;
+    int32_t patch_count[OCH_MAX_SPECIES][OCH_MAX_PATCHES];
+    double delta[OCH_MAX_PATCHES][OCH_MAX_PATCHES];
+    double B2[OCH_MAX_SPECIES][OCH_MAX_SPECIES];
+} och_species_model;
+/* Generic: the mass-action fixed point, warm-started from X (n_patches values,
+ * in/out), GenericWertheim.cpp:184-208 */
+void och_ms_update_X(const och_species_model *m, const double *rhos, double *X);
+/* f'_s for every species at one cell (X in/out for Generic); a species whose
+ * density is exactly 0 gets 0 */
+void och_ms_der(const och_species_model *m, const double *rhos, double *X, double *out);
+/* the bulk free-energy density at one cell (Generic: X from 0, as bonding_free_energy) */
+double och_ms_bulk(const och_species_model *m, double density_conversion, const double *rhos);
+typedef struct och_ms och_ms;
+/* integrator 0 euler, 1 euler_mobility, 2 pseudospectral; p->k is unused (m->k per species) */
+och_ms *och_ms_create(int integrator, const och_params *p, const och_species_model *m, const double *rho);
+void och_ms_step(och_ms *s, double *rho);
+void och_ms_observables(const och_ms *s, double density_conversion, const double *rho, double *mass, double *fe);
+const double *och_ms_X(const och_ms *s);
+void och_ms_destroy(och_ms *s);
 /* F = average_free_energy(rho) with dF/drho (may be NULL) and the explicit
  * dF/dparams[OCH_NGRADS] (may be NULL) */
 double och_free_energy_grad(const och_params *p, const och_model *m, const double *rho, double *dF_drho,

@@ -110,6 +110,14 @@ def lib(fast=False):
         "och_psm_last_M0": (D, [ctypes.c_void_p]),
         "och_psm_destroy": (None, [ctypes.c_void_p]),
         "och_psm_div_mob_grad": (None, [PP, P, P, P]),
+        "och_ms_update_X": (None, [P, P, P]),
+        "och_ms_der": (None, [P, P, P, P]),
+        "och_ms_bulk": (D, [P, D, P]),
+        "och_ms_create": (ctypes.c_void_p, [I, PP, P, P]),
+        "och_ms_step": (None, [ctypes.c_void_p, P]),
+        "och_ms_observables": (None, [ctypes.c_void_p, D, P, P, P]),
+        "och_ms_X": (ctypes.c_void_p, [ctypes.c_void_p]),
+        "och_ms_destroy": (None, [ctypes.c_void_p]),
         "och_philox4x32_10": (None, [P, P, P]),
         "och_cell_normals": (None, [ctypes.c_uint64, I64, I64, P]),
         "och_euler_mobility_noise_step": (None, [PP, MP, D, ctypes.c_uint64, I64, P]),
@@ -368,3 +376,87 @@ def psm_div_mob_grad(params, mob, g):
     out = np.zeros(params.bins)
     _L().och_psm_div_mob_grad(ctypes.byref(params.c()), _ptr(mob), _ptr(g), _ptr(out))
     return out
+
+
+# ---- multi-species models (layout of ch_species_model)
+SALEH, RICCI, GENERIC = 2, 3, 4
+MAX_SPECIES, MAX_PATCHES = 8, 16
+
+
+class SpeciesModel(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("n_species", ctypes.c_int32), ("k", ctypes.c_double * MAX_SPECIES),
+                ("saleh_valence", ctypes.c_int32 * 3), ("reserved0", ctypes.c_int32),
+                ("saleh_B2", ctypes.c_double), ("saleh_B3", ctypes.c_double), ("saleh_delta_AA", ctypes.c_double),
+                ("saleh_delta_BB", ctypes.c_double), ("ricci_B2", ctypes.c_double), ("ricci_delta_00", ctypes.c_double),
+                ("ricci_delta_12", ctypes.c_double), ("n_patches", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+                ("patch_count", (ctypes.c_int32 * MAX_PATCHES) * MAX_SPECIES),
+                ("delta", (ctypes.c_double * MAX_PATCHES) * MAX_PATCHES),
+                ("B2", (ctypes.c_double * MAX_SPECIES) * MAX_SPECIES)]
+
+
+def saleh_model(k, valence=(4, 4, 2), B2=1.0, B3=0.0, delta_AA=1e3, delta_BB=2e3, model_cls=None):
+    m = (model_cls or SpeciesModel)()
+    m.model, m.n_species = SALEH, 3
+    for i, v in enumerate(np.broadcast_to(k, 3)):
+        m.k[i] = v
+    for i, v in enumerate(valence):
+        m.saleh_valence[i] = v
+    m.saleh_B2, m.saleh_B3, m.saleh_delta_AA, m.saleh_delta_BB = B2, B3, delta_AA, delta_BB
+    return m
+
+
+def ricci_model(k, B2=1.0, delta_00=1e3, delta_12=5e2, model_cls=None):
+    m = (model_cls or SpeciesModel)()
+    m.model, m.n_species = RICCI, 2
+    for i, v in enumerate(np.broadcast_to(k, 2)):
+        m.k[i] = v
+    m.ricci_B2, m.ricci_delta_00, m.ricci_delta_12 = B2, delta_00, delta_12
+    return m
+
+
+def generic_model(k, species_patches, deltas, B2s, model_cls=None):
+    """species_patches: list of patch-type lists; deltas: {(p, q): delta}; B2s: {(s, t): B2}."""
+    m = (model_cls or SpeciesModel)()
+    m.model, m.n_species = GENERIC, len(species_patches)
+    for i, v in enumerate(np.broadcast_to(k, len(species_patches))):
+        m.k[i] = v
+    m.n_patches = max(max(p) for p in species_patches) + 1
+    for s_, pl in enumerate(species_patches):
+        for q in pl:
+            m.patch_count[s_][q] += 1
+    for (p_, q), d in deltas.items():
+        m.delta[p_][q] = m.delta[q][p_] = d
+    for (a, b), v in B2s.items():
+        m.B2[a][b] = m.B2[b][a] = v
+    return m
+
+
+def ms_der(model, rhos, X=None):
+    rhos = np.ascontiguousarray(rhos, dtype=np.float64)
+    X = np.ones(MAX_PATCHES) if X is None else np.ascontiguousarray(X, dtype=np.float64)
+    out = np.zeros(model.n_species)
+    _L().och_ms_der(ctypes.byref(model), _ptr(rhos), _ptr(X), _ptr(out))
+    return out, X
+
+
+def ms_bulk(model, rhos, dcf=1.0):
+    rhos = np.ascontiguousarray(rhos, dtype=np.float64)
+    return _L().och_ms_bulk(ctypes.byref(model), dcf, _ptr(rhos))
+
+
+def run_ms(integrator, params, model, rho, steps, dcf=1.0, observe=False):
+    """`steps` steps of a multi-species system; rho is species-major (S * bins)."""
+    rho = np.array(rho, dtype=np.float64, copy=True)
+    L = _L()
+    h = L.och_ms_create(integrator, ctypes.byref(params.c()), ctypes.byref(model), _ptr(rho))
+    try:
+        for _ in range(steps):
+            L.och_ms_step(h, _ptr(rho))
+        obs = None
+        if observe:
+            mass, fe = ctypes.c_double(), ctypes.c_double()
+            L.och_ms_observables(h, dcf, _ptr(rho), ctypes.byref(mass), ctypes.byref(fe))
+            obs = (mass.value, fe.value)
+    finally:
+        L.och_ms_destroy(h)
+    return (rho, obs) if observe else rho

@@ -28,7 +28,7 @@ def test_every_header_in_include_is_exported(ch):
 
 def test_abi_version_and_status_strings_without_gpu(ch):
     L = ch._lib.lib()
-    assert L.ch_abi_version() == 3
+    assert L.ch_abi_version() == 4
     assert L.ch_status_string(3) == b"NaN encountered"
 
 
@@ -69,18 +69,20 @@ def test_noise_keys_from_reference_config(ch):
     assert ch.desc_from_config({**cfg, "mobility": {"type": "constant"}}, dims=2).with_noise == 0
 
 
-def test_desc_layout_matches_header(ch, tmp_path):
-    """sizeof / offsetof of every ch_desc field, from the C header, equal the ctypes mirror."""
-    fields = [f for f, _ in ch._lib.ChDesc._fields_]
+@pytest.mark.parametrize("cname,pyname", [("ch_desc", "ChDesc"), ("ch_species_model", "ChSpeciesModel")])
+def test_desc_layout_matches_header(ch, tmp_path, cname, pyname):
+    """sizeof / offsetof of every field of the ABI structs, from the C header, equal the ctypes mirror."""
+    cls = getattr(ch._lib, pyname)
+    fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ch_hip.h"\nint main(void) {\n'
-                   '    printf("%zu\\n", sizeof(ch_desc));\n' +
-                   "".join(f'    printf("%zu\\n", offsetof(ch_desc, {f}));\n' for f in fields) + "    return 0;\n}\n")
+                   f'    printf("%zu\\n", sizeof({cname}));\n' +
+                   "".join(f'    printf("%zu\\n", offsetof({cname}, {f}));\n' for f in fields) + "    return 0;\n}\n")
     exe = tmp_path / "layout"
     import subprocess
     subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    want = [ctypes.sizeof(ch._lib.ChDesc)] + [getattr(ch._lib.ChDesc, f).offset for f in fields]
+    want = [ctypes.sizeof(cls)] + [getattr(cls, f).offset for f in fields]
     assert got == want
 
 
@@ -93,3 +95,23 @@ def test_pseudospectral_mobility_keys_from_reference_config(ch):
     d = ch.desc_from_config(cfg, dims=2)
     assert d.integrator == ch.PSEUDOSPECTRAL_MOBILITY and d.S == 0.5 and d.use_dealias == 1
     assert (d.use_gmres, d.gmres_restart, d.gmres_max_iter, d.gmres_tol) == (1, 30, 200, 1e-8)
+
+
+def test_species_model_from_reference_config(ch):
+    import oracle as O
+    cfg = {"N": 32, "dt": 1e-4, "k": [1.0, 2.0, 3.0], "free_energy": "generic_wertheim", "integrator": "euler",
+           "generic_wertheim": {"species": [{"patches": [0, 0, 1]}, {"patches": [1, 2]}, {"patches": [2]}],
+                                "deltas": [{"interaction": "0-1", "value": 800.0}, {"interaction": "2-2", "value": 50.0}],
+                                "B2s": [{"interaction": f"{a}-{b}", "value": 0.1 * (a + b)}
+                                        for a in range(3) for b in range(a, 3)]}}
+    m = ch.species_model_from_config(cfg)
+    assert (m.model, m.n_species, m.n_patches) == (ch.GENERIC_WERTHEIM, 3, 3)
+    assert list(m.k)[:3] == [1.0, 2.0, 3.0] and m.patch_count[0][0] == 2 and m.patch_count[0][1] == 1
+    assert m.delta[1][0] == 800.0 and m.B2[2][1] == pytest.approx(0.3)
+    assert ch.desc_from_config(cfg, dims=2).model == ch.GENERIC_WERTHEIM
+    # the product's struct and the oracle's have one layout
+    assert ctypes.sizeof(ch.ChSpeciesModel) == ctypes.sizeof(O.SpeciesModel)
+    sal = ch.species_model_from_config({"N": 8, "dt": 1e-4, "k": 1.0, "free_energy": "saleh",
+                                         "saleh": {"valence": [4, 4, 2], "B2": 1.0, "delta_AA": 1e3,
+                                                   "delta_BB": {"value": 2e3}}})
+    assert list(sal.saleh_valence) == [4, 4, 2] and sal.saleh_delta_BB == 2e3 and list(sal.k)[:3] == [1.0] * 3
