@@ -373,6 +373,42 @@ def w_psm(ch, ctx, args):
     return out
 
 
+def w_species(ch, ctx, args):
+    """Multi-species Wertheim systems, 2D N=512 FP64, rho_s = 0.03 (1 + U[-0.1, 0.1]):
+    SalehWertheim (3 species, valence 4/4/2, B2 = 0.7, delta_AA = 1e3,
+    delta_BB = 2e3) with euler, dt = 1e-4, and GenericWertheim with the same
+    topology (X by mass-action sweeps, warm-started per cell) with
+    euler_mobility and free-energy mobility.  Rate = species-cell updates / s."""
+    N = 512
+    rng = np.random.default_rng(7)
+    out = {}
+    shapes = {
+        "saleh_euler": (ch.saleh_model(1.0, valence=(4, 4, 2), B2=0.7, delta_AA=1e3, delta_BB=2e3), "euler",
+                        "constant", "saleh"),
+        "generic_em_free_energy": (ch.generic_model(1.0, [[0] * 4, [1] * 4, [0, 1]], {(0, 0): 1e3, (1, 1): 2e3},
+                                                    {(a, b): 0.7 for a in range(3) for b in range(3)}),
+                                   "euler_mobility", "free_energy", "generic_wertheim"),
+    }
+    nst = 256 if args.steps >= 100 else 64
+    for name, (m, integ, mob, fe) in shapes.items():
+        d = ch.make_desc(dims=2, N=N, integrator=integ, free_energy=fe, mobility=mob, M=1.0, dt=1e-4, dx=1.0)
+        s = ch.System(ctx, d, species_model=m)
+        rho0 = 0.03 * (1 + 0.1 * (2 * rng.random(3 * N * N) - 1))
+        s.upload(rho0)
+        s.evolve(64)
+        sync(ctx)
+        t0 = time.perf_counter()
+        s.evolve(nst)
+        sync(ctx)
+        dt = time.perf_counter() - t0
+        rho = s.download()
+        s.close()
+        out[name] = {"us_per_time_step": round(dt / nst * 1e6, 2),
+                     "Gspecies-cell-updates/s": round(3 * N * N * nst / dt / 1e9, 3), "time_steps": nst,
+                     "finite": bool(np.isfinite(rho).all())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,7 +420,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--workload", default=None,
                     choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
-                             "wertheim", "noise", "psm", "copy"])
+                             "wertheim", "noise", "psm", "species", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -399,7 +435,7 @@ def main():
              "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
              "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
              "wertheim": lambda: w_wertheim(ch, ctx, args), "noise": lambda: w_noise(ch, ctx, args),
-             "psm": lambda: w_psm(ch, ctx, args),
+             "psm": lambda: w_psm(ch, ctx, args), "species": lambda: w_species(ch, ctx, args),
              "copy": lambda: w_copy(ch, ctx, args)}
     if args.workload and args.workload != "euler":
         r = table[args.workload]()
@@ -443,7 +479,7 @@ def main():
                                "fp32_rel_linf_vs_fp64": fp32_error(ch, ctx, "euler", 0.01, N, 1000),
                                "error_after_steps": 1000}
         # a failing extra workload is reported, never allowed to drop the headline line
-        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise", "psm"):
+        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise", "psm", "species"):
             try:
                 extra[name] = table[name]()
             except Exception as e:  # noqa: BLE001

@@ -63,9 +63,11 @@ public:
         _traj = std::filesystem::path(trajp);
         if(!std::filesystem::is_directory(_traj)) critical("Trajectory output path '{}' does not exist", trajp);
         if(!_print_vtk && !chtoml::find(config, "load_from")) {
-            for(const char *prefix : {"traj", "pressure", "mu"}) {
-                std::ofstream o((_traj / (std::string(prefix) + "_0.dat")).string(), std::ios_base::out);
-            }
+            for(int i = 0; i < _state.model->N_species(); i++)
+                for(const char *prefix : {"traj", "pressure", "mu"}) {
+                    std::ofstream o((_traj / (std::string(prefix) + "_" + std::to_string(i) + ".dat")).string(),
+                                    std::ios_base::out);
+                }
         }
     }
 
@@ -91,25 +93,33 @@ public:
         return best;
     }
 
+    // Printer.cpp:97-114: one file per species and the total density
     void print_current_state(const std::string &prefix, long long t) {
         double factor = _state.density_to_user(1.0);
-        write_native((_out / (prefix + "0.dat")).string(), "density", _state.rho, factor, 0, t);
-        write_vtk((_out / (prefix + "0.vtk")).string(), "density", _state.rho, factor, 0, t);
+        for(int i = 0; i < _state.model->N_species(); i++) {
+            const std::string s = std::to_string(i);
+            write_native((_out / (prefix + s + ".dat")).string(), "density", _state.rho, factor, i, t);
+            write_vtk((_out / (prefix + s + ".vtk")).string(), "density", _state.rho, factor, i, t);
+        }
         write_native((_out / (prefix + "density.dat")).string(), "density", _state.rho, factor, -1, t);
         write_vtk((_out / (prefix + "density.vtk")).string(), "density", _state.rho, factor, -1, t);
     }
 
+    // Printer.cpp:116-138, one species; trajectory_frame_done() after the
+    // species of a frame (advances the log cadence once per frame)
     void add_to_trajectory(const std::string &prefix, const std::string &obs, const MultiField<double> &field,
-                           double factor, long long t) {
+                           double factor, int species, long long t) {
+        const std::string s = std::to_string(species);
         if(_print_vtk) {
-            write_vtk((_traj / (prefix + "_0_" + std::to_string(t) + ".vtk")).string(), obs, field, factor, 0, t);
+            write_vtk((_traj / (prefix + "_" + s + "_" + std::to_string(t) + ".vtk")).string(), obs, field, factor,
+                      species, t);
         }
         else {
-            std::ofstream o((_traj / (prefix + "_0.dat")).string(), std::ios_base::app);
-            write_native(o, obs, field, factor, 0, t);
+            std::ofstream o((_traj / (prefix + "_" + s + ".dat")).string(), std::ios_base::app);
+            write_native(o, obs, field, factor, species, t);
         }
-        _traj_printed++;
     }
+    void trajectory_frame_done() { _traj_printed++; }
 
     void write_native(std::ostream &o, const std::string &obs, const MultiField<double> &field, double factor,
                       int species, long long t) {

@@ -69,7 +69,8 @@ class HIPIntegrator : public Integrator<dims> {
 public:
     HIPIntegrator(SimulationState<dims> &s, FreeEnergyModel *model, const chtoml::Table &config, int integrator_id)
         : Integrator<dims>(s, model, config) {
-        if(model->N_species() != 1) this->critical("The HIP integrators support single-species models only");
+        if(model->N_species() != 1 && !model->multi_species())
+            this->critical("The HIP integrators support single-species models only");
         _desc = ch_desc{};
         _desc.dims = dims;
         _desc.N = this->_N_per_dim;
@@ -81,6 +82,12 @@ public:
         _desc.dx = this->_dx;
         _desc.k = this->_k_laplacian[0];
         model->describe(_desc);
+        if(model->multi_species()) {
+            _multi = true;
+            _species = ch_species_model{};
+            model->describe_species(_species);
+            for(int s = 0; s < model->N_species(); s++) _species.k[s] = this->_k_laplacian[s];
+        }
         _device = this->template _config_optional_value<int>(config, "device", 0);
     }
     ~HIPIntegrator() override {
@@ -91,7 +98,8 @@ public:
     void setup(const IMobility<dims> &mobility) {
         mobility.describe(_desc);
         _check(ch_ctx_create(_device, &_ctx), "ch_ctx_create");
-        _check(ch_sys_create(_ctx, &_desc, &_sys), "ch_sys_create");
+        if(_multi) _check(ch_sys_create_species(_ctx, &_desc, &_species, &_sys), "ch_sys_create_species");
+        else _check(ch_sys_create(_ctx, &_desc, &_sys), "ch_sys_create");
         _check(ch_sys_upload(_sys, this->_rho.data()), "ch_sys_upload");
         _output_ready = true;
     }
@@ -119,6 +127,8 @@ public:
 
 protected:
     ch_desc _desc{};
+    ch_species_model _species{};
+    bool _multi = false;
     int _device = 0;
     ch_ctx *_ctx = nullptr;
     ch_sys *_sys = nullptr;

@@ -39,8 +39,9 @@ public:
         std::string model_name = _config_value<std::string>(config, "free_energy");
         if(model_name == "landau") _state.model = std::make_unique<Landau>(config);
         else if(model_name == "simple_wertheim") _state.model = std::make_unique<SimpleWertheim>(config);
-        else if(model_name == "saleh" || model_name == "generic_wertheim" || model_name == "ricci")
-            critical("Free energy model '{}' is not part of this build", model_name);
+        else if(model_name == "saleh") _state.model = std::make_unique<SalehWertheim>(config);
+        else if(model_name == "generic_wertheim") _state.model = std::make_unique<GenericWertheim>(config);
+        else if(model_name == "ricci") _state.model = std::make_unique<RicciWertheim>(config);
         else critical("Unsupported free energy model '{}'", model_name);
 
         _system = std::make_unique<CahnHilliard<DIM>>(_state, _state.model.get(), config);
@@ -71,7 +72,9 @@ public:
             if(_printer->should_print_last(t)) _print_current_state("last_", t);
             if(_printer->should_print_traj(t)) {
                 _state.integrator->sync();
-                _printer->add_to_trajectory("traj", "density", _state.rho, rho_factor, t);
+                for(int i = 0; i < _state.model->N_species(); i++)
+                    _printer->add_to_trajectory("traj", "density", _state.rho, rho_factor, i, t);
+                _printer->trajectory_frame_done();
             }
             if(_print_mass_every > 0 && t % _print_mass_every == 0) {
                 double F = _system->average_free_energy(), m = _system->average_mass();

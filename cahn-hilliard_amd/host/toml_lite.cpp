@@ -233,23 +233,45 @@ bool parse(const std::string &text, Table &out, std::string &err) {
         if(c.eof()) break;
         if(c.peek() == '[') {
             c.p++;
+            // [[a.b]]: append a new table to the array a.b and fill that one
+            const bool array_of_tables = c.peek() == '[';
+            if(array_of_tables) c.p++;
             std::vector<std::string> parts;
             if(!parse_key(c, parts)) break;
-            if(c.peek() != ']') {
-                c.fail("expected ']'");
+            if(c.peek() != ']' || (array_of_tables && (c.p + 1 >= c.t.size() || c.t[c.p + 1] != ']'))) {
+                c.fail(array_of_tables ? "expected ']]'" : "expected ']'");
                 break;
             }
-            c.p++;
+            c.p += array_of_tables ? 2 : 1;
             current = &out;
             bool ok = true;
-            for(auto &k : parts) {
-                Node &n = (*current)[k];
+            for(size_t i = 0; i < parts.size(); i++) {
+                Node &n = (*current)[parts[i]];
+                const bool last = i + 1 == parts.size();
+                if(array_of_tables && last) {
+                    if(n.kind == Node::NONE) n.kind = Node::ARRAY;
+                    if(n.kind != Node::ARRAY) {
+                        ok = c.fail("'" + parts[i] + "' is not an array of tables");
+                        break;
+                    }
+                    Node t;
+                    t.kind = Node::TABLE;
+                    t.tab = std::make_shared<Table>();
+                    n.arr.push_back(t);
+                    current = n.arr.back().tab.get();
+                    break;
+                }
                 if(n.kind == Node::NONE) {
                     n.kind = Node::TABLE;
                     n.tab = std::make_shared<Table>();
                 }
+                // a dotted header through an array of tables continues in its last element
+                if(n.kind == Node::ARRAY && !n.arr.empty() && n.arr.back().kind == Node::TABLE) {
+                    current = n.arr.back().tab.get();
+                    continue;
+                }
                 if(n.kind != Node::TABLE) {
-                    ok = c.fail("'" + k + "' is not a table");
+                    ok = c.fail("'" + parts[i] + "' is not a table");
                     break;
                 }
                 current = n.tab.get();

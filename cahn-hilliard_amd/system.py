@@ -105,6 +105,48 @@ def delta_value(delta):
     return delta
 
 
+def _fill_k(m, k):
+    for i, v in enumerate(np.broadcast_to(np.asarray(k, dtype=float), (m.n_species,))):
+        m.k[i] = v
+
+
+def saleh_model(k, valence=(4, 4, 2), B2=1.0, B3=0.0, delta_AA=1e3, delta_BB=1e3):
+    """ChSpeciesModel of SalehWertheim (internal units; k per species or one k)."""
+    m = L.ChSpeciesModel()
+    m.model, m.n_species = L.SALEH_WERTHEIM, 3
+    _fill_k(m, k)
+    for i, v in enumerate(valence):
+        m.saleh_valence[i] = int(v)
+    m.saleh_B2, m.saleh_B3, m.saleh_delta_AA, m.saleh_delta_BB = B2, B3, delta_AA, delta_BB
+    return m
+
+
+def ricci_model(k, B2=1.0, delta_00=1e3, delta_12=1e3):
+    """ChSpeciesModel of RicciWertheim (internal units)."""
+    m = L.ChSpeciesModel()
+    m.model, m.n_species = L.RICCI_WERTHEIM, 2
+    _fill_k(m, k)
+    m.ricci_B2, m.ricci_delta_00, m.ricci_delta_12 = B2, delta_00, delta_12
+    return m
+
+
+def generic_model(k, species_patches, deltas, B2s):
+    """ChSpeciesModel of GenericWertheim: species_patches lists the patch types
+    of each species, deltas maps (p, q) and B2s maps (s, t) to values (both symmetric)."""
+    m = L.ChSpeciesModel()
+    m.model, m.n_species = L.GENERIC_WERTHEIM, len(species_patches)
+    _fill_k(m, k)
+    m.n_patches = max(max(pl) for pl in species_patches) + 1
+    for s_, pl in enumerate(species_patches):
+        for q in pl:
+            m.patch_count[s_][q] += 1
+    for (p_, q), d in deltas.items():
+        m.delta[p_][q] = m.delta[q][p_] = d
+    for (a, b_), v in B2s.items():
+        m.B2[a][b_] = m.B2[b_][a] = v
+    return m
+
+
 def species_model_from_config(cfg):
     """ChSpeciesModel of a multi-species free energy (saleh, ricci,
     generic_wertheim), or None for the single-species models.  Keys and unit
