@@ -60,18 +60,35 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
         return WHOLE || (lx >= H && lx < RW - H && gx0 + lx < N);
     };
 
-    T lam[C][RPW], psi[C][RPW], nxt[C][RPW], wv[C][RPW];
+    // psi_n of the next step (and, with <= 4 cells per lane, of the one after)
+    // is in flight while a step computes
+    constexpr bool AHEAD2 = C * RPW <= 4;
+    T lam[C][RPW], psi[C][RPW], nxt[C][RPW], nxt2[AHEAD2 ? C : 1][AHEAD2 ? RPW : 1], wv[C][RPW];
     load(lam_in, lam);
     load(tape + (size_t)s_top * NN, nxt);
+    if constexpr(AHEAD2)
+        if(KA > 1) load(tape + (size_t)(s_top - 1) * NN, nxt2);
     const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
     double ge = 0, gk = 0, gm = 0, gd = 0;
 #pragma unroll 1
     for(int st = 0; st < KA; st++) {
+        if constexpr(AHEAD2) {
 #pragma unroll
-        for(int j = 0; j < RPW; j++)
+            for(int j = 0; j < RPW; j++)
 #pragma unroll
-            for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
-        if(st + 1 < KA) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
+                for(int c = 0; c < C; c++) {
+                    psi[c][j] = nxt[c][j];
+                    nxt[c][j] = nxt2[c][j];
+                }
+            if(st + 2 < KA) load(tape + (size_t)(s_top - st - 2) * NN, nxt2);
+        }
+        else {
+#pragma unroll
+            for(int j = 0; j < RPW; j++)
+#pragma unroll
+                for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
+            if(st + 1 < KA) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
+        }
 
         // ---- phase 1: w = L lambda
 #pragma unroll
