@@ -320,10 +320,9 @@ static hipError_t adj_tiles(const Params &p, int KA, const void *li, void *lo, c
     }
 }
 
-template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ>
+template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ, int NT = 256>
 static hipError_t launch_adj_em(const Params &p, const void *lam_in, void *lam_out, const void *psi, double *gacc,
                                 hipStream_t s) {
-    constexpr int NT = 256;
     const int tiles_x = p.N / TX, tiles_y = DIMS >= 2 ? p.N / TY : 1, tiles_z = DIMS == 3 ? p.N / TZ : 1;
     hipLaunchKernelGGL((k_adjoint_em<T, DIMS, MODEL, MOB, TX, TY, TZ, NT>), dim3(tiles_x * tiles_y * tiles_z), dim3(NT),
                        0, s, (const T *)lam_in, (T *)lam_out, (const T *)psi, p, tiles_x, tiles_y, gacc);
@@ -346,7 +345,13 @@ static hipError_t adj_em_tiles(const Params &p, const void *li, void *lo, const 
         return launch_adj_em<T, 2, MODEL, MOB, 4, 4, 1>(p, li, lo, psi, gacc, s);
     }
     else {
-        if(N >= 16) return launch_adj_em<T, 3, MODEL, MOB, 16, 4, 4>(p, li, lo, psi, gacc, s);
+        // Params::adj_tile3d: 16x4x4 / 256 threads, 16x8x4 / 512, 16x8x8 / 512 (gradient slots are
+        // sized for the first, the tile count of the others is smaller)
+        if(N >= 16) {
+            if(p.adj_tile3d == 1) return launch_adj_em<T, 3, MODEL, MOB, 16, 8, 4, 512>(p, li, lo, psi, gacc, s);
+            if(p.adj_tile3d == 2) return launch_adj_em<T, 3, MODEL, MOB, 16, 8, 8, 512>(p, li, lo, psi, gacc, s);
+            return launch_adj_em<T, 3, MODEL, MOB, 16, 4, 4>(p, li, lo, psi, gacc, s);
+        }
         return launch_adj_em<T, 3, MODEL, MOB, 4, 4, 4>(p, li, lo, psi, gacc, s);
     }
 }
@@ -500,9 +505,8 @@ hipError_t adjoint_launch(const Params &p, int precision, int model, int mob, bo
 }
 
 // recompute of a checkpoint window: K steps from `in`, all K states to tape_out
-template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K>
+template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K, int NT = 256>
 static hipError_t launch_tape(const Params &p, const void *in, void *tape_out, hipStream_t s) {
-    constexpr int NT = 256;
     const int tiles_x = p.N / TX, tiles_y = DIMS >= 2 ? p.N / TY : 1, tiles_z = DIMS == 3 ? p.N / TZ : 1;
     hipLaunchKernelGGL((k_stencil<T, DIMS, MODEL, MOB, EM, TX, TY, TZ, K, NT, true>), dim3(tiles_x * tiles_y * tiles_z),
                        dim3(NT), 0, s, (const T *)in, (T *)tape_out, p, tiles_x, tiles_y);
@@ -545,7 +549,12 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
         return launch_tape<T, 2, MODEL, MOB, EM, 4, 4, 1, 1>(p, in, out, s);
     }
     else {
-        if(N >= 16) return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1>(p, in, out, s);
+        if(N >= 16) {
+            // the forward's 3D tile (Params::tile3d)
+            if(p.tile3d == 1) return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1, 512>(p, in, out, s);
+            if(p.tile3d == 2) return launch_tape<T, 3, MODEL, MOB, EM, 16, 16, 4, 1, 512>(p, in, out, s);
+            return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1>(p, in, out, s);
+        }
         return launch_tape<T, 3, MODEL, MOB, EM, 4, 4, 4, 1>(p, in, out, s);
     }
 }

@@ -215,6 +215,10 @@ static chd::Params make_params(const ch_desc *d) {
     }
     p.wave_cfg = chd::wave_default_cfg(d->N);
     p.adj_cfg = chd::wave_default_cfg(d->N);
+    // measured on C5 (SimpleWertheim 3D N=128 euler_mobility): 16x8x8 with 512
+    // threads and the 16x8x4 adjoint tile beat the other shapes
+    p.tile3d = 1;
+    p.adj_tile3d = 1;
     p.S = d->S;
     double bins = std::pow((double)d->N, d->dims);
     p.vbin_over_bins = d->dx * d->dx * d->dx / bins;
@@ -331,6 +335,8 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     s->tb = 1;
     while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
     if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
+    if(const char *r = std::getenv("CH_TILE3D")) s->p.tile3d = std::max(0, std::min(2, std::atoi(r)));
+    if(const char *r = std::getenv("CH_ADJ_TILE3D")) s->p.adj_tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_ADJ_CFG")) {
         const int v = std::atoi(r);
         if(v == 0 || v == 3 || v == 4 || v == 5) s->p.adj_cfg = v;

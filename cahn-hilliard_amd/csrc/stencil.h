@@ -134,9 +134,8 @@ __global__ __launch_bounds__(NT) void k_stencil(const T *__restrict__ in, T *__r
     store_tile<T, DIMS, TX, TY, TZ, RX, RXY, H>(A, dst, N, tx, ty, tz, NT);
 }
 
-template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K>
+template <typename T, int DIMS, int MODEL, int MOB, bool EM, int TX, int TY, int TZ, int K, int NT = 256>
 hipError_t launch_stencil_cfg(const Params &p, const void *in, void *out, hipStream_t s) {
-    constexpr int NT = 256;
     const int tiles_x = p.N / TX;
     const int tiles_y = DIMS >= 2 ? p.N / TY : 1;
     const int tiles_z = DIMS == 3 ? p.N / TZ : 1;

@@ -8,6 +8,7 @@ namespace chd {
 
 // Tile shapes (TX, TY, TZ) per dimension; the largest that fits N is used.
 //   1D: 256 | 64 | 16 | 4      2D: 64x16 | 16x16 | 4x4      3D: 16x8x8 | 4x4x4
+// (3D, N >= 16: Params::tile3d selects 16x8x8 with 256 or 512 threads, or 16x16x4 with 512)
 template <typename T, int DIMS, int MODEL, int MOB, bool EM, int K>
 hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t s) {
     const int N = p.N;
@@ -23,7 +24,11 @@ hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t
         return launch_stencil_cfg<T, 2, MODEL, MOB, EM, 4, 4, 1, K>(p, in, out, s);
     }
     else {
-        if(N >= 16) return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 16, 8, 8, K>(p, in, out, s);
+        if(N >= 16) {
+            if(p.tile3d == 1) return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 16, 8, 8, K, 512>(p, in, out, s);
+            if(p.tile3d == 2) return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 16, 16, 4, K, 512>(p, in, out, s);
+            return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 16, 8, 8, K>(p, in, out, s);
+        }
         return launch_stencil_cfg<T, 3, MODEL, MOB, EM, 4, 4, 4, K>(p, in, out, s);
     }
 }
