@@ -10,7 +10,11 @@
 // neighbours through DPP wave shifts, y neighbours from registers and, for the
 // first / last row of a wave, through LDS.  KA backward steps run per launch on
 // a region carrying a 2 KA halo of lambda; psi_n comes from the tape, the next
-// step's rows fetched while the current step computes.
+// step's rows fetched while the current step computes.  As in the forward
+// kernel, a phase computes only the rows later phases read (lambda after
+// backward step st within 2(KA-1-st) rows of the interior, w of step st within
+// 2(KA-st)-1): a wave whose rows all lie further out skips the phase and the
+// tape rows it would have read.
 #pragma once
 
 #include "adjoint_math.h"
@@ -29,7 +33,7 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
     __shared__ WaveExchange<T, 2, NW, C> ex;  // lambda, w
     __shared__ double red[kGradSlots][NW];
 
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int N = p.N, mask = p.mask;
     int tile = 0, gx0 = 0, gy0 = 0;
     if constexpr(!WHOLE) {
@@ -37,9 +41,10 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
         gx0 = (tile % tiles_x) * W - H;
         gy0 = (tile / tiles_x) * TH - H;
     }
-    const int wa = WHOLE ? (w + NW - 1) % NW : w - 1;
-    const int wb = WHOLE ? (w + 1) % NW : w + 1;
-    const bool has_a = WHOLE || w > 0, has_b = WHOLE || w < NW - 1;
+    // waves above and below, wrapping: in halo mode the first and last waves'
+    // outer rows are halo rows whose results are discarded, so what they read
+    // there does not matter and the exchange needs no branch
+    const int wa = (w + NW - 1) % NW, wb = (w + 1) % NW;
     const size_t NN = (size_t)N * N;
 
     auto load = [&](const T *__restrict__ f, T(&dst)[C][RPW]) {
@@ -60,14 +65,30 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
         return WHOLE || (lx >= H && lx < RW - H && gx0 + lx < N);
     };
 
+    // wave w computes a phase whose results are read up to d rows beyond the interior
+    auto needed = [&](int d) {
+        if constexpr(WHOLE) return true;
+        else return w * RPW + RPW > H - d && w * RPW < RH - H + d;
+    };
+    // backward step st reads psi where its lambda update is needed
+    auto reads_tape = [&](int st) { return needed(2 * (KA - 1 - st)); };
+
     // psi_n of the next step (and, with <= 4 cells per lane, of the one after)
     // is in flight while a step computes
     constexpr bool AHEAD2 = C * RPW <= 4;
     T lam[C][RPW], psi[C][RPW], nxt[C][RPW], nxt2[AHEAD2 ? C : 1][AHEAD2 ? RPW : 1], wv[C][RPW];
     load(lam_in, lam);
-    load(tape + (size_t)s_top * NN, nxt);
+#pragma unroll
+    for(int j = 0; j < RPW; j++)
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+            wv[c][j] = 0;
+            nxt[c][j] = 0;
+            if constexpr(AHEAD2) nxt2[c][j] = 0;
+        }
+    if(reads_tape(0)) load(tape + (size_t)s_top * NN, nxt);
     if constexpr(AHEAD2)
-        if(KA > 1) load(tape + (size_t)(s_top - 1) * NN, nxt2);
+        if(KA > 1 && reads_tape(1)) load(tape + (size_t)(s_top - 1) * NN, nxt2);
     const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
     double ge = 0, gk = 0, gm = 0, gd = 0;
 #pragma unroll 1
@@ -80,14 +101,14 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
                     psi[c][j] = nxt[c][j];
                     nxt[c][j] = nxt2[c][j];
                 }
-            if(st + 2 < KA) load(tape + (size_t)(s_top - st - 2) * NN, nxt2);
+            if(st + 2 < KA && reads_tape(st + 2)) load(tape + (size_t)(s_top - st - 2) * NN, nxt2);
         }
         else {
 #pragma unroll
             for(int j = 0; j < RPW; j++)
 #pragma unroll
                 for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
-            if(st + 1 < KA) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
+            if(st + 1 < KA && reads_tape(st + 1)) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
         }
 
         // ---- phase 1: w = L lambda
@@ -97,18 +118,20 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
             ex.last[0][w][c][lane] = lam[c][RPW - 1];
         }
         __syncthreads();
-        T la[C], lb[C];
+        if(needed(2 * (KA - st) - 1)) {
+            T la[C], lb[C];
 #pragma unroll
-        for(int c = 0; c < C; c++) {
-            la[c] = has_a ? ex.last[0][wa][c][lane] : lam[c][0];
-            lb[c] = has_b ? ex.first[0][wb][c][lane] : lam[c][RPW - 1];
-        }
+            for(int c = 0; c < C; c++) {
+                la[c] = ex.last[0][wa][c][lane];
+                lb[c] = ex.first[0][wb][c][lane];
+            }
 #pragma unroll
-        for(int j = 0; j < RPW; j++) {
-            T m[C][2], q[C][2];
-            row_neighbours<T, C, RPW, WHOLE>(lam, j, la, lb, m, q);
+            for(int j = 0; j < RPW; j++) {
+                T m[C][2], q[C][2];
+                row_neighbours<T, C, RPW, WHOLE>(lam, j, la, lb, m, q);
 #pragma unroll
-            for(int c = 0; c < C; c++) wv[c][j] = laplacian<T, 2>(lam[c][j], m[c], q[c], inv_dx2);
+                for(int c = 0; c < C; c++) wv[c][j] = laplacian<T, 2>(lam[c][j], m[c], q[c], inv_dx2);
+            }
         }
         // ---- phase 2: lambda += dt M (f''(psi) w - 2k L w), gradient terms
 #pragma unroll
@@ -117,29 +140,31 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
             ex.last[1][w][c][lane] = wv[c][RPW - 1];
         }
         __syncthreads();
-        T wa_[C], wb_[C];
-#pragma unroll
-        for(int c = 0; c < C; c++) {
-            wa_[c] = has_a ? ex.last[1][wa][c][lane] : wv[c][0];
-            wb_[c] = has_b ? ex.first[1][wb][c][lane] : wv[c][RPW - 1];
-        }
-#pragma unroll
-        for(int j = 0; j < RPW; j++) {
-            T m[C][2], q[C][2];
-            row_neighbours<T, C, RPW, WHOLE>(wv, j, wa_, wb_, m, q);
-            const bool rin = row_in(j);
+        if(needed(2 * (KA - 1 - st))) {
+            T wa_[C], wb_[C];
 #pragma unroll
             for(int c = 0; c < C; c++) {
-                const T lw = laplacian<T, 2>(wv[c][j], m[c], q[c], inv_dx2);
-                const T fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
-                if(rin && col_in(c)) {
-                    const double wd = (double)wv[c][j], ps = (double)psi[c][j], lwd = (double)lw;
-                    ge += wd * ps;
-                    gk += lwd * ps;
-                    gm += wd * (double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * (lwd * ps);
-                    if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
+                wa_[c] = ex.last[1][wa][c][lane];
+                wb_[c] = ex.first[1][wb][c][lane];
+            }
+#pragma unroll
+            for(int j = 0; j < RPW; j++) {
+                T m[C][2], q[C][2];
+                row_neighbours<T, C, RPW, WHOLE>(wv, j, wa_, wb_, m, q);
+                const bool rin = row_in(j);
+#pragma unroll
+                for(int c = 0; c < C; c++) {
+                    const T lw = laplacian<T, 2>(wv[c][j], m[c], q[c], inv_dx2);
+                    const T fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
+                    if(rin && col_in(c)) {
+                        const double wd = (double)wv[c][j], ps = (double)psi[c][j], lwd = (double)lw;
+                        ge += wd * ps;
+                        gk += lwd * ps;
+                        gm += wd * (double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * (lwd * ps);
+                        if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
+                    }
+                    lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
                 }
-                lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
             }
         }
     }

@@ -12,7 +12,12 @@
 // Halo mode: K steps run per launch on a region 64C x (NW RPW) that carries a
 // 2K halo on every side; each phase invalidates one more ring, so after K
 // steps the (64C - 4K) x (NW RPW - 4K) interior is exact and is stored.  Tiles
-// that do not divide N are clipped at the store.
+// that do not divide N are clipped at the store.  A phase computes only the
+// rows that later phases read: psi after step s is read within 2(K-1-s) rows
+// of the interior and mu of step s within 2(K-s)-1, so a wave whose rows all
+// lie further out skips the phase.  Every row such a band reads lies in the
+// band of the phase before, so a skipped wave's stale rows reach only rows
+// that are themselves discarded.
 // Whole mode (N == 64 C == NW RPW): the region is the entire periodic grid; x
 // neighbours use DPP wave rotates and y neighbours wrap through LDS, so no
 // halo is recomputed and every cell is stored.
@@ -92,19 +97,29 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
     constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
     constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
     constexpr int H = WHOLE ? 0 : 2 * K;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int N = p.N, mask = p.mask;
-    const int wa = WHOLE ? (w + NW - 1) % NW : w - 1;  // wave above
-    const int wb = WHOLE ? (w + 1) % NW : w + 1;       // wave below
-    const bool has_a = WHOLE || w > 0, has_b = WHOLE || w < NW - 1;
+    // waves above and below, wrapping: in halo mode the first and last waves'
+    // outer rows are halo rows whose results are discarded, so what they read
+    // there does not matter and the exchange needs no branch
+    const int wa = (w + NW - 1) % NW, wb = (w + 1) % NW;
 
     T psi[C][RPW], mu[C][RPW], mob[HASMOB ? C : 1][HASMOB ? RPW : 1];
 #pragma unroll
     for(int j = 0; j < RPW; j++) {
         const size_t row = (size_t)((gy0 + w * RPW + j) & mask) * N;
 #pragma unroll
-        for(int c = 0; c < C; c++) psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
+        for(int c = 0; c < C; c++) {
+            psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
+            mu[c][j] = 0;
+            if constexpr(HASMOB) mob[c][j] = 0;
+        }
     }
+    // wave w computes a phase whose results are read up to d rows beyond the interior
+    auto needed = [&](int d) {
+        if constexpr(WHOLE) return true;
+        else return w * RPW + RPW > H - d && w * RPW < RH - H + d;
+    };
 
     // the tile interior (the whole grid in WHOLE mode) of psi into dst
     auto store_interior = [&](T *__restrict__ dst) {
@@ -132,20 +147,22 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
             ex.last[0][w][c][lane] = psi[c][RPW - 1];
         }
         __syncthreads();
-        T above[C], below[C];
-#pragma unroll
-        for(int c = 0; c < C; c++) {
-            above[c] = has_a ? ex.last[0][wa][c][lane] : psi[c][0];
-            below[c] = has_b ? ex.first[0][wb][c][lane] : psi[c][RPW - 1];
-        }
-#pragma unroll
-        for(int j = 0; j < RPW; j++) {
-            T m[C][2], q[C][2];
-            row_neighbours<T, C, RPW, WHOLE>(psi, j, above, below, m, q);
+        if(needed(2 * (K - s) - 1)) {
+            T above[C], below[C];
 #pragma unroll
             for(int c = 0; c < C; c++) {
-                mu[c][j] = chem_pot<T, 2, MODEL>(p, psi[c][j], m[c], q[c]);
-                if constexpr(HASMOB) mob[c][j] = mobility<T, MOB>(p, psi[c][j]);
+                above[c] = ex.last[0][wa][c][lane];
+                below[c] = ex.first[0][wb][c][lane];
+            }
+#pragma unroll
+            for(int j = 0; j < RPW; j++) {
+                T m[C][2], q[C][2];
+                row_neighbours<T, C, RPW, WHOLE>(psi, j, above, below, m, q);
+#pragma unroll
+                for(int c = 0; c < C; c++) {
+                    mu[c][j] = chem_pot<T, 2, MODEL>(p, psi[c][j], m[c], q[c]);
+                    if constexpr(HASMOB) mob[c][j] = mobility<T, MOB>(p, psi[c][j]);
+                }
             }
         }
         // ---- phase 2: psi update from mu (and M)
@@ -159,36 +176,38 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
             }
         }
         __syncthreads();
-        T mu_a[C], mu_b[C], M_a[C], M_b[C];
+        if(needed(2 * (K - 1 - s))) {
+            T mu_a[C], mu_b[C], M_a[C], M_b[C];
 #pragma unroll
-        for(int c = 0; c < C; c++) {
-            mu_a[c] = has_a ? ex.last[1][wa][c][lane] : mu[c][0];
-            mu_b[c] = has_b ? ex.first[1][wb][c][lane] : mu[c][RPW - 1];
-            if constexpr(HASMOB) {
-                M_a[c] = has_a ? ex.last[2][wa][c][lane] : mob[c][0];
-                M_b[c] = has_b ? ex.first[2][wb][c][lane] : mob[c][RPW - 1];
+            for(int c = 0; c < C; c++) {
+                mu_a[c] = ex.last[1][wa][c][lane];
+                mu_b[c] = ex.first[1][wb][c][lane];
+                if constexpr(HASMOB) {
+                    M_a[c] = ex.last[2][wa][c][lane];
+                    M_b[c] = ex.first[2][wb][c][lane];
+                }
             }
-        }
 #pragma unroll
-        for(int j = 0; j < RPW; j++) {
-            T m[C][2], q[C][2];
-            row_neighbours<T, C, RPW, WHOLE>(mu, j, mu_a, mu_b, m, q);
-            if constexpr(HASMOB) {
-                T Mm[C][2], Mq[C][2];
-                row_neighbours<T, C, RPW, WHOLE>(mob, j, M_a, M_b, Mm, Mq);
+            for(int j = 0; j < RPW; j++) {
+                T m[C][2], q[C][2];
+                row_neighbours<T, C, RPW, WHOLE>(mu, j, mu_a, mu_b, m, q);
+                if constexpr(HASMOB) {
+                    T Mm[C][2], Mq[C][2];
+                    row_neighbours<T, C, RPW, WHOLE>(mob, j, M_a, M_b, Mm, Mq);
 #pragma unroll
-                for(int c = 0; c < C; c++)
-                    psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], mob[c][j], Mm[c], Mq[c]);
-            }
-            else {
+                    for(int c = 0; c < C; c++)
+                        psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], mob[c][j], Mm[c], Mq[c]);
+                }
+                else {
 #pragma unroll
-                for(int c = 0; c < C; c++) {
-                    if constexpr(!EM) {
-                        psi[c][j] = euler_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c]);
-                    }
-                    else {
-                        const T Mm[2] = {Mc, Mc}, Mq[2] = {Mc, Mc};
-                        psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], Mc, Mm, Mq);
+                    for(int c = 0; c < C; c++) {
+                        if constexpr(!EM) {
+                            psi[c][j] = euler_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c]);
+                        }
+                        else {
+                            const T Mm[2] = {Mc, Mc}, Mq[2] = {Mc, Mc};
+                            psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], Mc, Mm, Mq);
+                        }
                     }
                 }
             }
