@@ -69,6 +69,12 @@ struct ch_sys {
     double *sp_X = nullptr;
     void *sp_mu = nullptr, *sp_mob = nullptr, *sp_rho_hat = nullptr, *sp_spec[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // stencil integrators: the launches of the last remainder of nsteps % G
+    // steps (per starting buffer), captured when the same remainder recurs
+    hipGraphExec_t rexec[2] = {nullptr, nullptr};
+    int64_t rsteps[2] = {0, 0};
+    int rcur[2] = {0, 0};       // the buffer holding the field after the remainder
+    int64_t rseen[2] = {0, 0};  // the last remainder run by direct launches
 };
 
 static ch_status fail(ch_ctx *ctx, ch_status st, const char *fmt, ...) {
@@ -287,6 +293,7 @@ static void free_sys(ch_sys *s) {
         if(e) (void)hipEventDestroy(e);
     for(int i = 0; i < 2; i++) {
         if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
+        if(s->rexec[i]) (void)hipGraphExecDestroy(s->rexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
     void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
@@ -804,13 +811,49 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
         for(int64_t r = 0; r < reps; r++) HIPCHK(c, hipGraphLaunch(s->gexec[par], c->stream));
         nsteps -= reps * G;  // G / tb launches, an even number: parity unchanged
     }
-    for(; nsteps >= s->tb; nsteps -= s->tb) HIPCHK(c, launch_steps(s, s->cur, s->tb));
-    // the remainder (< tb) in launches of decreasing depth: at most log2(tb)
-    for(int K = s->tb / 2; K >= 1; K /= 2)
-        if(nsteps >= K) {
-            HIPCHK(c, launch_steps(s, s->cur, K));
-            nsteps -= K;
+    // the rest in launches of tb steps, then of decreasing depth: at most log2(tb)
+    auto rest = [&](int &cur, int64_t n) -> hipError_t {
+        hipError_t e = hipSuccess;
+        for(; n >= s->tb && e == hipSuccess; n -= s->tb) e = launch_steps(s, cur, s->tb);
+        for(int K = s->tb / 2; K >= 1 && e == hipSuccess; K /= 2)
+            if(n >= K) {
+                e = launch_steps(s, cur, K);
+                n -= K;
+            }
+        return e;
+    };
+    if(G == 0 || nsteps < s->tb) {
+        HIPCHK(c, rest(s->cur, nsteps));
+        return CH_OK;
+    }
+    // a remainder of at least one full launch that recurs from the same buffer:
+    // one graph, kept until another remainder recurs
+    const int par = s->cur;
+    if(!s->rexec[par] || s->rsteps[par] != nsteps) {
+        if(s->rseen[par] != nsteps) {
+            s->rseen[par] = nsteps;
+            HIPCHK(c, rest(s->cur, nsteps));
+            return CH_OK;
         }
+        if(s->rexec[par]) {
+            HIPCHK(c, hipGraphExecDestroy(s->rexec[par]));
+            s->rexec[par] = nullptr;
+        }
+        hipGraph_t graph;
+        int cur = par;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = rest(cur, nsteps);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+        if(e != hipSuccess) return fail(c, CH_EHIP, "step launch during capture failed: %s", hipGetErrorString(e));
+        if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+        e = hipGraphInstantiate(&s->rexec[par], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        s->rsteps[par] = nsteps;
+        s->rcur[par] = cur;
+    }
+    HIPCHK(c, hipGraphLaunch(s->rexec[par], c->stream));
+    s->cur = s->rcur[par];
     return CH_OK;
 }
 
