@@ -111,11 +111,12 @@ def w_euler(ch, ctx, args, precision="fp64"):
     steps = args.steps if precision == "fp64" else max(10, args.steps // 4)
     wall = timed_run(ctx, s, steps, args.warmup, args.inner)
     spl = s.steps_per_launch
+    shape = s.region_shape
     t_launch = kernel_time_per_launch(ctx, s, max(1024, 10 * args.inner), spl)
     mass, fe = s.observables()
     s.close()
     return dict(wall=wall, steps=steps, gcell=N * N * args.inner * steps / wall / 1e9, spl=spl, t_launch=t_launch,
-                mass=mass, fe=fe)
+                mass=mass, fe=fe, shape=shape)
 
 
 def w_pseudospectral(ch, ctx, args, precision="fp64"):
@@ -466,7 +467,7 @@ def main():
     achieved = 16.0 * N * N * r["spl"] / r["t_launch"] / 1e9
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs of this
     # kernel configuration (profiles/hbm_traffic.json), null for any other
-    cfg = int(os.environ.get("CH_WAVE_CFG", 5 if N >= 512 else 4))
+    cfg = r["shape"]
     traffic = None
     try:
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")) as f:

@@ -112,6 +112,7 @@ def lib():
         "ch_sys_download_mobility": (ctypes.c_int, [P, P]),
         "ch_sys_evolve": (ctypes.c_int, [P, I64]),
         "ch_sys_steps_per_launch": (ctypes.c_int32, [P]),
+        "ch_sys_region_shape": (ctypes.c_int32, [P]),
         "ch_sys_gmres_iterations": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
         "ch_sq_nshells": (I32, [I32]),

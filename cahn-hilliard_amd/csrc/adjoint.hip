@@ -531,7 +531,7 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
     const int N = p.N;
     if constexpr(DIMS == 2) {
         if(N >= 64) {
-            if(wave_default_cfg(N) == 5) return tape_wave<T, MODEL, MOB, EM, WaveCfg5>(p, K, in, out, s);
+            if(adj_default_cfg(N) == 5) return tape_wave<T, MODEL, MOB, EM, WaveCfg5>(p, K, in, out, s);
             return tape_wave<T, MODEL, MOB, EM, WaveCfg4>(p, K, in, out, s);
         }
     }

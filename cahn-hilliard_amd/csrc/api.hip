@@ -61,6 +61,7 @@ struct ch_sys {
     void *psm_vec[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // V, w, b, x, rhs, r, av
     double *psm_dev = nullptr;   // kPsmParts partials, 4 scalars, Hcol, y
     double *psm_host = nullptr;  // pinned, kPsmParts doubles
+    double *obs_host = nullptr;  // pinned: mass and free energy, written by the finalize kernel
     int gmres_iters = 0;
     // multi-species models: host and device model, X (Generic), mu, mobility,
     // per-species spectra and two scratch spectra
@@ -220,7 +221,7 @@ static chd::Params make_params(const ch_desc *d) {
     p.dcf = d->density_conversion > 0.0 ? d->density_conversion : 1.0;
     }
     p.wave_cfg = chd::wave_default_cfg(d->N);
-    p.adj_cfg = chd::wave_default_cfg(d->N);
+    p.adj_cfg = chd::adj_default_cfg(d->N);
     // measured on C5 (SimpleWertheim 3D N=128 euler_mobility): 16x8x8 with 512
     // threads and the 16x8x4 adjoint tile beat the other shapes
     p.tile3d = 1;
@@ -312,6 +313,7 @@ static void free_sys(ch_sys *s) {
         if(b) (void)hipFree(b);
     delete s->spm_host;
     if(s->psm_host) (void)hipHostFree(s->psm_host);
+    if(s->obs_host) (void)hipHostFree(s->obs_host);
     delete s;
 }
 
@@ -859,18 +861,25 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
 
 extern "C" int32_t ch_sys_steps_per_launch(const ch_sys *s) { return s ? s->tb : 0; }
 
+extern "C" int32_t ch_sys_region_shape(const ch_sys *s) {
+    if(!s || s->d.dims != 2 || s->d.N < 64 || is_spectral(s->d.integrator) || is_species(s) || has_noise(s)) return -1;
+    return s->p.wave_cfg;
+}
+
 extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_energy) {
     if(!s) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    // the two sums land in pinned host memory straight from the finalize
+    // kernel: no copy behind the reduction, one stream synchronisation
+    if(!s->obs_host) HIPCHK(c, hipHostMalloc((void **)&s->obs_host, 2 * sizeof(double)));
     if(is_species(s))
-        HIPCHK(c, chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, s->result, c->stream));
+        HIPCHK(c, chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, s->obs_host, c->stream));
     else
         HIPCHK(c, chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts,
-                                   s->result, c->stream));
-    double h[2];
-    HIPCHK(c, hipMemcpyAsync(h, s->result, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+                                   s->obs_host, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double h[2] = {s->obs_host[0], s->obs_host[1]};
     const double m = h[0] * s->p.vbin_over_bins, f = h[1] * s->p.vbin_over_bins;
     if(mass) *mass = m;
     if(free_energy) *free_energy = f;
@@ -1025,8 +1034,9 @@ struct ch_batch {
     int64_t total = 0;                       // elements over all replicas
     std::vector<chd::BatchRep> host_reps;
     chd::BatchRep *reps = nullptr;
-    int2 *items[kBatchDepths] = {nullptr, nullptr, nullptr, nullptr};  // work list per K = 1, 2, 4, 8
-    int nitems[kBatchDepths] = {0, 0, 0, 0};
+    // work list per region width (64 or 128 columns) and K = 1, 2, 4, 8
+    int2 *items[2][kBatchDepths] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+    int nitems[2][kBatchDepths] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;
     void *stage = nullptr;                   // largest replica, doubles
@@ -1041,8 +1051,9 @@ static void free_batch(ch_batch *b) {
         if(b->gexec[i]) (void)hipGraphExecDestroy(b->gexec[i]);
         if(b->buf[i]) (void)hipFree(b->buf[i]);
     }
-    for(int i = 0; i < kBatchDepths; i++)
-        if(b->items[i]) (void)hipFree(b->items[i]);
+    for(auto &fam : b->items)
+        for(int2 *it : fam)
+            if(it) (void)hipFree(it);
     void *bufs[] = {b->reps, b->stage, b->result};
     for(void *x : bufs)
         if(x) (void)hipFree(x);
@@ -1078,7 +1089,7 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
         }
         if(esh) b->shape = std::max(0, std::min(chd::kBatchShapes - 1, std::atoi(esh)));
     }
-    std::vector<int2> items[kBatchDepths];
+    std::vector<int2> items[2][kBatchDepths];
     int64_t maxbins = 0;
     for(int r = 0; r < n; r++) {
         chd::BatchRep rep{};
@@ -1086,15 +1097,18 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
         rep.off = b->total;
         const int N = descs[r].N;
         rep.whole = N == chd::kBatchRegion ? 1 : 0;
-        for(int i = 0; i < kBatchDepths; i++) {
-            if(rep.whole) {
-                items[i].push_back(int2{r, 0});
-                continue;
+        for(int fam = 0; fam < 2; fam++)
+            for(int i = 0; i < kBatchDepths; i++) {
+                if(rep.whole) {
+                    items[fam][i].push_back(int2{r, 0});
+                    continue;
+                }
+                // shapes 0 and 3 stand for the 64- and 128-column families
+                const int Tx = chd::batch_tile_x(fam ? 3 : 0, 1 << i), Ty = chd::batch_tile_y(1 << i);
+                const int ntx = (N + Tx - 1) / Tx, nty = (N + Ty - 1) / Ty;
+                for(int ty = 0; ty < nty; ty++)
+                    for(int tx = 0; tx < ntx; tx++) items[fam][i].push_back(int2{r, tx | (ty << 16)});
             }
-            const int T = chd::batch_tile(1 << i), nt = (N + T - 1) / T;
-            for(int ty = 0; ty < nt; ty++)
-                for(int tx = 0; tx < nt; tx++) items[i].push_back(int2{r, tx | (ty << 16)});
-        }
         b->total += (int64_t)N * N;
         maxbins = std::max<int64_t>(maxbins, (int64_t)N * N);
         b->host_reps.push_back(rep);
@@ -1105,11 +1119,13 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
               hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
               hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
-    for(int i = 0; ok && i < kBatchDepths; i++) {
-        b->nitems[i] = (int)items[i].size();
-        ok = hipMalloc((void **)&b->items[i], items[i].size() * sizeof(int2)) == hipSuccess &&
-             hipMemcpy(b->items[i], items[i].data(), items[i].size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
-    }
+    for(int fam = 0; fam < 2; fam++)
+        for(int i = 0; ok && i < kBatchDepths; i++) {
+            const std::vector<int2> &v = items[fam][i];
+            b->nitems[fam][i] = (int)v.size();
+            ok = hipMalloc((void **)&b->items[fam][i], v.size() * sizeof(int2)) == hipSuccess &&
+                 hipMemcpy(b->items[fam][i], v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+        }
     if(ok) ok = hipMemcpy(b->reps, b->host_reps.data(), n * sizeof(chd::BatchRep), hipMemcpyHostToDevice) == hipSuccess;
     if(!ok) {
         free_batch(b);
@@ -1166,9 +1182,9 @@ extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host)
 }
 
 static hipError_t batch_launch(ch_batch *b, int shape, int K, int from) {
-    const int i = log2i(K);
-    return chd::batch_launch(b->precision, b->model, shape, K, b->buf[from], b->buf[from ^ 1], b->reps, b->items[i],
-                             b->nitems[i], b->ctx->stream);
+    const int i = log2i(K), fam = chd::batch_columns_per_lane(shape) - 1;
+    return chd::batch_launch(b->precision, b->model, shape, K, b->buf[from], b->buf[from ^ 1], b->reps,
+                             b->items[fam][i], b->nitems[fam][i], b->ctx->stream);
 }
 
 static hipError_t batch_step(ch_batch *b, int &cur, int K) {

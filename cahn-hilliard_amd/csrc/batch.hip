@@ -6,49 +6,61 @@
 // a work list of (replica, tile) items; one launch advances every item by K
 // steps with the register/DPP wave kernel (stencil_wave.h).  A replica with
 // N = 64 is one item covering the whole periodic grid (no halo); larger
-// replicas are covered by halo tiles.  Every item is a 64 x 64 region, so the
-// blocks of a launch cost the same whatever the replica sizes.  The region is
-// split over the waves in one of kBatchShapes ways; the shape and K are picked
-// by timing the candidates on the batch itself (ch_batch_autotune).
+// replicas are covered by halo tiles.  Halo items are 64 x 64 regions (one
+// column per lane) or, for the wide shapes, 128 x 64 regions (two columns per
+// lane: a larger share of each region is interior, and one DPP move serves two
+// cells); a whole N = 64 replica is one 64 x 64 region split over the same
+// number of waves.  The region is split over the waves in one of kBatchShapes
+// ways; the shape and K are picked by timing the candidates on the batch itself
+// (ch_batch_autotune).
 // Per-replica epsilon, k, dt, M, dx and N travel in a device array of Params.
 #include "ch_internal.h"
 #include "stencil_wave.h"
 
 namespace chd {
 
-// item = (replica, tx | ty << 16): one 64 x 64 region (whole replica for N = 64)
-template <typename T, int MODEL, int K, int RPW, int NW>
+// item = (replica, tx | ty << 16): one halo region of C x 64 columns and 64 rows,
+// or the whole replica for N = 64 (64 x 64, one column per lane)
+template <typename T, int MODEL, int K, int C, int RPW, int NW>
 __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T *__restrict__ bufB,
                                                   const BatchRep *__restrict__ reps, const int2 *__restrict__ items) {
-    static_assert(RPW * NW == kBatchRegion, "batch regions are 64 x 64");
-    constexpr int W = kBatchRegion - 4 * K, TH = kBatchRegion - 4 * K;
-    using CFG = WaveCfg<1, RPW, NW>;
-    __shared__ WaveExchange<T, 2, NW, 1> ex;
+    static_assert(RPW * NW == kBatchRegion, "batch regions are 64 rows high");
+    using HALO = WaveCfg<C, RPW, NW>;
+    using WHOLE = WaveCfg<1, RPW, NW>;
+    constexpr int W = HALO::RW - 4 * K, TH = HALO::RH - 4 * K;
+    // one exchange buffer, seen with the whole region's (narrower) layout too
+    __shared__ WaveExchange<T, 2, NW, C> ex;
     const int2 it = items[blockIdx.x];
     const Params p = reps[it.x].p;
     const int64_t off = reps[it.x].off;
     const T *in = bufA + off;
     T *out = bufB + off;
     if(reps[it.x].whole) {
-        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, CFG, true>(in, out, p, 0, 0, ex);
+        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, WHOLE, true>(
+            in, out, p, 0, 0, *reinterpret_cast<WaveExchange<T, 2, NW, 1> *>(&ex));
     }
     else {
         const int tx = it.y & 0xFFFF, ty = it.y >> 16;
-        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, CFG, false>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K,
-                                                                          ex);
+        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, HALO, false>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K,
+                                                                           ex);
     }
 }
 
-int batch_tile(int K) { return kBatchRegion - 4 * K; }
+int batch_columns_per_lane(int shape) { return shape >= 3 ? 2 : 1; }
+int batch_tile_x(int shape, int K) { return 64 * batch_columns_per_lane(shape) - 4 * K; }
+int batch_tile_y(int K) { return kBatchRegion - 4 * K; }
 
-// shape s: 0 = 16 rows x 4 waves, 1 = 8 x 8, 2 = 4 x 16
+// shape s (rows per wave x waves; halo regions of 64 or 128 columns):
+//   0 = 16 x 4, 1 = 8 x 8, 2 = 4 x 16 (64 columns);  3 = 4 x 16, 4 = 8 x 8 (128 columns)
 template <typename T, int MODEL, int K>
 static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchRep *reps, const int2 *items,
                                  int nitems, hipStream_t s) {
     switch(shape) {
-    case 0: hipLaunchKernelGGL((k_batch<T, MODEL, K, 16, 4>), dim3(nitems), dim3(256), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 1: hipLaunchKernelGGL((k_batch<T, MODEL, K, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 2: hipLaunchKernelGGL((k_batch<T, MODEL, K, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 0: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 16, 4>), dim3(nitems), dim3(256), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 1: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 2: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 3: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 4: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

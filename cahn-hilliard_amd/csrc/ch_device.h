@@ -214,11 +214,15 @@ __device__ __forceinline__ void mode_q2(const Params &p, int ix, int iy, int iz,
 }
 
 // number of 2D wave-kernel region shapes selectable through Params::wave_cfg
-constexpr int kWaveCfgs = 7;
-// default shape: 128 x 64 regions of 16 waves (2 columns per lane) for N >= 512,
-// 64 x 64 regions of 16 waves below, where more, smaller blocks fill the CUs
-// (measured on MI355X over shapes 0-6 at 8 steps per launch)
-__host__ __device__ constexpr int wave_default_cfg(int N) { return N >= 512 ? 5 : 4; }
+constexpr int kWaveCfgs = 9;
+// default forward shape, measured on MI355X over shapes 0-8 at 8 steps per
+// launch (FP64 and FP32): 64 x 64 regions of 16 waves up to N = 512, where more,
+// smaller blocks fill the CUs; 192 x 64 at N = 1024 (224 blocks, one per CU);
+// 128 x 80 from N = 2048
+__host__ __device__ constexpr int wave_default_cfg(int N) { return N <= 512 ? 4 : (N == 1024 ? 8 : 7); }
+// reverse sweep and its forward recompute (shapes 0, 3, 4, 5): 128 x 64 regions
+// from N = 512, 64 x 64 below
+__host__ __device__ constexpr int adj_default_cfg(int N) { return N >= 512 ? 5 : 4; }
 
 // half-spectrum entry q = ((z N) + y) Nh + kx -> (half-axis index ix, iy, iz)
 // as mode_q2 takes them

@@ -123,10 +123,13 @@ struct BatchRep {
     int64_t off;   // element offset of the replica inside each ping-pong buffer
     int whole;     // 1: N == 64, one periodic item
 };
-// batch regions are kBatchRegion x kBatchRegion cells; interior side of a K-step item
+// batch regions are kBatchRegion rows high and 64 or 128 columns wide (shape);
+// interior of a K-step halo item
 constexpr int kBatchRegion = 64;
-constexpr int kBatchShapes = 3;
-int batch_tile(int K);
+constexpr int kBatchShapes = 5;
+int batch_columns_per_lane(int shape);
+int batch_tile_x(int shape, int K);
+int batch_tile_y(int K);
 hipError_t batch_launch(int precision, int model, int shape, int K, const void *a, void *b, const BatchRep *reps,
                         const int2 *items, int nitems, hipStream_t s);
 // per-replica FP64 observables: result[2 r] = sum rho, result[2 r + 1] = sum f-density

@@ -37,7 +37,7 @@ hipError_t stencil_tiles(const Params &p, const void *in, void *out, hipStream_t
 // (p.wave_cfg), columns per lane x rows per wave x waves (region W x H):
 //   0 = 1 x 16 x 4 (64 x 64)    1 = 2 x 8 x 4 (128 x 32)   2 = 2 x 8 x 8 (128 x 64)
 //   3 = 1 x 8 x 8 (64 x 64)     4 = 1 x 4 x 16 (64 x 64)   5 = 2 x 4 x 16 (128 x 64)
-//   6 = 1 x 8 x 16 (64 x 128)
+//   6 = 1 x 8 x 16 (64 x 128)   7 = 2 x 5 x 16 (128 x 80)  8 = 3 x 4 x 16 (192 x 64)
 // Fewer cells per lane means more waves per CU to hide latency, at the price
 // of more row exchanges through LDS.  K in {1, 2, 4, 8} steps per launch where
 // the region leaves an interior.
@@ -48,7 +48,9 @@ using WaveCfg3 = WaveCfg<1, 8, 8>;
 using WaveCfg4 = WaveCfg<1, 4, 16>;
 using WaveCfg5 = WaveCfg<2, 4, 16>;
 using WaveCfg6 = WaveCfg<1, 8, 16>;
-static_assert(kWaveCfgs == 7, "one WaveCfg per p.wave_cfg value");
+using WaveCfg7 = WaveCfg<2, 5, 16>;
+using WaveCfg8 = WaveCfg<3, 4, 16>;
+static_assert(kWaveCfgs == 9, "one WaveCfg per p.wave_cfg value");
 
 template <class CFG, int K>
 constexpr bool wave_ok() { return CFG::RW - 4 * K > 0 && CFG::RH - 4 * K > 0; }
@@ -76,6 +78,8 @@ hipError_t stencil_k(const Params &p, int K, const void *in, void *out, hipStrea
                 case 2: return wave_k<T, MODEL, MOB, EM, WaveCfg2>(p, K, in, out, s);
                 case 5: return wave_k<T, MODEL, MOB, EM, WaveCfg5>(p, K, in, out, s);
                 case 6: return wave_k<T, MODEL, MOB, EM, WaveCfg6>(p, K, in, out, s);
+                case 7: return wave_k<T, MODEL, MOB, EM, WaveCfg7>(p, K, in, out, s);
+                case 8: return wave_k<T, MODEL, MOB, EM, WaveCfg8>(p, K, in, out, s);
                 default: break;
                 }
             }

@@ -305,6 +305,11 @@ class System:
         return int(L.lib().ch_sys_steps_per_launch(self.handle))
 
     @property
+    def region_shape(self):
+        """Region shape index of the 2D wave kernel, -1 where it is not used."""
+        return int(L.lib().ch_sys_region_shape(self.handle))
+
+    @property
     def gmres_iterations(self):
         """GMRES iterations of the last pseudospectral_mobility step (use_gmres)."""
         return int(L.lib().ch_sys_gmres_iterations(self.handle))

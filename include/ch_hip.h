@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 4
+#define CH_ABI_VERSION 5
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -174,6 +174,11 @@ ch_status ch_sys_evolve(ch_sys *sys, int64_t nsteps);
  * measurement (the reference always does one step per launch,
  * EulerCUDA.cu:60-69).  Override with the CH_TB_STEPS environment variable. */
 int32_t ch_sys_steps_per_launch(const ch_sys *sys);
+/* Region shape index (0-8) of the 2D register/DPP stencil kernel, -1 for the
+ * systems that do not use it (1D, 3D, N < 64, spectral, species, noise).  No
+ * reference counterpart: introspection for measurement.  Override with the
+ * CH_WAVE_CFG environment variable. */
+int32_t ch_sys_region_shape(const ch_sys *sys);
 /* GMRES iterations of the last pseudospectral_mobility step with use_gmres
  * (0 otherwise); the reference computes the count and discards it
  * (PseudospectralMobilityCPU.cpp:375-376). */
@@ -209,8 +214,9 @@ ch_status ch_batch_evolve(ch_batch *batch, int64_t nsteps);
 ch_status ch_batch_observables(ch_batch *batch, double *mass, double *free_energy);
 int32_t ch_batch_steps_per_launch(const ch_batch *batch);
 /* Autotuner (no reference counterpart): times every region split of the
- * batch kernel (shape 0: 16 rows x 4 waves, 1: 8 x 8, 2: 4 x 16 per 64 x 64
- * region) at 2, 4 and 8 steps per launch on the batch's current fields and
+ * batch kernel (rows per wave x waves over 64-row regions: shape 0: 16 x 4,
+ * 1: 8 x 8, 2: 4 x 16 with 64 columns; 3: 4 x 16, 4: 8 x 8 with 128 columns,
+ * two per lane) at 2, 4 and 8 steps per launch on the batch's current fields and
  * keeps the fastest per time step; the fields are not changed.  Runs by
  * itself on the first ch_batch_evolve unless the environment pins the choice
  * (CH_BATCH_TB, CH_BATCH_SHAPE).  Synchronous; either output may be NULL. */

@@ -37,7 +37,8 @@ def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
     each replica the bits of the same system run alone."""
     Ns = [64, 128, 64, 256, 128, 512, 64]
     descs, fields = make_replicas(ch, precision, Ns)
-    for shape, tb in (("0", "1"), ("0", "2"), ("0", "4"), ("1", "4"), ("2", "4"), ("1", "8"), ("2", "8")):
+    for shape, tb in (("0", "1"), ("0", "2"), ("0", "4"), ("1", "4"), ("2", "4"), ("1", "8"), ("2", "8"),
+                      ("3", "1"), ("3", "4"), ("3", "8"), ("4", "2"), ("4", "8")):
         monkeypatch.setenv("CH_BATCH_TB", tb)
         monkeypatch.setenv("CH_BATCH_SHAPE", shape)
         b = ch.Batch(ctx, descs)
@@ -100,7 +101,7 @@ def test_autotune_keeps_state_and_results(ch, ctx):
         b.upload(r, f)
     shape, k = b.autotune()
     print(f"autotuned: shape {shape}, {k} steps per launch")
-    assert shape in (0, 1, 2) and k in (2, 4, 8) and b.steps_per_launch == k
+    assert shape in (0, 1, 2, 3, 4) and k in (2, 4, 8) and b.steps_per_launch == k
     for r, f in enumerate(fields):
         assert np.array_equal(b.download(r), f.astype(np.float32).astype(np.float64))
     b.evolve(45)

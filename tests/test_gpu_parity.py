@@ -233,7 +233,7 @@ def test_temporal_blocking_variants_bitwise(ch, ctx, monkeypatch, integrator, mo
     rho0 = O.init_density(2, N, 0.5, 0.1, 0, 21)
     kw = dict(dims=2, N=N, dt=0.01, dx=1.0, k=1.0, M=1.0, rho_min=0.1)
     outs = {}
-    for cfg in ("0", "1", "2", "3", "4", "5", "6"):
+    for cfg in ("0", "1", "2", "3", "4", "5", "6", "7", "8"):
         for tb in ("1", "2", "4", "8"):
             if cfg == "1" and tb == "8":
                 continue  # a 128 x 32 region has no interior left after 8 steps
