@@ -398,20 +398,43 @@ __device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, co
     F::template pass3_final<INV>(lane, v, tw, tstride);
 }
 
+// The system's twiddle table (N entries, W_N^m) staged in LDS when N <= 1024:
+// one wave runs per SIMD at most (a line per wave), so nothing hides a global
+// load, and every transform pass after the first barrier would otherwise wait
+// on its twiddles.  The copy's loads issue together with the line's.
+template <typename T2, int TN>
+struct TwiddleStage {
+    static constexpr bool on = TN <= 1024;
+    T2 t[on ? TN : 1];
+    __device__ __forceinline__ const T2 *fill(const T2 *__restrict__ tw, int lane) {
+        if constexpr(on) {
+#pragma unroll
+            for(int i = 0; i < TN / 64; i++) t[lane + 64 * i] = tw[lane + 64 * i];
+            return t;  // visible after the transform's first barrier
+        }
+        else {
+            (void)lane;
+            return tw;
+        }
+    }
+};
+
 // axis pass (as k_col), one line per 64-thread block
 template <typename T, int MODE, int LOG2N>
 __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__restrict__ src,
                                              typename Complex<T>::type *__restrict__ dst,
                                              typename Complex<T>::type *__restrict__ rho_hat,
-                                             const typename Complex<T>::type *__restrict__ tw, Params p, int lines,
+                                             const typename Complex<T>::type *tw, Params p, int lines,
                                              long P, long ls, long stride, int axis) {
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E;
     __shared__ T2 lds[F::LDS];
+    __shared__ TwiddleStage<T2, (1 << LOG2N)> stage;  // table length N = line length
     const int l = xcd_tile(blockIdx.x, gridDim.x);
     if(l >= lines) return;
     const int lane = threadIdx.x;
+    tw = stage.fill(tw, lane);
     const int Nh = p.N / 2 + 1;
     const size_t base = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls;
     const int tstride = p.N >> LOG2N;
@@ -439,6 +462,107 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
     }
 #pragma unroll
     for(int m = 0; m < E; m++) dst[base + (size_t)(lane + 64 * m) * stride] = v[m];
+}
+
+// ---- two waves per line of n = 2 n2 points (n2 = 64 E): the axis pass for
+// n = 1024 and 2048, where one wave's chain of transform passes, not memory,
+// bounds the single-wave kernel (a 2D N = 1024 grid has only 513 lines).
+// Forward, decimation in time: wave h transforms the entries 2 i + h
+// (i = lane + 64 m) with the n2-point wave FFT; X_k = E_k + W_n^k O_k and
+// X_{k + n2} = E_k - W_n^k O_k, so after one exchange wave h holds X_{k + h n2}.
+// Inverse, decimation in frequency: e_k = X_k + X_{k + n2},
+// o_k = (X_k - X_{k + n2}) W_n^{-k}; wave 0 transforms e into the outputs 2 j,
+// wave 1 transforms o into the outputs 2 j + 1.  Each exchange goes through the
+// waves' own transform buffers (free between transforms).
+template <typename T, int MODE, int LOG2N>
+__global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *__restrict__ src,
+                                              typename Complex<T>::type *__restrict__ dst,
+                                              typename Complex<T>::type *__restrict__ rho_hat,
+                                              const typename Complex<T>::type *tw, Params p, int lines, long P,
+                                              long ls, long stride, int axis) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N - 1>;
+    constexpr int E = F::E, n2 = F::n;
+    __shared__ T2 lds[2][F::LDS];
+    __shared__ TwiddleStage<T2, 2 * n2> stage;  // table length N = line length
+    const int l = xcd_tile(blockIdx.x, gridDim.x);
+    if(l >= lines) return;
+    const int lane = threadIdx.x & 63;
+    const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if constexpr(TwiddleStage<T2, 2 * n2>::on) {
+#pragma unroll
+        for(int i = 0; i < 2 * n2 / 128; i++) stage.t[threadIdx.x + 128 * i] = tw[threadIdx.x + 128 * i];
+        tw = stage.t;  // visible after the first barrier
+    }
+    const int Nh = p.N / 2 + 1;
+    const size_t base = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls;
+    const int ts = 2 * (p.N >> LOG2N);  // the n2-point transforms use W_n2^m = W_n^(2m)
+    T2 *own = lds[h], *other = lds[h ^ 1];
+    T2 v[E], rh[MODE == 2 ? E : 1];
+    // forward input: entries 2 i + h; inverse input: entries k + h n2
+#pragma unroll
+    for(int m = 0; m < E; m++) {
+        const int i = lane + 64 * m;
+        v[m] = src[base + (size_t)(MODE == 1 ? i + h * n2 : 2 * i + h) * stride];
+    }
+    if constexpr(MODE == 2) {
+#pragma unroll
+        for(int m = 0; m < E; m++) rh[m] = rho_hat[base + (size_t)(lane + 64 * m + h * n2) * stride];
+    }
+    auto forward = [&]() {
+        wave_transform<T2, LOG2N - 1, false>(lane, v, own, tw, ts);
+        // wave 1 turns O_k into W_n^k O_k; both publish, then combine
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const int k = lane + 64 * m;
+            if(h == 1) v[m] = cmul(v[m], tw[k * (ts / 2)]);
+        }
+        __syncthreads();
+#pragma unroll
+        for(int m = 0; m < E; m++) own[lane + 64 * m] = v[m];
+        __syncthreads();
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const T2 o = other[lane + 64 * m];
+            v[m] = h == 0 ? cadd(v[m], o) : csub(o, v[m]);  // E + WO | E - WO
+        }
+    };
+    auto inverse = [&]() {
+        __syncthreads();
+#pragma unroll
+        for(int m = 0; m < E; m++) own[lane + 64 * m] = v[m];
+        __syncthreads();
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const int k = lane + 64 * m;
+            const T2 o = other[lane + 64 * m];
+            if(h == 0) v[m] = cadd(v[m], o);  // X_k + X_{k + n2}
+            else v[m] = cmul(csub(o, v[m]), conj_(tw[k * (ts / 2)]));  // (X_k - X_{k + n2}) W^-k
+        }
+        wave_transform<T2, LOG2N - 1, true>(lane, v, own, tw, ts);
+    };
+    if constexpr(MODE == 1) inverse();
+    else forward();
+    if constexpr(MODE == 2) {
+        const T inv_bins = (T)p.inv_bins;
+        const int ix = l % Nh;
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const int j = lane + 64 * m + h * n2;
+            int iy, iz;
+            if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
+            else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
+            v[m] = spectral_update<T, T2>(p, ix, iy, iz, v[m], rh[m], inv_bins);
+            rho_hat[base + (size_t)j * stride] = rh[m];
+        }
+        inverse();
+    }
+    // forward output: entries k + h n2; inverse output: entries 2 j + h
+#pragma unroll
+    for(int m = 0; m < E; m++) {
+        const int i = lane + 64 * m;
+        dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride] = v[m];
+    }
 }
 
 // c2r pre-processing of a row of n + 1 half-spectrum entries into the packed
@@ -490,15 +614,17 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
 // x pass, forward (as k_xfwd), one row per 64-thread block; n = 2^LOG2N = N / 2
 template <typename T, int MODEL, bool FPRIME, int LOG2N>
 __global__ __launch_bounds__(64) void k_xfwdw(const T *__restrict__ in, typename Complex<T>::type *__restrict__ out,
-                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              const typename Complex<T>::type *tw, Params p, int rows,
                                               long rs, long ks) {
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
+    __shared__ TwiddleStage<T2, 2 * n> stage;  // table length N = 2 n
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
+    tw = stage.fill(tw, lane);
     const T2 *rin = reinterpret_cast<const T2 *>(in + (size_t)row * 2 * n);
     T2 v[E];
 #pragma unroll
@@ -514,18 +640,20 @@ __global__ __launch_bounds__(64) void k_xfwdw(const T *__restrict__ in, typename
 // x pass, inverse (as k_xinv)
 template <typename T, int LOG2N>
 __global__ __launch_bounds__(64) void k_xinvw(const typename Complex<T>::type *__restrict__ in, T *__restrict__ out,
-                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              const typename Complex<T>::type *tw, Params p, int rows,
                                               long rs, long ks) {
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
+    __shared__ TwiddleStage<T2, 2 * n> stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
     T2 v[E];
-    c2r_pre<T, E>(in + (size_t)row * rs, ks, lane, tw, v);
-    wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
+    c2r_pre<T, E>(in + (size_t)row * rs, ks, lane, tw, v);  // before the first barrier: global table
+    const T2 *twl = stage.fill(tw, lane);
+    wave_transform<T2, LOG2N, true>(lane, v, lds, twl, 2);
     T2 *rout = reinterpret_cast<T2 *>(out + (size_t)row * 2 * n);
 #pragma unroll
     for(int m = 0; m < E; m++) rout[lane + 64 * m] = v[m];
@@ -534,18 +662,20 @@ __global__ __launch_bounds__(64) void k_xinvw(const typename Complex<T>::type *_
 // x pass between two steps (as k_xrow): c2r, f', r2c, in place on W
 template <typename T, int MODEL, int LOG2N>
 __global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restrict__ W,
-                                              const typename Complex<T>::type *__restrict__ tw, Params p, int rows,
+                                              const typename Complex<T>::type *tw, Params p, int rows,
                                               long rs, long ks, T *__restrict__ rho_out) {
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
+    __shared__ TwiddleStage<T2, 2 * n> stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
     T2 *w = W + (size_t)row * rs;
     T2 v[E];
-    c2r_pre<T, E>(w, ks, lane, tw, v);
+    c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
+    tw = stage.fill(tw, lane);
     wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
     if(rho_out) {
         T2 *ro = reinterpret_cast<T2 *>(rho_out + (size_t)row * 2 * n);
@@ -633,6 +763,12 @@ template <typename T, int MODE, int LOG2N>
 static void launch_col(const typename Complex<T>::type *src, typename Complex<T>::type *dst,
                        typename Complex<T>::type *rh, const typename Complex<T>::type *tw, const Params &p, int lines,
                        long P, long ls, long stride, int axis, bool contig, hipStream_t s) {
+    if constexpr(LOG2N == 10 || LOG2N == 11) {
+        (void)contig;
+        hipLaunchKernelGGL((k_colw2<T, MODE, LOG2N>), dim3((lines + 7) / 8 * 8), dim3(128), 0, s, src, dst, rh, tw, p,
+                           lines, P, ls, stride, axis);
+        return;
+    }
     if constexpr(wave_len<LOG2N>()) {
         (void)contig;
         hipLaunchKernelGGL((k_colw<T, MODE, LOG2N>), dim3((lines + 7) / 8 * 8), dim3(64), 0, s, src, dst, rh, tw, p, lines,

@@ -85,6 +85,26 @@ def test_pseudospectral_against_oracle(ch, ctx, dims, N, steps):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("N,steps", [(512, 20), (1024, 20), (2048, 6)])
+def test_pseudospectral_large_grids_against_oracle(ch, ctx, N, steps):
+    """2D grids whose axis passes run the wave kernels: one wave per line
+    (N = 512) and two waves per line (N = 1024, 2048), with the one-wave row
+    passes.  FP64 within 1e-10 relative L-inf of the oracle, mass within 1e-12;
+    FP32 within 1e-4."""
+    print(f"pseudospectral 2D N={N}, {steps} steps", flush=True)
+    rho0 = O.init_density(2, N, 0.0, 0.3, 0, 5)
+    kw = dict(dims=2, N=N, dt=0.1, dx=1.0, k=1.0)
+    ref = O.run(O.PSEUDOSPECTRAL, O.Params(**kw), O.Model(epsilon=0.9), rho0, steps)
+    out = run_gpu(ch, ctx, ch.make_desc(integrator="pseudospectral", epsilon=0.9, **kw), rho0, steps)
+    err = rel_linf(out, ref)
+    print(f"N={N}: fp64 rel L-inf vs oracle = {err:.3e}")
+    assert err < 1e-10
+    assert abs(out.sum() - rho0.sum()) < 1e-12 * N * N
+    out32 = run_gpu(ch, ctx, ch.make_desc(integrator="pseudospectral", epsilon=0.9, precision="fp32", **kw), rho0,
+                    steps)
+    assert rel_linf(out32, ref) < 1e-4
+
+
 def test_pseudospectral_options_against_oracle(ch, ctx):
     """S splitting, dealiasing, dx != 1 and M != 1 (PseudospectralCPU.cpp:23-25,119-133)."""
     for dims in (1, 2, 3):
