@@ -43,6 +43,7 @@ class ChDesc(ctypes.Structure):
 
 
 MAX_SPECIES, MAX_PATCHES = 8, 16
+OBS_QUEUE = 8  # CH_OBS_QUEUE
 
 
 class ChSpeciesModel(ctypes.Structure):
@@ -115,6 +116,9 @@ def lib():
         "ch_sys_region_shape": (ctypes.c_int32, [P]),
         "ch_sys_gmres_iterations": (ctypes.c_int32, [P]),
         "ch_sys_observables": (ctypes.c_int, [P, DD, DD]),
+        "ch_sys_observables_queue": (ctypes.c_int, [P]),
+        "ch_sys_observables_fetch": (ctypes.c_int, [P, DD, DD]),
+        "ch_sys_observables_pending": (ctypes.c_int32, [P]),
         "ch_sq_nshells": (I32, [I32]),
         "ch_sys_structure_factor": (ctypes.c_int, [P, P, I32]),
         "ch_sys_sq_loss": (ctypes.c_int, [P, P, I32, DD, P]),

@@ -61,7 +61,11 @@ struct ch_sys {
     void *psm_vec[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // V, w, b, x, rhs, r, av
     double *psm_dev = nullptr;   // kPsmParts partials, 4 scalars, Hcol, y
     double *psm_host = nullptr;  // pinned, kPsmParts doubles
-    double *obs_host = nullptr;  // pinned: mass and free energy, written by the finalize kernel
+    // pinned: mass and free energy written by the finalize kernel, the pair of
+    // the synchronous call first, then the CH_OBS_QUEUE slots of the queue
+    double *obs_host = nullptr;
+    hipEvent_t obs_ev[CH_OBS_QUEUE] = {};
+    int obs_head = 0, obs_count = 0;
     int gmres_iters = 0;
     // multi-species models: host and device model, X (Generic), mu, mobility,
     // per-species spectra and two scratch spectra
@@ -314,6 +318,8 @@ static void free_sys(ch_sys *s) {
     delete s->spm_host;
     if(s->psm_host) (void)hipHostFree(s->psm_host);
     if(s->obs_host) (void)hipHostFree(s->obs_host);
+    for(hipEvent_t e : s->obs_ev)
+        if(e) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -866,20 +872,62 @@ extern "C" int32_t ch_sys_region_shape(const ch_sys *s) {
     return s->p.wave_cfg;
 }
 
+// the FP64 sums of the current field into pinned host memory, straight from
+// the finalize kernel (no copy behind the reduction): the synchronous pair
+// (slot -1) or queue slot `slot`
+static hipError_t enqueue_observables(ch_sys *s, int slot) {
+    ch_ctx *c = s->ctx;
+    if(!s->obs_host) {
+        hipError_t e = hipHostMalloc((void **)&s->obs_host, 2 * (1 + CH_OBS_QUEUE) * sizeof(double));
+        if(e != hipSuccess) return e;
+    }
+    double *dst = s->obs_host + 2 * (1 + slot);
+    if(is_species(s)) return chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, dst, c->stream);
+    return chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts, dst,
+                            c->stream);
+}
+static ch_status observables_result(ch_sys *s, const double *pair, double *mass, double *free_energy);
+
 extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_energy) {
     if(!s) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    // the two sums land in pinned host memory straight from the finalize
-    // kernel: no copy behind the reduction, one stream synchronisation
-    if(!s->obs_host) HIPCHK(c, hipHostMalloc((void **)&s->obs_host, 2 * sizeof(double)));
-    if(is_species(s))
-        HIPCHK(c, chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, s->obs_host, c->stream));
-    else
-        HIPCHK(c, chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts,
-                                   s->obs_host, c->stream));
+    HIPCHK(c, enqueue_observables(s, -1));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const double h[2] = {s->obs_host[0], s->obs_host[1]};
+    return observables_result(s, s->obs_host, mass, free_energy);
+}
+
+extern "C" ch_status ch_sys_observables_queue(ch_sys *s) {
+    if(!s) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->obs_count == CH_OBS_QUEUE)
+        return fail(c, CH_EINVAL, "%d observables already queued: fetch before queueing more", CH_OBS_QUEUE);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int slot = (s->obs_head + s->obs_count) % CH_OBS_QUEUE;
+    if(!s->obs_ev[slot]) HIPCHK(c, hipEventCreateWithFlags(&s->obs_ev[slot], hipEventDisableTiming));
+    HIPCHK(c, enqueue_observables(s, slot));
+    HIPCHK(c, hipEventRecord(s->obs_ev[slot], c->stream));
+    s->obs_count++;
+    return CH_OK;
+}
+
+extern "C" ch_status ch_sys_observables_fetch(ch_sys *s, double *mass, double *free_energy) {
+    if(!s) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->obs_count == 0) return fail(c, CH_EINVAL, "no observables queued");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int slot = s->obs_head;
+    HIPCHK(c, hipEventSynchronize(s->obs_ev[slot]));
+    s->obs_head = (s->obs_head + 1) % CH_OBS_QUEUE;
+    s->obs_count--;
+    return observables_result(s, s->obs_host + 2 * (1 + slot), mass, free_energy);
+}
+
+extern "C" int32_t ch_sys_observables_pending(const ch_sys *s) { return s ? s->obs_count : 0; }
+
+static ch_status observables_result(ch_sys *s, const double *pair, double *mass, double *free_energy) {
+    ch_ctx *c = s->ctx;
+    const double h[2] = {pair[0], pair[1]};
     const double m = h[0] * s->p.vbin_over_bins, f = h[1] * s->p.vbin_over_bins;
     if(mass) *mass = m;
     if(free_energy) *free_energy = f;
@@ -1034,9 +1082,9 @@ struct ch_batch {
     int64_t total = 0;                       // elements over all replicas
     std::vector<chd::BatchRep> host_reps;
     chd::BatchRep *reps = nullptr;
-    // work list per region width (64 or 128 columns) and K = 1, 2, 4, 8
-    int2 *items[2][kBatchDepths] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
-    int nitems[2][kBatchDepths] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    // work list per region width (64, 128 or 192 columns) and K = 1, 2, 4, 8
+    int2 *items[chd::kBatchWidths][kBatchDepths] = {};
+    int nitems[chd::kBatchWidths][kBatchDepths] = {};
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;
     void *stage = nullptr;                   // largest replica, doubles
@@ -1089,7 +1137,7 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
         }
         if(esh) b->shape = std::max(0, std::min(chd::kBatchShapes - 1, std::atoi(esh)));
     }
-    std::vector<int2> items[2][kBatchDepths];
+    std::vector<int2> items[chd::kBatchWidths][kBatchDepths];
     int64_t maxbins = 0;
     for(int r = 0; r < n; r++) {
         chd::BatchRep rep{};
@@ -1097,14 +1145,14 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
         rep.off = b->total;
         const int N = descs[r].N;
         rep.whole = N == chd::kBatchRegion ? 1 : 0;
-        for(int fam = 0; fam < 2; fam++)
+        for(int fam = 0; fam < chd::kBatchWidths; fam++)
             for(int i = 0; i < kBatchDepths; i++) {
                 if(rep.whole) {
                     items[fam][i].push_back(int2{r, 0});
                     continue;
                 }
-                // shapes 0 and 3 stand for the 64- and 128-column families
-                const int Tx = chd::batch_tile_x(fam ? 3 : 0, 1 << i), Ty = chd::batch_tile_y(1 << i);
+                // shapes 0, 3 and 5 stand for the 64-, 128- and 192-column families
+                const int Tx = chd::batch_tile_x(fam == 0 ? 0 : (fam == 1 ? 3 : 5), 1 << i), Ty = chd::batch_tile_y(1 << i);
                 const int ntx = (N + Tx - 1) / Tx, nty = (N + Ty - 1) / Ty;
                 for(int ty = 0; ty < nty; ty++)
                     for(int tx = 0; tx < ntx; tx++) items[fam][i].push_back(int2{r, tx | (ty << 16)});
@@ -1119,7 +1167,7 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
               hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
               hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
-    for(int fam = 0; fam < 2; fam++)
+    for(int fam = 0; fam < chd::kBatchWidths; fam++)
         for(int i = 0; ok && i < kBatchDepths; i++) {
             const std::vector<int2> &v = items[fam][i];
             b->nitems[fam][i] = (int)v.size();
@@ -1195,7 +1243,8 @@ static hipError_t batch_step(ch_batch *b, int &cur, int K) {
 
 // Times every (shape, K) candidate on the batch's own data: each trial
 // launch reads the current buffer and writes the other one, so the state is
-// unchanged.  Keeps the fastest per time step.
+// unchanged.  A candidate's time is the best of three runs of 8 launches (the
+// candidates differ by a few per cent); keeps the fastest per time step.
 static ch_status batch_tune(ch_batch *b) {
     ch_ctx *c = b->ctx;
     hipEvent_t e0, e1;
@@ -1206,14 +1255,18 @@ static ch_status batch_tune(ch_batch *b) {
     hipError_t err = hipSuccess;
     for(int shape = 0; shape < chd::kBatchShapes && err == hipSuccess; shape++) {
         for(int K : {2, 4, 8}) {
-            constexpr int warm = 2, reps = 8;
+            constexpr int warm = 2, reps = 8, runs = 3;
             for(int r = 0; r < warm && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
-            if(err == hipSuccess) err = hipEventRecord(e0, c->stream);
-            for(int r = 0; r < reps && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
-            if(err == hipSuccess) err = hipEventRecord(e1, c->stream);
-            if(err == hipSuccess) err = hipEventSynchronize(e1);
-            float ms = 0;
-            if(err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            float ms = 1e30f;
+            for(int run = 0; run < runs && err == hipSuccess; run++) {
+                err = hipEventRecord(e0, c->stream);
+                for(int r = 0; r < reps && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
+                if(err == hipSuccess) err = hipEventRecord(e1, c->stream);
+                if(err == hipSuccess) err = hipEventSynchronize(e1);
+                float t = 0;
+                if(err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+                ms = std::min(ms, t);
+            }
             if(err != hipSuccess) break;
             const float per_step = ms / (reps * K);
             if(per_step < best) {

@@ -7,10 +7,10 @@
 // steps with the register/DPP wave kernel (stencil_wave.h).  A replica with
 // N = 64 is one item covering the whole periodic grid (no halo); larger
 // replicas are covered by halo tiles.  Halo items are 64 x 64 regions (one
-// column per lane) or, for the wide shapes, 128 x 64 regions (two columns per
-// lane: a larger share of each region is interior, and one DPP move serves two
-// cells); a whole N = 64 replica is one 64 x 64 region split over the same
-// number of waves.  The region is split over the waves in one of kBatchShapes
+// column per lane) or, for the wide shapes, 128 x 64 or 192 x 64 regions (two
+// or three columns per lane: a larger share of each region is interior, and
+// one DPP move serves several cells); a whole N = 64 replica is one 64 x 64
+// region split over the same number of waves.  The region is split over the waves in one of kBatchShapes
 // ways; the shape and K are picked by timing the candidates on the batch itself
 // (ch_batch_autotune).
 // Per-replica epsilon, k, dt, M, dx and N travel in a device array of Params.
@@ -46,12 +46,13 @@ __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T
     }
 }
 
-int batch_columns_per_lane(int shape) { return shape >= 3 ? 2 : 1; }
+int batch_columns_per_lane(int shape) { return shape >= 5 ? 3 : (shape >= 3 ? 2 : 1); }
 int batch_tile_x(int shape, int K) { return 64 * batch_columns_per_lane(shape) - 4 * K; }
 int batch_tile_y(int K) { return kBatchRegion - 4 * K; }
 
-// shape s (rows per wave x waves; halo regions of 64 or 128 columns):
-//   0 = 16 x 4, 1 = 8 x 8, 2 = 4 x 16 (64 columns);  3 = 4 x 16, 4 = 8 x 8 (128 columns)
+// shape s (rows per wave x waves; halo regions of 64, 128 or 192 columns):
+//   0 = 16 x 4, 1 = 8 x 8, 2 = 4 x 16 (64 columns);  3 = 4 x 16, 4 = 8 x 8 (128 columns);
+//   5 = 4 x 16, 6 = 8 x 8 (192 columns)
 template <typename T, int MODEL, int K>
 static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchRep *reps, const int2 *items,
                                  int nitems, hipStream_t s) {
@@ -61,6 +62,8 @@ static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchR
     case 2: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
     case 3: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
     case 4: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 5: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 6: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

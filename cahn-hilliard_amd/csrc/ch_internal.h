@@ -123,10 +123,11 @@ struct BatchRep {
     int64_t off;   // element offset of the replica inside each ping-pong buffer
     int whole;     // 1: N == 64, one periodic item
 };
-// batch regions are kBatchRegion rows high and 64 or 128 columns wide (shape);
-// interior of a K-step halo item
+// batch regions are kBatchRegion rows high and 64, 128 or 192 columns wide
+// (shape); interior of a K-step halo item
 constexpr int kBatchRegion = 64;
-constexpr int kBatchShapes = 5;
+constexpr int kBatchShapes = 7;
+constexpr int kBatchWidths = 3;
 int batch_columns_per_lane(int shape);
 int batch_tile_x(int shape, int K);
 int batch_tile_y(int K);

@@ -319,6 +319,20 @@ class System:
         self._chk(L.lib().ch_sys_observables(self.handle, ctypes.byref(m), ctypes.byref(f)))
         return m.value, f.value
 
+    def queue_observables(self):
+        """Enqueue the observables of the current field; read them later with
+        fetch_observables (first in, first out)."""
+        self._chk(L.lib().ch_sys_observables_queue(self.handle))
+
+    def fetch_observables(self):
+        m, f = ctypes.c_double(), ctypes.c_double()
+        self._chk(L.lib().ch_sys_observables_fetch(self.handle, ctypes.byref(m), ctypes.byref(f)))
+        return m.value, f.value
+
+    @property
+    def observables_pending(self):
+        return int(L.lib().ch_sys_observables_pending(self.handle))
+
     def average_mass(self):
         return self.observables()[0]
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 5
+#define CH_ABI_VERSION 6
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -189,6 +189,21 @@ int32_t ch_sys_gmres_iterations(const ch_sys *sys);
  * Returns CH_ENAN when either sum is NaN. */
 ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 
+/* Queued observables (no reference counterpart).  ch_sys_observables_queue
+ * enqueues the reductions of ch_sys_observables on the current field behind
+ * the work already issued and returns at once; ch_sys_observables_fetch waits
+ * for the oldest queued pair and returns it (first in, first out), with
+ * CH_ENAN as ch_sys_observables.  A driver that prints every few steps
+ * (src/main.cpp:120-128) queues at each print point and fetches one interval
+ * later, so the device does not wait for the host between intervals; the
+ * values and their order are those of the synchronous call.  At most
+ * CH_OBS_QUEUE pairs are pending: queueing beyond that, or fetching with none
+ * pending, returns CH_EINVAL. */
+#define CH_OBS_QUEUE 8
+ch_status ch_sys_observables_queue(ch_sys *sys);
+ch_status ch_sys_observables_fetch(ch_sys *sys, double *mass, double *free_energy);
+int32_t ch_sys_observables_pending(const ch_sys *sys);
+
 /* ---- structure factor (2D): the circular average of utils/sq_from_traj.py:72-103
  * (mean removed, |FFT|^2 / N^2, shell = rint(|k| / dq)), computed with the
  * library's own FFT.  nshells must equal ch_sq_nshells(N) = rint(N sqrt(2) / 2) + 1. */
@@ -216,7 +231,8 @@ int32_t ch_batch_steps_per_launch(const ch_batch *batch);
 /* Autotuner (no reference counterpart): times every region split of the
  * batch kernel (rows per wave x waves over 64-row regions: shape 0: 16 x 4,
  * 1: 8 x 8, 2: 4 x 16 with 64 columns; 3: 4 x 16, 4: 8 x 8 with 128 columns,
- * two per lane) at 2, 4 and 8 steps per launch on the batch's current fields and
+ * two per lane; 5: 4 x 16, 6: 8 x 8 with 192 columns, three per lane) at 2, 4
+ * and 8 steps per launch on the batch's current fields and
  * keeps the fastest per time step; the fields are not changed.  Runs by
  * itself on the first ch_batch_evolve unless the environment pins the choice
  * (CH_BATCH_TB, CH_BATCH_SHAPE).  Synchronous; either output may be NULL. */

@@ -28,7 +28,7 @@ def test_every_header_in_include_is_exported(ch):
 
 def test_abi_version_and_status_strings_without_gpu(ch):
     L = ch._lib.lib()
-    assert L.ch_abi_version() == 5
+    assert L.ch_abi_version() == 6
     assert L.ch_status_string(3) == b"NaN encountered"
 
 
@@ -115,3 +115,12 @@ def test_species_model_from_reference_config(ch):
                                          "saleh": {"valence": [4, 4, 2], "B2": 1.0, "delta_AA": 1e3,
                                                    "delta_BB": {"value": 2e3}}})
     assert list(sal.saleh_valence) == [4, 4, 2] and sal.saleh_delta_BB == 2e3 and list(sal.k)[:3] == [1.0] * 3
+
+
+def test_header_constants_match_the_bindings(ch):
+    import re
+    L = ch._lib
+    text = open(L.HEADER).read()
+    consts = dict(re.findall(r"#define\s+(CH_[A-Z_]+)\s+(\d+)", text))
+    assert int(consts["CH_OBS_QUEUE"]) == L.OBS_QUEUE
+    assert int(consts["CH_MAX_SPECIES"]) == L.MAX_SPECIES and int(consts["CH_MAX_PATCHES"]) == L.MAX_PATCHES

@@ -152,6 +152,41 @@ def test_observables_against_oracle(ch, ctx, integrator, dims):
         s.close()
 
 
+def test_queued_observables_equal_synchronous_ones(ch, ctx):
+    """ch_sys_observables_queue / _fetch: the values of the synchronous call at
+    each queue point, first in first out, behind later evolve calls; at most
+    CH_OBS_QUEUE pending; fetching with none pending is an error."""
+    rho0 = O.init_density(2, 256, 0.0, 0.3, 0, 4)
+    d = ch.make_desc(dims=2, N=256, dt=0.01)
+    a = ch.System(ctx, d)
+    a.upload(rho0)
+    want = []
+    for _ in range(5):
+        a.evolve(37)
+        want.append(a.observables())
+    a.close()
+    b = ch.System(ctx, d)
+    b.upload(rho0)
+    got = []
+    for i in range(5):
+        b.evolve(37)
+        b.queue_observables()
+        if i >= 1:
+            got.append(b.fetch_observables())  # one interval late
+    assert b.observables_pending == 1
+    got.append(b.fetch_observables())
+    assert got == want
+    with pytest.raises(ch.CHError):
+        b.fetch_observables()
+    for _ in range(ch._lib.OBS_QUEUE):
+        b.queue_observables()
+    with pytest.raises(ch.CHError):
+        b.queue_observables()
+    vals = [b.fetch_observables() for _ in range(ch._lib.OBS_QUEUE)]
+    assert all(v == want[-1] for v in vals) and b.observables_pending == 0
+    b.close()
+
+
 def test_observables_nan_status(ch, ctx):
     s = ch.System(ctx, ch.make_desc(dims=2, N=16))
     rho = np.zeros(256)
