@@ -124,6 +124,12 @@ public:
         if(!integrator->observables(m, f)) critical("Encountered a nan while computing the total free energy");
         return f;
     }
+    // average_free_energy and average_mass of the current state, read later
+    // with fetch_observables (first in, first out)
+    void queue_observables() { integrator->queue_observables(); }
+    void fetch_observables(double &F, double &m) {
+        if(!integrator->fetch_observables(m, F)) critical("Encountered a nan while computing the total free energy");
+    }
     // src/CahnHilliard.cpp:322-361 (host side, output only)
     MultiField<double> pressure() {
         integrator->sync();

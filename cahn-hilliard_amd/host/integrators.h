@@ -121,6 +121,14 @@ public:
         _check(st, "ch_sys_observables");
         return true;
     }
+    // the same pair, queued behind the issued steps and read later (FIFO)
+    void queue_observables() { _check(ch_sys_observables_queue(_sys), "ch_sys_observables_queue"); }
+    bool fetch_observables(double &mass, double &fe) {
+        ch_status st = ch_sys_observables_fetch(_sys, &mass, &fe);
+        if(st == CH_ENAN) return false;
+        _check(st, "ch_sys_observables_fetch");
+        return true;
+    }
     const ch_desc &desc() const { return _desc; }
     ch_ctx *context() { return _ctx; }
     ch_sys *system() { return _sys; }

@@ -9,6 +9,7 @@
 // (CahnHilliard::evolve(n)), which the library replays as captured graphs.
 #include <cstdio>
 #include <ctime>
+#include <deque>
 #include <fstream>
 #include <iostream>
 
@@ -68,6 +69,26 @@ public:
         std::ofstream mass_output((_output_path / "energy.dat").string(), _openmode);
         const long long t_end = _initial_t + _steps;
         long long t = _initial_t;
+        // energy lines: the sums of step t are queued before the next steps are
+        // issued and the line is written once those steps are on their way, so
+        // the device does not wait for the host at every print; the lines and
+        // their order are the reference's
+        std::deque<long long> pending;
+        auto energy_line = [&](long long ts, double F, double m) {
+            std::string out = fmt_general(ts * _system->dt, 5) + " " + fmt_general(F, 8) + " " + fmt_general(m, 5) +
+                              " " + std::to_string(ts);
+            if(_print_pressure) out += " " + fmt_general(_system->average_pressure(), 5);
+            mass_output << out << std::endl;
+            std::cout << out << std::endl;
+        };
+        auto write_pending = [&](size_t keep) {
+            while(pending.size() > keep) {
+                double F, m;
+                _system->fetch_observables(F, m);
+                energy_line(pending.front(), F, m);
+                pending.pop_front();
+            }
+        };
         while(t < t_end) {
             if(_printer->should_print_last(t)) _print_current_state("last_", t);
             if(_printer->should_print_traj(t)) {
@@ -77,12 +98,17 @@ public:
                 _printer->trajectory_frame_done();
             }
             if(_print_mass_every > 0 && t % _print_mass_every == 0) {
-                double F = _system->average_free_energy(), m = _system->average_mass();
-                std::string out = fmt_general(t * _system->dt, 5) + " " + fmt_general(F, 8) + " " + fmt_general(m, 5) +
-                                  " " + std::to_string(t);
-                if(_print_pressure) out += " " + fmt_general(_system->average_pressure(), 5);
-                mass_output << out << std::endl;
-                std::cout << out << std::endl;
+                if(_print_pressure) {
+                    // the pressure needs the host copy of this state: no queueing
+                    write_pending(0);
+                    double F = _system->average_free_energy(), m = _system->average_mass();
+                    energy_line(t, F, m);
+                }
+                else {
+                    write_pending(CH_OBS_QUEUE - 1);
+                    _system->queue_observables();
+                    pending.push_back(t);
+                }
             }
             // steps until the next output event
             long long next = _printer->next_event(t, t_end);
@@ -90,7 +116,9 @@ public:
             next = std::max(next, t + 1);
             _system->evolve(next - t);
             t = next;
+            write_pending(1);
         }
+        write_pending(0);
         mass_output.close();
         _print_current_state("last_", _steps);
     }
