@@ -164,10 +164,9 @@ __global__ __launch_bounds__(NT) void k_em_noise(const T *__restrict__ in, T *__
     }
 }
 
-template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ>
+template <typename T, int DIMS, int MODEL, int MOB, int TX, int TY, int TZ, int NT = 256>
 hipError_t launch_noise_cfg(const Params &p, const void *in, void *out, const int64_t *step_base, int step_off,
                             hipStream_t s) {
-    constexpr int NT = 256;
     const int tiles_x = p.N / TX;
     const int tiles_y = DIMS >= 2 ? p.N / TY : 1;
     const int tiles_z = DIMS == 3 ? p.N / TZ : 1;
@@ -176,7 +175,8 @@ hipError_t launch_noise_cfg(const Params &p, const void *in, void *out, const in
     return hipGetLastError();
 }
 
-// tiles: 1D 256 | 4, 2D 64 x 16 | 4 x 4, 3D 8 x 8 x 8 | 4 x 4 x 4 (the large one where N allows)
+// tiles: 1D 256 | 4, 2D 64 x 32 | 4 x 4, 3D 16 x 8 x 8 | 8 x 8 x 8 | 4 x 4 x 4 (the largest N allows;
+// Params::noise_tile selects the earlier 64 x 16 / 8 x 8 x 8 shapes)
 template <typename T, int DIMS, int MODEL, int MOB>
 hipError_t noise_tiles(const Params &p, const void *in, void *out, const int64_t *step_base, int step_off,
                        hipStream_t s) {
@@ -185,10 +185,18 @@ hipError_t noise_tiles(const Params &p, const void *in, void *out, const int64_t
         return launch_noise_cfg<T, 1, MODEL, MOB, 4, 1, 1>(p, in, out, step_base, step_off, s);
     }
     else if constexpr(DIMS == 2) {
-        if(p.N >= 64) return launch_noise_cfg<T, 2, MODEL, MOB, 64, 16, 1>(p, in, out, step_base, step_off, s);
+        if(p.N >= 64) {
+            if(p.noise_tile == 1) return launch_noise_cfg<T, 2, MODEL, MOB, 64, 16, 1, 512>(p, in, out, step_base, step_off, s);
+            if(p.noise_tile == 2) return launch_noise_cfg<T, 2, MODEL, MOB, 64, 32, 1, 512>(p, in, out, step_base, step_off, s);
+            return launch_noise_cfg<T, 2, MODEL, MOB, 64, 16, 1>(p, in, out, step_base, step_off, s);
+        }
         return launch_noise_cfg<T, 2, MODEL, MOB, 4, 4, 1>(p, in, out, step_base, step_off, s);
     }
     else {
+        if(p.N >= 16) {
+            if(p.noise_tile == 1) return launch_noise_cfg<T, 3, MODEL, MOB, 8, 8, 8, 512>(p, in, out, step_base, step_off, s);
+            if(p.noise_tile == 2) return launch_noise_cfg<T, 3, MODEL, MOB, 16, 8, 8, 512>(p, in, out, step_base, step_off, s);
+        }
         if(p.N >= 8) return launch_noise_cfg<T, 3, MODEL, MOB, 8, 8, 8>(p, in, out, step_base, step_off, s);
         return launch_noise_cfg<T, 3, MODEL, MOB, 4, 4, 4>(p, in, out, step_base, step_off, s);
     }

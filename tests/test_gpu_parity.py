@@ -85,6 +85,31 @@ def test_pseudospectral_against_oracle(ch, ctx, dims, N, steps):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("dims,N,steps,integrator", [(2, 2048, 19, "euler"), (2, 4096, 5, "euler"),
+                                                     (2, 2048, 11, "euler_mobility"), (3, 256, 2, "euler")])
+def test_stencil_large_grids_against_oracle(ch, ctx, dims, N, steps, integrator):
+    """The default region shapes at the largest grids (2D N = 2048, 4096: 128 x 80
+    regions; 3D N = 256: 16 x 8 x 8 tiles), with step counts that run launches
+    of every depth (19 = 8 + 8 + 2 + 1): FP64 within 1e-12 relative L-inf of
+    the oracle, mass within 1e-12."""
+    print(f"{integrator} {dims}D N={N}, {steps} steps", flush=True)
+    rho0 = O.init_density(dims, N, 0.0, 0.3, 0, 7)
+    kw = dict(dims=dims, N=N, dt=0.01, dx=1.0, k=1.0, M=1.0)
+    ref = O.run(O.EULER if integrator == "euler" else O.EULER_MOBILITY, O.Params(**kw), O.Model(epsilon=0.9),
+                rho0, steps)
+    s = ch.System(ctx, ch.make_desc(integrator=integrator, epsilon=0.9, **kw))
+    if dims == 2:
+        assert s.region_shape == 7
+    s.upload(rho0)
+    s.evolve(steps)
+    out = s.download()
+    s.close()
+    err = rel_linf(out, ref)
+    print(f"rel L-inf vs oracle = {err:.3e}")
+    assert err < 1e-12
+    assert abs(out.sum() - rho0.sum()) < 1e-12 * N ** dims
+
+
 @pytest.mark.parametrize("N,steps", [(512, 20), (1024, 20), (2048, 6)])
 def test_pseudospectral_large_grids_against_oracle(ch, ctx, N, steps):
     """2D grids whose axis passes run the wave kernels: one wave per line
