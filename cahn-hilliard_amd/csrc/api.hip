@@ -67,6 +67,7 @@ struct ch_sys {
     hipEvent_t obs_ev[CH_OBS_QUEUE] = {};
     int obs_head = 0, obs_count = 0;
     int gmres_iters = 0;
+    bool psm_fuse = true;  // CH_PSM_FUSE=0: the element-wise kernels around whole transforms
     // multi-species models: host and device model, X (Generic), mu, mobility,
     // per-species spectra and two scratch spectra
     int S = 1;
@@ -356,6 +357,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     if(const char *r = std::getenv("CH_TILE3D")) s->p.tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_ADJ_TILE3D")) s->p.adj_tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_NOISE_TILE")) s->p.noise_tile = std::max(0, std::min(2, std::atoi(r)));
+    if(const char *r = std::getenv("CH_PSM_FUSE")) s->psm_fuse = std::atoi(r) != 0;
     if(const char *r = std::getenv("CH_ADJ_CFG")) {
         const int v = std::atoi(r);
         if(v == 0 || v == 3 || v == 4 || v == 5) s->p.adj_cfg = v;
@@ -721,6 +723,7 @@ static chd::PsmArgs psm_args(ch_sys *s) {
     B.rhs = s->psm_vec[4];
     B.r = s->psm_vec[5];
     B.av = s->psm_vec[6];
+    B.fuse = s->psm_fuse;
     return B;
 }
 

@@ -39,6 +39,11 @@ hipError_t ps_steps_tape(const SpectralArgs &a, hipStream_t s, int64_t nsteps, v
 // fprime) into spec; c2r of spec (overwritten) into out
 hipError_t spec_r2c(const SpectralArgs &a, const void *real_in, void *spec_out, bool fprime, hipStream_t s);
 hipError_t spec_c2r(const SpectralArgs &a, void *spec, void *real_out, hipStream_t s);
+// pseudospectral_mobility: divh = sum_d (i k_d) deal r2c(w c2r((i k_d) deal src) / N^d),
+// w = M - M0 (sub, M0 = *M0p) or M, with the element-wise work inside the
+// transform passes; hipErrorNotSupported outside 2D N = 1024, 2048
+hipError_t psm_div_fused(const SpectralArgs &a, const void *src, void *gh, void *divh, const void *mob,
+                         const double *M0p, int sub, hipStream_t s);
 // pseudospectral reverse step (ps_adjoint.hip): lam_in = dL/drho_{n+1} ->
 // lam_out = dL/drho_n with rho_n = `rho`; spec[3] half spectra and real[2]
 // fields of scratch; kPsGradBlocks x kGradSlots partial sums added to gacc
@@ -69,6 +74,7 @@ struct PsmArgs {
     void *w, *b, *x, *rhs, *r, *av;
     double *Hcol, *ydev;          // restart + 2 doubles each (device)
     double *host_partials;        // pinned host memory, kPsmParts doubles
+    bool fuse;                    // divergence term through psm_div_fused where it applies
 };
 // one splitting step, asynchronous (capturable)
 hipError_t psm_step_simple(const PsmArgs &B, hipStream_t s);

@@ -19,6 +19,7 @@
 // several per block.  Twiddles come from an FP64 table W_N^m = e^{-2 pi i m/N}
 // built on the host from exact integer indices (rounded once for FP32).
 #include "ch_internal.h"
+#include "psm_ops.h"
 #include "wave_fft.h"
 
 namespace chd {
@@ -474,12 +475,18 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
 // o_k = (X_k - X_{k + n2}) W_n^{-k}; wave 0 transforms e into the outputs 2 j,
 // wave 1 transforms o into the outputs 2 j + 1.  Each exchange goes through the
 // waves' own transform buffers (free between transforms).
+// MODE 0 forward, 1 inverse, 2 forward + spectral update + inverse (as k_col);
+// for the pseudospectral_mobility divergence term (gaxis = gradient axis):
+// 3 the gradient factor (i k_gaxis) deal on load, then inverse (src -> dst);
+// 4 forward, then rho_hat (the divergence) =|+= (i k_gaxis) deal (transform),
+// no dst.  Modes 3 and 4 use the per-entry operations of psm_ops.h.
 template <typename T, int MODE, int LOG2N>
 __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *__restrict__ src,
                                               typename Complex<T>::type *__restrict__ dst,
                                               typename Complex<T>::type *__restrict__ rho_hat,
                                               const typename Complex<T>::type *tw, Params p, int lines, long P,
-                                              long ls, long stride, int axis) {
+                                              long ls, long stride, int axis, int gaxis = 0) {
+    constexpr bool INV_IN = MODE == 1 || MODE == 3;  // input entries k + h n2 (else 2 i + h)
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N - 1>;
     constexpr int E = F::E, n2 = F::n;
@@ -503,7 +510,21 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
 #pragma unroll
     for(int m = 0; m < E; m++) {
         const int i = lane + 64 * m;
-        v[m] = src[base + (size_t)(MODE == 1 ? i + h * n2 : 2 * i + h) * stride];
+        v[m] = src[base + (size_t)(INV_IN ? i + h * n2 : 2 * i + h) * stride];
+    }
+    // the line's mode indices: ix = l % Nh; entry j along the pass axis
+    auto modes = [&](int j, int &ix, int &iy, int &iz) {
+        ix = l % Nh;
+        if(axis == 1) { iy = j; iz = l / Nh; }
+        else { iy = l / Nh; iz = j; }
+    };
+    if constexpr(MODE == 3) {
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            int ix, iy, iz;
+            modes(lane + 64 * m + h * n2, ix, iy, iz);
+            v[m] = psm_grad_term<T>(p, ix, iy, iz, gaxis, v[m]);
+        }
     }
     if constexpr(MODE == 2) {
 #pragma unroll
@@ -541,7 +562,7 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
         }
         wave_transform<T2, LOG2N - 1, true>(lane, v, own, tw, ts);
     };
-    if constexpr(MODE == 1) inverse();
+    if constexpr(INV_IN) inverse();
     else forward();
     if constexpr(MODE == 2) {
         const T inv_bins = (T)p.inv_bins;
@@ -556,6 +577,17 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             rho_hat[base + (size_t)j * stride] = rh[m];
         }
         inverse();
+    }
+    if constexpr(MODE == 4) {
+#pragma unroll
+        for(int m = 0; m < E; m++) {
+            const int j = lane + 64 * m + h * n2;
+            int ix, iy, iz;
+            modes(j, ix, iy, iz);
+            const size_t q = base + (size_t)j * stride;
+            rho_hat[q] = psm_div_term<T>(p, ix, iy, iz, gaxis, v[m], gaxis == 0 ? T2{0, 0} : rho_hat[q]);
+        }
+        return;
     }
     // forward output: entries k + h n2; inverse output: entries 2 j + h
 #pragma unroll
@@ -684,6 +716,36 @@ __global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restr
     }
 #pragma unroll
     for(int m = 0; m < E; m++) v[m] = T2{fprime<T, MODEL>(p, v[m].x), fprime<T, MODEL>(p, v[m].y)};
+    wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
+    r2c_post<T, LOG2N>(v, lds, lane, tw, w, ks);
+}
+
+// x pass of the pseudospectral_mobility flux (as k_xroww with the flux in place
+// of f'): c2r of a gradient row, g -> (M - M0) g / N^d (psm_flux), r2c, in place
+template <typename T, int LOG2N>
+__global__ __launch_bounds__(64) void k_xfluxw(typename Complex<T>::type *__restrict__ W,
+                                               const typename Complex<T>::type *tw, Params p, int rows, long rs,
+                                               long ks, const T *__restrict__ mob, const double *__restrict__ M0p,
+                                               int sub) {
+    using T2 = typename Complex<T>::type;
+    using F = WaveFFT<T2, LOG2N>;
+    constexpr int E = F::E, n = F::n;
+    __shared__ T2 lds[F::LDS];
+    __shared__ TwiddleStage<T2, 2 * n> stage;
+    const int row = xcd_tile(blockIdx.x, gridDim.x);
+    if(row >= rows) return;
+    const int lane = threadIdx.x;
+    T2 *w = W + (size_t)row * rs;
+    const T2 *mrow = reinterpret_cast<const T2 *>(mob + (size_t)row * 2 * n);
+    const T M0 = sub ? (T)*M0p : (T)0, ib = (T)p.inv_bins;
+    T2 v[E], mv[E];
+    c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
+#pragma unroll
+    for(int m = 0; m < E; m++) mv[m] = mrow[lane + 64 * m];
+    tw = stage.fill(tw, lane);
+    wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
+#pragma unroll
+    for(int m = 0; m < E; m++) v[m] = T2{psm_flux<T>(v[m].x, mv[m].x, M0, ib), psm_flux<T>(v[m].y, mv[m].y, M0, ib)};
     wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
     r2c_post<T, LOG2N>(v, lds, lane, tw, w, ks);
 }
@@ -956,6 +1018,43 @@ hipError_t spec_r2c(const SpectralArgs &a, const void *real_in, void *spec_out, 
 hipError_t spec_c2r(const SpectralArgs &a, void *spec, void *real_out, hipStream_t s) {
     if(a.precision == CH_FP64) return spec_p<double>(a, nullptr, spec, real_out, false, true, s);
     return spec_p<float>(a, nullptr, spec, real_out, false, true, s);
+}
+
+// ---- pseudospectral_mobility divergence term with the element-wise work
+// folded into the transform passes (2D, N = 1024 or 2048): per axis d, the
+// inverse column pass with the gradient factor (k_colw2 mode 3, src -> gh),
+// the row pass c2r -> flux -> r2c (k_xfluxw, in place on gh) and the forward
+// column pass accumulating the divergence (k_colw2 mode 4, gh -> divh): three
+// launches in place of seven, the same operations in the same order
+template <typename T, int LOG2N>
+static hipError_t psm_div_t(const SpectralArgs &a, const void *src, void *gh, void *divh, const void *mob,
+                            const double *M0p, int sub, hipStream_t s) {
+    using T2 = typename Complex<T>::type;
+    const Params &p = a.p;
+    constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
+    const SpecLayout L = spec_layout(2, N);
+    const T2 *tw = (const T2 *)a.twiddles;
+    const int colb = (Nh + 7) / 8 * 8, rowb = (N + 7) / 8 * 8;
+    for(int d = 0; d < 2; d++) {
+        hipLaunchKernelGGL((k_colw2<T, 3, LOG2N>), dim3(colb), dim3(128), 0, s, (const T2 *)src, (T2 *)gh, nullptr, tw,
+                           p, Nh, L.yP, L.yls, L.ystride, 1, d);
+        hipLaunchKernelGGL((k_xfluxw<T, LX>), dim3(rowb), dim3(64), 0, s, (T2 *)gh, tw, p, N, L.rs, L.ks, (const T *)mob,
+                           M0p, sub);
+        hipLaunchKernelGGL((k_colw2<T, 4, LOG2N>), dim3(colb), dim3(128), 0, s, (const T2 *)gh, nullptr, (T2 *)divh, tw,
+                           p, Nh, L.yP, L.yls, L.ystride, 1, d);
+    }
+    return hipGetLastError();
+}
+
+hipError_t psm_div_fused(const SpectralArgs &a, const void *src, void *gh, void *divh, const void *mob,
+                         const double *M0p, int sub, hipStream_t s) {
+    if(a.p.dims != 2 || (a.p.log2N != 10 && a.p.log2N != 11)) return hipErrorNotSupported;
+    if(a.precision == CH_FP64) {
+        if(a.p.log2N == 10) return psm_div_t<double, 10>(a, src, gh, divh, mob, M0p, sub, s);
+        return psm_div_t<double, 11>(a, src, gh, divh, mob, M0p, sub, s);
+    }
+    if(a.p.log2N == 10) return psm_div_t<float, 10>(a, src, gh, divh, mob, M0p, sub, s);
+    return psm_div_t<float, 11>(a, src, gh, divh, mob, M0p, sub, s);
 }
 
 // ---- 2D inverse transform (column IFFT + c2r) of a.work into `out`

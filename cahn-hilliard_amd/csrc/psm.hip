@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "ch_internal.h"
+#include "psm_ops.h"
 
 namespace chd {
 
@@ -112,11 +113,10 @@ __global__ __launch_bounds__(256) void k_psm_mu(const C2<T> *__restrict__ fh, co
 template <typename T>
 __global__ __launch_bounds__(256) void k_psm_grad(const C2<T> *__restrict__ src, C2<T> *__restrict__ out, Params p,
                                                   int64_t hat, int axis) {
-#pragma clang fp contract(off)
-    PSM_MODES
-        const T kd = (T)(p.k_unit * axis_wavenumber(p, ix, iy, iz, axis)), d = (T)deal;
-        const C2<T> z = src[q];
-        out[q] = C2<T>{-kd * z.y * d, kd * z.x * d};
+    for(int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < hat; q += (int64_t)gridDim.x * 256) {
+        int ix, iy, iz;
+        mode_index(p, q, ix, iy, iz);
+        out[q] = psm_grad_term<T>(p, ix, iy, iz, axis, src[q]);
     }
 }
 
@@ -124,13 +124,10 @@ __global__ __launch_bounds__(256) void k_psm_grad(const C2<T> *__restrict__ src,
 template <typename T>
 __global__ __launch_bounds__(256) void k_psm_divacc(const C2<T> *__restrict__ F, C2<T> *__restrict__ div, Params p,
                                                     int64_t hat, int axis) {
-#pragma clang fp contract(off)
-    PSM_MODES
-        const T kd = (T)(p.k_unit * axis_wavenumber(p, ix, iy, iz, axis)), d = (T)deal;
-        const C2<T> f{F[q].x * d, F[q].y * d};
-        const C2<T> t{-kd * f.y, kd * f.x};
-        if(axis == 0) div[q] = C2<T>{(T)0 + t.x, (T)0 + t.y};
-        else div[q] = C2<T>{div[q].x + t.x, div[q].y + t.y};
+    for(int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < hat; q += (int64_t)gridDim.x * 256) {
+        int ix, iy, iz;
+        mode_index(p, q, ix, iy, iz);
+        div[q] = psm_div_term<T>(p, ix, iy, iz, axis, F[q], axis == 0 ? C2<T>{0, 0} : div[q]);
     }
 }
 
@@ -177,7 +174,7 @@ __global__ __launch_bounds__(256) void k_psm_flux(T *__restrict__ g, const T *__
                                                   const double *__restrict__ M0p, int sub) {
 #pragma clang fp contract(off)
     const T M0 = sub ? (T)*M0p : (T)0, ib = (T)p.inv_bins;
-    PSM_CELLS g[i] = (mob[i] - M0) * (g[i] * ib);
+    PSM_CELLS g[i] = psm_flux<T>(g[i], mob[i], M0, ib);
 }
 
 // out = f'(rho) - S rho (:309-314)
@@ -319,6 +316,10 @@ struct Ops {
 
     // B.divh = sum_d (i k_d) deal r2c(w c2r((i k_d) deal src) / N^d), w = M - M0 (sub) or M
     hipError_t div_w_grad(const void *src, int sub) {
+        if(B.fuse) {
+            const hipError_t ef = psm_div_fused(B.a, src, B.gh, B.divh, B.mob, B.scal, sub, s);
+            if(ef != hipErrorNotSupported) return ef;
+        }
         hipError_t e = hipSuccess;
         for(int d = 0; d < p().dims && e == hipSuccess; d++) {
             hipLaunchKernelGGL((k_psm_grad<T>), dim3(grid_h), dim3(256), 0, s, (const C2<T> *)src, c(B.gh), p(), hat, d);

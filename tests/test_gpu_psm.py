@@ -167,3 +167,43 @@ def test_psm_rejects_reverse_mode(ch, ctx):
     with pytest.raises(ch.CHError):
         s.record(4, 0)
     s.close()
+
+
+LARGE = [
+    ("landau2d_N1024_regularised_dealias", dict(dims=2, N=1024, dt=0.02, dx=1.0, k=1.0, mobility="regularised", M=1.0,
+                                                rho_min=0.2, S=0.5, use_dealias=True, **LANDAU), (0.5, 0.3)),
+    ("wertheim2d_N2048_linear", dict(dims=2, N=2048, dt=3e-4, dx=1.0, k=1.0, mobility="linear", M=1.0, rho_ref=0.05,
+                                     S=0.0, use_dealias=True, **WERTHEIM), (0.05, 0.05)),
+]
+
+
+@pytest.mark.parametrize("name,kw,init", LARGE, ids=[c[0] for c in LARGE])
+def test_fused_divergence_equals_elementwise_kernels(ch, ctx, monkeypatch, name, kw, init):
+    """2D N = 1024 and 2048 fold the divergence term's element-wise kernels into
+    the transform passes; the result is the same bits as the element-wise path
+    (CH_PSM_FUSE=0), for the splitting and the GMRES schemes."""
+    print(f"case {name}", flush=True)
+    rho0 = initial(kw, *init)
+    outs = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("CH_PSM_FUSE", fuse)
+        outs["split", fuse] = run_gpu(ch, ctx, kw, rho0, (5, 2))
+        it = []
+        outs["gmres", fuse] = run_gpu(ch, ctx, kw, rho0, (2,), iters=it, use_gmres=True, gmres_tol=1e-10)
+        outs["iters", fuse] = it
+    assert np.array_equal(outs["split", "1"], outs["split", "0"])
+    assert np.array_equal(outs["gmres", "1"], outs["gmres", "0"])
+    assert outs["iters", "1"] == outs["iters", "0"]
+
+
+def test_fused_splitting_matches_oracle_at_1024(ch, ctx):
+    """The fused path against the oracle (splitting, FP64, 4 steps): <= 1e-10."""
+    name, kw, init = LARGE[0]
+    rho0 = initial(kw, *init)
+    p, m = oracle_of(kw)
+    want = O.run_psm(p, m, rho0, 4)
+    got = run_gpu(ch, ctx, kw, rho0, (4,))
+    err = rel_linf(got, want)
+    print(f"{name}: fused splitting fp64 rel L-inf vs oracle = {err:.3e}")
+    assert err < 1e-10
+    assert abs(got.sum() - rho0.sum()) < 1e-12 * np.abs(rho0).sum()
