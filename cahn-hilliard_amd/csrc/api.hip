@@ -75,12 +75,13 @@ struct ch_sys {
     double *sp_X = nullptr;
     void *sp_mu = nullptr, *sp_mob = nullptr, *sp_rho_hat = nullptr, *sp_spec[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    // stencil integrators: the launches of the last remainder of nsteps % G
-    // steps (per starting buffer), captured when the same remainder recurs
+    // stencil integrators: the launches of a count of at most 4 G steps, or of
+    // the remainder nsteps % G of a longer run (per starting buffer), captured
+    // when the same count recurs
     hipGraphExec_t rexec[2] = {nullptr, nullptr};
     int64_t rsteps[2] = {0, 0};
-    int rcur[2] = {0, 0};       // the buffer holding the field after the remainder
-    int64_t rseen[2] = {0, 0};  // the last remainder run by direct launches
+    int rcur[2] = {0, 0};       // the buffer holding the field after those steps
+    int64_t rseen[2] = {0, 0};  // the last such count run by direct launches
 };
 
 static ch_status fail(ch_ctx *ctx, ch_status st, const char *fmt, ...) {
@@ -807,7 +808,9 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
     if(s->d.integrator == CH_PSEUDOSPECTRAL_MOBILITY) return evolve_psm(s, nsteps);
     if(has_noise(s)) return evolve_noise(s, nsteps);
     const int G = s->graph_steps;
-    if(G > 0 && nsteps >= G) {
+    // up to 4 G steps, a recurring count is one cached graph of all its launches
+    // (below); longer runs replay the G-step graph, then the remainder
+    if(G > 0 && nsteps > 4 * G) {
         const int par = s->cur;
         if(!s->gexec[par]) {
             hipGraph_t graph;
@@ -841,8 +844,8 @@ extern "C" ch_status ch_sys_evolve(ch_sys *s, int64_t nsteps) {
         HIPCHK(c, rest(s->cur, nsteps));
         return CH_OK;
     }
-    // a remainder of at least one full launch that recurs from the same buffer:
-    // one graph, kept until another remainder recurs
+    // a count (or remainder) of at least one full launch that recurs from the
+    // same buffer: one graph, kept until another count recurs
     const int par = s->cur;
     if(!s->rexec[par] || s->rsteps[par] != nsteps) {
         if(s->rseen[par] != nsteps) {

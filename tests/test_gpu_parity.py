@@ -285,13 +285,13 @@ def test_graph_replay_equals_eager(ch, ctx, monkeypatch):
 
 @pytest.mark.parametrize("N", [64, 1024])
 def test_remainder_graph_equals_direct_launches(ch, ctx, N):
-    """Remainders of nsteps % 64 that recur run as one cached graph per starting
-    buffer (captured on the second occurrence, kept until another remainder
-    recurs): the same bits as the direct launches, whatever the sequence of
-    counts and parities."""
+    """Recurring counts of at most 256 steps, and the remainders nsteps % 64 of
+    longer runs, run as one cached graph per starting buffer (captured on the
+    second occurrence, kept until another count recurs): the same bits as the
+    direct launches, whatever the sequence of counts and parities."""
     rho0 = O.init_density(2, N, 0.0, 0.3, 0, 9)
     d = ch.make_desc(dims=2, N=N, dt=0.01)
-    counts = [100, 100, 100, 20, 100, 20, 20, 100, 9, 9, 9, 36, 36, 36, 100]
+    counts = [100, 100, 100, 20, 100, 20, 20, 100, 9, 9, 9, 36, 36, 36, 100, 300, 300, 300, 256, 256, 7]
     s = ch.System(ctx, d)
     s.upload(rho0)
     for n in counts:
