@@ -305,6 +305,27 @@ def test_remainder_graph_equals_direct_launches(ch, ctx, N):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("N", [64, 1024])
+def test_pseudospectral_cached_graphs_equal_direct_steps(ch, ctx, N):
+    """Pseudospectral chains: recurring counts of at most 256 steps replay one
+    cached graph, longer runs the 64-step graph and the remainder; the same bits
+    as one step per call."""
+    rho0 = O.init_density(2, N, 0.0, 0.3, 0, 13)
+    d = ch.make_desc(dims=2, N=N, integrator="pseudospectral", dt=0.1, epsilon=0.9)
+    counts = [20, 20, 20, 5, 20, 5, 5, 300, 300, 7]
+    s = ch.System(ctx, d)
+    s.upload(rho0)
+    for n in counts:
+        s.evolve(n)
+    a = s.download()
+    s.upload(rho0)
+    for _ in range(sum(counts)):
+        s.evolve(1)
+    b = s.download()
+    s.close()
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("integrator,mob", [("euler", "constant"), ("euler_mobility", "regularised")])
 def test_temporal_blocking_variants_bitwise(ch, ctx, monkeypatch, integrator, mob):
     """Every steps-per-launch depth and rows-per-wave split of the 2D wave kernel
