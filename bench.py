@@ -451,7 +451,7 @@ def main():
                              "wertheim", "noise", "psm", "species", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}).",
+        print(f"bench.py runs on one GPU only: the library has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

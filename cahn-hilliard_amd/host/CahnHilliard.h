@@ -6,7 +6,7 @@
 // x fastest, the generator seeded by the Manager with srand48(seed)); unit
 // scaling; mobility before integrator; the integrator factory keyed on
 // `integrator`.  The factory builds the HIP integrators whatever `use_CUDA`
-// says: this build has no CPU integrators.  average_mass / average_free_energy
+// says: the HIP drivers have no CPU integrators.  average_mass / average_free_energy
 // are the library's FP64 device reductions; a NaN is fatal as in the reference.
 #pragma once
 
@@ -82,7 +82,7 @@ public:
         V_bin = CUB(dx);
 
         mobility.reset(build_mobility<dims>(config, _sim_state));
-        if(!sim_state.use_CUDA) info("use_CUDA = false: this build always integrates on the GPU");
+        if(!sim_state.use_CUDA) info("use_CUDA = false: the HIP drivers always integrate on the GPU");
         if(user_integrator == "euler") integrator = std::make_unique<EulerHIP<dims>>(_sim_state, m, config);
         else if(user_integrator == "euler_mobility")
             integrator = std::make_unique<EulerMobilityHIP<dims>>(_sim_state, m, config);

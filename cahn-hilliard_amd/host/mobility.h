@@ -61,7 +61,7 @@ IMobility<dims> *build_mobility(const chtoml::Table &config, SimulationState<dim
         double rho_ref = h._config_value<double>(config, "mobility.rho_ref") / CUB(state.user_to_internal);
         return new IMobility<dims>(state, CH_MOB_LINEAR, M, 0.0, rho_ref);
     }
-    if(type == "gel") h.critical("The 'gel' mobility is not part of this build");
+    if(type == "gel") h.critical("The 'gel' mobility is not supported by the HIP library");
     throw std::runtime_error("Unknown mobility type: " + type);
 }
 

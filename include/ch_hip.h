@@ -34,7 +34,7 @@ typedef enum ch_status {
     CH_EINVAL = 1,        /* bad argument or descriptor                         */
     CH_EHIP = 2,          /* HIP runtime error (message in ch_ctx_last_error)   */
     CH_ENAN = 3,          /* NaN met in a reduction (CahnHilliard.cpp:290-293)  */
-    CH_EUNSUPPORTED = 4,  /* valid request this build does not implement        */
+    CH_EUNSUPPORTED = 4,  /* valid request the library does not implement       */
     CH_ENOMEM = 5,        /* device allocation failed                           */
     CH_ESTATE = 6         /* call out of order (e.g. adjoint before record)     */
 } ch_status;
