@@ -177,19 +177,21 @@ LARGE = [
 ]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("name,kw,init", LARGE, ids=[c[0] for c in LARGE])
-def test_fused_divergence_equals_elementwise_kernels(ch, ctx, monkeypatch, name, kw, init):
+def test_fused_divergence_equals_elementwise_kernels(ch, ctx, monkeypatch, name, kw, init, precision):
     """2D N = 1024 and 2048 fold the divergence term's element-wise kernels into
     the transform passes; the result is the same bits as the element-wise path
-    (CH_PSM_FUSE=0), for the splitting and the GMRES schemes."""
-    print(f"case {name}", flush=True)
+    (CH_PSM_FUSE=0), for the splitting and the GMRES schemes, FP64 and FP32."""
+    print(f"case {name} {precision}", flush=True)
     rho0 = initial(kw, *init)
     outs = {}
+    tol = 1e-10 if precision == "fp64" else 1e-6
     for fuse in ("1", "0"):
         monkeypatch.setenv("CH_PSM_FUSE", fuse)
-        outs["split", fuse] = run_gpu(ch, ctx, kw, rho0, (5, 2))
+        outs["split", fuse] = run_gpu(ch, ctx, kw, rho0, (5, 2), precision)
         it = []
-        outs["gmres", fuse] = run_gpu(ch, ctx, kw, rho0, (2,), iters=it, use_gmres=True, gmres_tol=1e-10)
+        outs["gmres", fuse] = run_gpu(ch, ctx, kw, rho0, (2,), precision, iters=it, use_gmres=True, gmres_tol=tol)
         outs["iters", fuse] = it
     assert np.array_equal(outs["split", "1"], outs["split", "0"])
     assert np.array_equal(outs["gmres", "1"], outs["gmres", "0"])
