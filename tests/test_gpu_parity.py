@@ -151,6 +151,53 @@ def test_euler_c1_config_against_oracle(ch, ctx):
     assert abs(out.sum() - rho0.sum()) < 1e-12
 
 
+def test_euler_c1_full_length_against_oracle(ch, ctx):
+    """The C1 workload at its full length: Landau 2D N=128, dt=0.01, 10,000 steps,
+    FP64 against the oracle (the field has left the linear regime and reached
+    |psi| ~ 0.9): within 1e-12 relative L-inf (measured 8e-15), mass within
+    1e-12, free energy within 1e-12 relative."""
+    rho0 = O.init_density(2, 128, 0.0, 0.01, 0, 123456)
+    kw = dict(dims=2, N=128, dt=0.01, dx=1.0, k=1.0)
+    p = O.Params(**kw)
+    ref = O.run(O.EULER, p, O.Model(epsilon=0.9), rho0, 10000)
+    s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.9, **kw))
+    s.upload(rho0)
+    for _ in range(100):
+        s.evolve(100)
+    out = s.download()
+    m, f = s.observables()
+    s.close()
+    err = rel_linf(out, ref)
+    print(f"C1 10,000 steps: fp64 rel L-inf vs oracle = {err:.3e}")
+    assert err < 1e-12
+    assert abs(out.sum() - rho0.sum()) < 1e-12
+    assert f == pytest.approx(O.average_free_energy(p, O.Model(epsilon=0.9), out), rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pseudospectral_c2_full_length_invariants(ch, ctx, precision):
+    """The C2 workload at its full length (Landau 2D N=1024, dt=0.1, 20,000
+    steps; too long for the oracle): the field stays finite and bounded, the
+    mass is conserved (FP64 to 1e-12 of the sum of |psi0|, FP32 fields to
+    1e-5), and the free energy has decreased from its initial value."""
+    rho0 = O.init_density(2, 1024, 0.0, 0.01, 0, 123456)
+    s = ch.System(ctx, ch.make_desc(dims=2, N=1024, integrator="pseudospectral", dt=0.1, epsilon=0.9,
+                                    precision=precision))
+    s.upload(rho0)
+    _, f0 = s.observables()
+    s.evolve(20000)
+    out = s.download()
+    m, f = s.observables()
+    s.close()
+    scale = np.abs(rho0).sum()
+    drift = abs(out.sum() - rho0.sum()) / scale
+    print(f"C2 {precision} 20,000 steps: mass drift {drift:.3e} of sum|psi0|, F {f0:.6e} -> {f:.6e}, "
+          f"max |psi| {np.abs(out).max():.4f}")
+    assert np.isfinite(out).all() and np.abs(out).max() < 1.5
+    assert drift < (1e-12 if precision == "fp64" else 1e-5)
+    assert f < f0
+
+
 @pytest.mark.parametrize("integrator", ["euler", "euler_mobility", "pseudospectral"])
 @pytest.mark.parametrize("dims", [1, 2, 3])
 def test_observables_against_oracle(ch, ctx, integrator, dims):
