@@ -174,22 +174,29 @@ def w_batch(ch, ctx, args):
     w = time.perf_counter() - t0
     b.observables()
     b.close()
-    # FP32 field error: the same replicas from the same start, FP32 vs FP64 batches
+    # FP32 field error: the same replicas from the same start, FP32 vs FP64
+    # batches, worst replica, after `nst` steps and after C3's full 5000
     out = {}
+    marks = (nst, 5000) if nst < 5000 else (nst,)
     for prec in ("fp32", "fp64"):
         dd = [ch.make_desc(dims=2, N=d.N, integrator="euler", precision=prec, dt=0.01, dx=1.0, k=1.0,
                            epsilon=d.epsilon) for d in descs]
         bb = ch.Batch(ctx, dd)
         for i, f in enumerate(fields):
             bb.upload(i, f)
-        bb.evolve(nst)
-        out[prec] = [bb.download(i) for i in range(len(dd))]
+        done = 0
+        for t in marks:
+            bb.evolve(t - done)
+            done = t
+            out[prec, t] = [bb.download(i) for i in range(len(dd))]
         bb.close()
-    err = max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(out["fp32"], out["fp64"]))
+    errs = {t: max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(out["fp32", t], out["fp64", t]))
+            for t in marks}
     return {"Gcell-updates/s": round(cells * nst / w / 1e9, 3), "replicas": len(descs), "cells": cells,
             "time_steps": nst, "autotuned": {"shape": shape, "steps_per_launch": spl},
             "us_per_time_step": round(w / nst * 1e6, 3),
-            "fp32_rel_linf_vs_fp64": float(f"{err:.3e}"), "error_after_steps": nst}
+            "fp32_rel_linf_vs_fp64": float(f"{errs[nst]:.3e}"), "error_after_steps": nst,
+            "fp32_rel_linf_vs_fp64_by_steps": {str(t): float(f"{e:.3e}") for t, e in errs.items()}}
 
 
 def w_copy(ch, ctx, args):

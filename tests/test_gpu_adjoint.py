@@ -195,6 +195,49 @@ def test_c4_loss_gradient_chain(ch, ctx):
     assert grads[1] == pytest.approx(fd_k, rel=1e-5)
 
 
+def test_c4_full_size_finite_differences(ch, ctx):
+    """C4 at its real size: Landau 2D N=256, dt=0.01, 5000 steps, checkpoints
+    every 100; S(q) loss against the target of a run at eps=0.8, k=1.0; the
+    gradients at eps=0.9, k=1.2 wrt eps, k and the initial field (along one
+    random direction) against central differences with h=1e-6: within 1e-6
+    relative."""
+    N, T = 256, 5000
+    kw = dict(dims=2, N=N, dt=0.01, dx=1.0, M=1.0)
+    psi0 = O.init_density(2, N, 0.0, 0.01, 0, 123456)
+    tgt_sys = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.8, k=1.0, **kw))
+    tgt_sys.upload(psi0)
+    tgt_sys.evolve(T)
+    target = tgt_sys.structure_factor()
+    tgt_sys.close()
+
+    def loss_at(eps, k, start):
+        s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=eps, k=k, **kw))
+        s.upload(start)
+        s.evolve(T)
+        v = s.sq_loss(target, grad=False)
+        s.close()
+        return v
+
+    s = ch.System(ctx, ch.make_desc(integrator="euler", epsilon=0.9, k=1.2, **kw))
+    s.upload(psi0)
+    s.record(T, 100)
+    loss, g = s.sq_loss(target)
+    lam0, grads = s.adjoint(g)
+    s.close()
+    h = 1e-6
+    fd_eps = (loss_at(0.9 + h, 1.2, psi0) - loss_at(0.9 - h, 1.2, psi0)) / (2 * h)
+    fd_k = (loss_at(0.9, 1.2 + h, psi0) - loss_at(0.9, 1.2 - h, psi0)) / (2 * h)
+    v = np.random.default_rng(7).standard_normal(N * N)
+    v /= np.linalg.norm(v)
+    fd_psi = (loss_at(0.9, 1.2, psi0 + h * v) - loss_at(0.9, 1.2, psi0 - h * v)) / (2 * h)
+    ad_psi = float(np.dot(lam0, v))
+    print(f"C4 full size: loss {loss:.6e}; d/deps {grads[0]:.9e} vs fd {fd_eps:.9e}; d/dk {grads[1]:.9e} vs fd "
+          f"{fd_k:.9e}; d/dpsi0.v {ad_psi:.9e} vs fd {fd_psi:.9e}")
+    assert grads[0] == pytest.approx(fd_eps, rel=1e-6)
+    assert grads[1] == pytest.approx(fd_k, rel=1e-6)
+    assert ad_psi == pytest.approx(fd_psi, rel=1e-6)
+
+
 EM_CASES = [
     # dims, N, steps, ckpt, model kwargs (desc), mobility kwargs (desc), init (density, A), dt
     (3, 16, 30, 7, dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4),
