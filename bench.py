@@ -158,7 +158,21 @@ def c3_replicas(ch):
     return descs, fields
 
 
-def w_batch(ch, ctx, args):
+def w_batch(ch, ctx, args, compensated=False):
+    """C3 throughput and FP32 error; `compensated`: FP32 updates as compensated
+    sums (CH_BATCH_COMP=1, read when a batch is created)."""
+    saved = os.environ.get("CH_BATCH_COMP")
+    os.environ["CH_BATCH_COMP"] = "1" if compensated else "0"
+    try:
+        return _w_batch(ch, ctx, args)
+    finally:
+        if saved is None:
+            os.environ.pop("CH_BATCH_COMP", None)
+        else:
+            os.environ["CH_BATCH_COMP"] = saved
+
+
+def _w_batch(ch, ctx, args):
     descs, fields = c3_replicas(ch)
     b = ch.Batch(ctx, descs)
     for i, f in enumerate(fields):
@@ -454,8 +468,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--workload", default=None,
-                    choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "adjoint",
-                             "wertheim", "noise", "psm", "species", "copy"])
+                    choices=["euler", "euler_fp32", "pseudospectral", "pseudospectral_fp32", "batch", "batch_compensated",
+                             "adjoint", "wertheim", "noise", "psm", "species", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the library has no multi-GPU path (asked for {args.gpus}).",
@@ -469,6 +483,7 @@ def main():
              "pseudospectral": lambda: w_pseudospectral(ch, ctx, args, "fp64"),
              "pseudospectral_fp32": lambda: w_pseudospectral(ch, ctx, args, "fp32"),
              "batch": lambda: w_batch(ch, ctx, args), "adjoint": lambda: w_adjoint(ch, ctx, args),
+             "batch_compensated": lambda: w_batch(ch, ctx, args, compensated=True),
              "wertheim": lambda: w_wertheim(ch, ctx, args), "noise": lambda: w_noise(ch, ctx, args),
              "psm": lambda: w_psm(ch, ctx, args), "species": lambda: w_species(ch, ctx, args),
              "copy": lambda: w_copy(ch, ctx, args)}
@@ -514,7 +529,8 @@ def main():
                                "fp32_rel_linf_vs_fp64": fp32_error(ch, ctx, "euler", 0.01, N, 1000),
                                "error_after_steps": 1000}
         # a failing extra workload is reported, never allowed to drop the headline line
-        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "adjoint", "wertheim", "noise", "psm", "species"):
+        for name in ("pseudospectral", "pseudospectral_fp32", "batch", "batch_compensated", "adjoint", "wertheim",
+                     "noise", "psm", "species"):
             try:
                 extra[name] = table[name]()
             except Exception as e:  # noqa: BLE001
@@ -522,7 +538,8 @@ def main():
         try:
             extra["pseudospectral_fp32"]["fp32_rel_linf_vs_fp64"] = fp32_error(ch, ctx, "pseudospectral", 0.1, N, 1000)
             extra["pseudospectral_fp32"]["error_after_steps"] = 1000
-            extra["batch"]["vs_fp64_single"] = round(extra["batch"]["Gcell-updates/s"] / r["gcell"], 3)
+            for name in ("batch", "batch_compensated"):
+                extra[name]["vs_fp64_single"] = round(extra[name]["Gcell-updates/s"] / r["gcell"], 3)
         except Exception as e:  # noqa: BLE001
             extra["error"] = f"{type(e).__name__}: {e}"
         out["extra"] = extra

@@ -1125,6 +1125,9 @@ struct ch_batch {
     int2 *items[chd::kBatchWidths][kBatchDepths] = {};
     int nitems[chd::kBatchWidths][kBatchDepths] = {};
     void *buf[2] = {nullptr, nullptr};
+    // FP32 with compensated updates (CH_BATCH_COMP=1): the low-order parts lost
+    // by each replica's updates, ping-pong beside buf
+    void *comp[2] = {nullptr, nullptr};
     int cur = 0;
     void *stage = nullptr;                   // largest replica, doubles
     double *result = nullptr;                // 2 n doubles
@@ -1137,6 +1140,7 @@ static void free_batch(ch_batch *b) {
     for(int i = 0; i < 2; i++) {
         if(b->gexec[i]) (void)hipGraphExecDestroy(b->gexec[i]);
         if(b->buf[i]) (void)hipFree(b->buf[i]);
+        if(b->comp[i]) (void)hipFree(b->comp[i]);
     }
     for(auto &fam : b->items)
         for(int2 *it : fam)
@@ -1206,6 +1210,12 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
               hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
               hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
+    const char *ecomp = std::getenv("CH_BATCH_COMP");
+    if(ok && b->precision == CH_FP32 && ecomp && std::atoi(ecomp) != 0)
+        ok = hipMalloc(&b->comp[0], b->total * b->elt) == hipSuccess &&
+             hipMalloc(&b->comp[1], b->total * b->elt) == hipSuccess &&
+             hipMemset(b->comp[0], 0, b->total * b->elt) == hipSuccess &&
+             hipMemset(b->comp[1], 0, b->total * b->elt) == hipSuccess;
     for(int fam = 0; fam < chd::kBatchWidths; fam++)
         for(int i = 0; ok && i < kBatchDepths; i++) {
             const std::vector<int2> &v = items[fam][i];
@@ -1238,6 +1248,8 @@ extern "C" ch_status ch_batch_upload(ch_batch *b, int32_t r, const double *rho_h
     const chd::BatchRep &rep = b->host_reps[r];
     const int64_t bins = (int64_t)rep.p.N * rep.p.N;
     char *dst = (char *)b->buf[b->cur] + rep.off * b->elt;
+    // a new state: nothing of the replica's earlier updates is owed to it
+    if(b->comp[0]) HIPCHK(c, hipMemsetAsync((char *)b->comp[b->cur] + rep.off * b->elt, 0, bins * b->elt, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if(b->precision == CH_FP64) {
         HIPCHK(c, hipMemcpyAsync(dst, rho_host, bins * 8, hipMemcpyHostToDevice, c->stream));
@@ -1271,7 +1283,7 @@ extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host)
 static hipError_t batch_launch(ch_batch *b, int shape, int K, int from) {
     const int i = log2i(K), fam = chd::batch_columns_per_lane(shape) - 1;
     return chd::batch_launch(b->precision, b->model, shape, K, b->buf[from], b->buf[from ^ 1], b->reps,
-                             b->items[fam][i], b->nitems[fam][i], b->ctx->stream);
+                             b->items[fam][i], b->nitems[fam][i], b->comp[from], b->comp[from ^ 1], b->ctx->stream);
 }
 
 static hipError_t batch_step(ch_batch *b, int &cur, int K) {

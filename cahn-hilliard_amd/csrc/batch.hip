@@ -21,9 +21,11 @@ namespace chd {
 
 // item = (replica, tx | ty << 16): one halo region of C x 64 columns and 64 rows,
 // or the whole replica for N = 64 (64 x 64, one column per lane)
-template <typename T, int MODEL, int K, int C, int RPW, int NW>
+// COMP: compensated updates, the lost low-order parts in compA -> compB
+template <typename T, int MODEL, int K, int C, int RPW, int NW, bool COMP = false>
 __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T *__restrict__ bufB,
-                                                  const BatchRep *__restrict__ reps, const int2 *__restrict__ items) {
+                                                  const BatchRep *__restrict__ reps, const int2 *__restrict__ items,
+                                                  const T *__restrict__ compA, T *__restrict__ compB) {
     static_assert(RPW * NW == kBatchRegion, "batch regions are 64 rows high");
     using HALO = WaveCfg<C, RPW, NW>;
     using WHOLE = WaveCfg<1, RPW, NW>;
@@ -35,14 +37,16 @@ __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T
     const int64_t off = reps[it.x].off;
     const T *in = bufA + off;
     T *out = bufB + off;
+    const T *cin = COMP ? compA + off : nullptr;
+    T *cout = COMP ? compB + off : nullptr;
     if(reps[it.x].whole) {
-        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, WHOLE, true>(
-            in, out, p, 0, 0, *reinterpret_cast<WaveExchange<T, 2, NW, 1> *>(&ex));
+        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, WHOLE, true, false, COMP>(
+            in, out, p, 0, 0, *reinterpret_cast<WaveExchange<T, 2, NW, 1> *>(&ex), cin, cout);
     }
     else {
         const int tx = it.y & 0xFFFF, ty = it.y >> 16;
-        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, HALO, false>(in, out, p, tx * W - 2 * K, ty * TH - 2 * K,
-                                                                           ex);
+        wave_region<T, MODEL, CH_MOB_CONSTANT, false, K, HALO, false, false, COMP>(
+            in, out, p, tx * W - 2 * K, ty * TH - 2 * K, ex, cin, cout);
     }
 }
 
@@ -53,42 +57,52 @@ int batch_tile_y(int K) { return kBatchRegion - 4 * K; }
 // shape s (rows per wave x waves; halo regions of 64, 128 or 192 columns):
 //   0 = 16 x 4, 1 = 8 x 8, 2 = 4 x 16 (64 columns);  3 = 4 x 16, 4 = 8 x 8 (128 columns);
 //   5 = 4 x 16, 6 = 8 x 8 (192 columns)
-template <typename T, int MODEL, int K>
+template <typename T, int MODEL, int K, bool COMP>
 static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchRep *reps, const int2 *items,
-                                 int nitems, hipStream_t s) {
+                                 int nitems, const void *ca, void *cb, hipStream_t s) {
+    const T *A = (const T *)a, *CA = (const T *)ca;
+    T *Bf = (T *)b, *CB = (T *)cb;
     switch(shape) {
-    case 0: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 16, 4>), dim3(nitems), dim3(256), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 1: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 2: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 3: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 4: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 5: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 4, 16>), dim3(nitems), dim3(1024), 0, s, (const T *)a, (T *)b, reps, items); break;
-    case 6: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 8, 8>), dim3(nitems), dim3(512), 0, s, (const T *)a, (T *)b, reps, items); break;
+    case 0: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 16, 4, COMP>), dim3(nitems), dim3(256), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 1: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 8, 8, COMP>), dim3(nitems), dim3(512), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 2: hipLaunchKernelGGL((k_batch<T, MODEL, K, 1, 4, 16, COMP>), dim3(nitems), dim3(1024), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 3: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 4, 16, COMP>), dim3(nitems), dim3(1024), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 4: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 8, 8, COMP>), dim3(nitems), dim3(512), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 5: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 4, 16, COMP>), dim3(nitems), dim3(1024), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 6: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 8, 8, COMP>), dim3(nitems), dim3(512), 0, s, A, Bf, reps, items, CA, CB); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool COMP>
 static hipError_t launch_batch_t(int shape, int K, const void *a, void *b, const BatchRep *reps, const int2 *items,
-                                 int nitems, hipStream_t s) {
+                                 int nitems, const void *ca, void *cb, hipStream_t s) {
     switch(K) {
-    case 1: return launch_batch_k<T, MODEL, 1>(shape, a, b, reps, items, nitems, s);
-    case 2: return launch_batch_k<T, MODEL, 2>(shape, a, b, reps, items, nitems, s);
-    case 4: return launch_batch_k<T, MODEL, 4>(shape, a, b, reps, items, nitems, s);
-    case 8: return launch_batch_k<T, MODEL, 8>(shape, a, b, reps, items, nitems, s);
+    case 1: return launch_batch_k<T, MODEL, 1, COMP>(shape, a, b, reps, items, nitems, ca, cb, s);
+    case 2: return launch_batch_k<T, MODEL, 2, COMP>(shape, a, b, reps, items, nitems, ca, cb, s);
+    case 4: return launch_batch_k<T, MODEL, 4, COMP>(shape, a, b, reps, items, nitems, ca, cb, s);
+    case 8: return launch_batch_k<T, MODEL, 8, COMP>(shape, a, b, reps, items, nitems, ca, cb, s);
     }
     return hipErrorInvalidValue;
 }
 
 hipError_t batch_launch(int precision, int model, int shape, int K, const void *a, void *b, const BatchRep *reps,
-                        const int2 *items, int nitems, hipStream_t s) {
+                        const int2 *items, int nitems, const void *ca, void *cb, hipStream_t s) {
     if(precision == CH_FP64) {
-        if(model == CH_LANDAU) return launch_batch_t<double, CH_LANDAU>(shape, K, a, b, reps, items, nitems, s);
-        return launch_batch_t<double, CH_SIMPLE_WERTHEIM>(shape, K, a, b, reps, items, nitems, s);
+        if(model == CH_LANDAU)
+            return launch_batch_t<double, CH_LANDAU, false>(shape, K, a, b, reps, items, nitems, nullptr, nullptr, s);
+        return launch_batch_t<double, CH_SIMPLE_WERTHEIM, false>(shape, K, a, b, reps, items, nitems, nullptr,
+                                                                 nullptr, s);
     }
-    if(model == CH_LANDAU) return launch_batch_t<float, CH_LANDAU>(shape, K, a, b, reps, items, nitems, s);
-    return launch_batch_t<float, CH_SIMPLE_WERTHEIM>(shape, K, a, b, reps, items, nitems, s);
+    // FP32 fields: compensated updates when the batch carries the low-order parts
+    if(ca) {
+        if(model == CH_LANDAU) return launch_batch_t<float, CH_LANDAU, true>(shape, K, a, b, reps, items, nitems, ca, cb, s);
+        return launch_batch_t<float, CH_SIMPLE_WERTHEIM, true>(shape, K, a, b, reps, items, nitems, ca, cb, s);
+    }
+    if(model == CH_LANDAU)
+        return launch_batch_t<float, CH_LANDAU, false>(shape, K, a, b, reps, items, nitems, nullptr, nullptr, s);
+    return launch_batch_t<float, CH_SIMPLE_WERTHEIM, false>(shape, K, a, b, reps, items, nitems, nullptr, nullptr, s);
 }
 
 }  // namespace chd

@@ -159,6 +159,20 @@ __device__ __forceinline__ T euler_update(const Params &p, T psi, T mu, const T 
     return fma_t((T)p.Mdt, laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2), psi);
 }
 
+// the same update as a compensated sum: the FP32 rounding lost in psi + dt M
+// lap(mu) is kept in cmp and given back on the next step (Kahan), so that long
+// FP32 runs do not accumulate it
+template <typename T, int DIMS>
+__device__ __forceinline__ T euler_update_comp(const Params &p, T psi, T mu, const T (&m)[DIMS], const T (&q)[DIMS],
+                                               T &cmp) {
+#pragma clang fp contract(off)
+    const T inc = (T)p.Mdt * laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2);
+    const T y = inc - cmp;
+    const T t = psi + y;
+    cmp = (t - psi) - y;
+    return t;
+}
+
 // psi + dt div J with face fluxes J_d(i) = (M_i + M_{i+e_d})/2 (mu_{i+e_d} - mu_i)/dx
 // (EulerMobilityCPU.cpp:44-82, divergence EulerCPU.cpp:114-128)
 template <typename T, int DIMS>

@@ -91,9 +91,13 @@ __device__ __forceinline__ void row_neighbours(const T (&v)[C][RPW], int j, cons
 // which is the whole grid (WHOLE).
 // TAPE: `out` holds K consecutive fields and the state after every step is
 // stored (the tile interior is valid after each of the K steps).
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false>
+// COMP (Euler only): compensated updates, the lost low-order parts read from
+// cin and written to cout beside the field (euler_update_comp)
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false, bool COMP = false>
 __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restrict__ out, const Params &p, int gx0,
-                                            int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex) {
+                                            int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex,
+                                            const T *__restrict__ cin = nullptr, T *__restrict__ cout = nullptr) {
+    static_assert(!COMP || (!EM && !TAPE), "compensated updates for the Euler integrator only");
     constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
     constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
     constexpr int H = WHOLE ? 0 : 2 * K;
@@ -104,7 +108,7 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
     // there does not matter and the exchange needs no branch
     const int wa = (w + NW - 1) % NW, wb = (w + 1) % NW;
 
-    T psi[C][RPW], mu[C][RPW], mob[HASMOB ? C : 1][HASMOB ? RPW : 1];
+    T psi[C][RPW], mu[C][RPW], mob[HASMOB ? C : 1][HASMOB ? RPW : 1], cmp[COMP ? C : 1][COMP ? RPW : 1];
 #pragma unroll
     for(int j = 0; j < RPW; j++) {
         const size_t row = (size_t)((gy0 + w * RPW + j) & mask) * N;
@@ -113,6 +117,7 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
             psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
             mu[c][j] = 0;
             if constexpr(HASMOB) mob[c][j] = 0;
+            if constexpr(COMP) cmp[c][j] = cin[row + ((gx0 + C * lane + c) & mask)];
         }
     }
     // wave w computes a phase whose results are read up to d rows beyond the interior
@@ -132,7 +137,10 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
             for(int c = 0; c < C; c++) {
                 const int lx = C * lane + c;
                 const int gx = gx0 + lx;
-                if(WHOLE || (lx >= H && lx < RW - H && gx < N)) dst[(size_t)gy * N + gx] = psi[c][j];
+                if(WHOLE || (lx >= H && lx < RW - H && gx < N)) {
+                    dst[(size_t)gy * N + gx] = psi[c][j];
+                    if constexpr(COMP) cout[(size_t)gy * N + gx] = cmp[c][j];
+                }
             }
         }
     };
@@ -201,7 +209,10 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
                 else {
 #pragma unroll
                     for(int c = 0; c < C; c++) {
-                        if constexpr(!EM) {
+                        if constexpr(COMP) {
+                            psi[c][j] = euler_update_comp<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], cmp[c][j]);
+                        }
+                        else if constexpr(!EM) {
                             psi[c][j] = euler_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c]);
                         }
                         else {
