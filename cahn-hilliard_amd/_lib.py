@@ -123,6 +123,8 @@ def lib():
         "ch_sys_structure_factor": (ctypes.c_int, [P, P, I32]),
         "ch_sys_sq_loss": (ctypes.c_int, [P, P, I32, DD, P]),
         "ch_batch_create": (ctypes.c_int, [P, I32, DP, PP]),
+        "ch_batch_create_ex": (ctypes.c_int, [P, I32, DP, I32, PP]),
+        "ch_batch_compensation": (I32, [P]),
         "ch_batch_destroy": (None, [P]),
         "ch_batch_upload": (ctypes.c_int, [P, I32, P]),
         "ch_batch_download": (ctypes.c_int, [P, I32, P]),

@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "ch_internal.h"
 
@@ -220,6 +222,8 @@ static chd::Params make_params(const ch_desc *d) {
     p.inv_dx2 = 1.0 / (d->dx * d->dx);
     p.dt = d->dt;
     p.Mdt = d->M * d->dt;
+    p.mu_lap = -2.0 * d->k / (d->dx * d->dx);
+    p.upd_lap = d->M * d->dt / (d->dx * d->dx);
     {
         const double q = 2.0 * 3.14159265358979323846 / (d->N * d->dx);
         p.q2_unit = q * q;
@@ -1125,9 +1129,11 @@ struct ch_batch {
     int2 *items[chd::kBatchWidths][kBatchDepths] = {};
     int nitems[chd::kBatchWidths][kBatchDepths] = {};
     void *buf[2] = {nullptr, nullptr};
-    // FP32 with compensated updates (CH_BATCH_COMP=1): the low-order parts lost
-    // by each replica's updates, ping-pong beside buf
+    int compensation = CH_COMP_NONE;  // CH_COMP_* (FP32 only)
+    // CH_COMP_STORED / CH_COMP_LOW8: the low-order parts of each replica's
+    // field (FP32 or int8 per element), ping-pong beside buf
     void *comp[2] = {nullptr, nullptr};
+    size_t comp_elt() const { return compensation == CH_COMP_LOW8 ? 1 : elt; }
     int cur = 0;
     void *stage = nullptr;                   // largest replica, doubles
     double *result = nullptr;                // 2 n doubles
@@ -1135,6 +1141,56 @@ struct ch_batch {
 };
 
 static int log2i(int k) { return k >= 8 ? 3 : (k >= 4 ? 2 : (k >= 2 ? 1 : 0)); }
+
+// Autotuned (shape, steps per launch) per batch composition: device,
+// precision, model, compensation and the number of replicas at each N.  Kept
+// for the process, and in the file named by CH_TUNE_CACHE when set (one line
+// "key shape K" per entry, read on first use, appended after each tuning), so
+// a batch of a composition already tuned starts at once.
+static std::mutex g_tune_mu;
+static std::map<std::string, std::pair<int, int>> g_tune;
+static bool g_tune_loaded = false;
+
+static std::string tune_key(const ch_batch *b) {
+    std::map<int, int> count;
+    for(const chd::BatchRep &r : b->host_reps) count[r.p.N]++;
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "dev%d/p%d/m%d/c%d", b->ctx->device, b->precision, b->model, b->compensation);
+    std::string k = buf;
+    for(const auto &e : count) k += "/" + std::to_string(e.first) + "x" + std::to_string(e.second);
+    return k;
+}
+
+static bool tune_lookup(const std::string &key, int &shape, int &K) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if(!g_tune_loaded) {
+        g_tune_loaded = true;
+        if(const char *path = std::getenv("CH_TUNE_CACHE")) {
+            if(FILE *f = std::fopen(path, "r")) {
+                char k[512];
+                int sh, kk;
+                while(std::fscanf(f, "%511s %d %d", k, &sh, &kk) == 3)
+                    if(sh >= 0 && sh < chd::kBatchShapes && (kk == 1 || kk == 2 || kk == 4 || kk == 8)) g_tune[k] = {sh, kk};
+                std::fclose(f);
+            }
+        }
+    }
+    auto it = g_tune.find(key);
+    if(it == g_tune.end()) return false;
+    shape = it->second.first;
+    K = it->second.second;
+    return true;
+}
+
+static void tune_store(const std::string &key, int shape, int K) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = {shape, K};
+    if(const char *path = std::getenv("CH_TUNE_CACHE"))
+        if(FILE *f = std::fopen(path, "a")) {
+            std::fprintf(f, "%s %d %d\n", key.c_str(), shape, K);
+            std::fclose(f);
+        }
+}
 
 static void free_batch(ch_batch *b) {
     for(int i = 0; i < 2; i++) {
@@ -1153,7 +1209,19 @@ static void free_batch(ch_batch *b) {
 
 extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *descs, ch_batch **out) {
     if(!ctx || !out || !descs || n < 1) return CH_EINVAL;
+    return ch_batch_create_ex(ctx, n, descs, descs[0].precision == CH_FP32 ? CH_COMP_LAUNCH : CH_COMP_NONE, out);
+}
+
+extern "C" int32_t ch_batch_compensation(const ch_batch *b) { return b ? b->compensation : -1; }
+
+extern "C" ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *descs, int32_t compensation,
+                                        ch_batch **out) {
+    if(!ctx || !out || !descs || n < 1) return CH_EINVAL;
     *out = nullptr;
+    if(compensation < CH_COMP_NONE || compensation > CH_COMP_LOW8)
+        return fail(ctx, CH_EINVAL, "unknown batch compensation %d", compensation);
+    if(descs[0].precision == CH_FP64 && compensation != CH_COMP_NONE)
+        return fail(ctx, CH_EINVAL, "FP64 batches take no compensation (CH_COMP_NONE)");
     for(int r = 0; r < n; r++) {
         ch_status st = validate(ctx, &descs[r]);
         if(st != CH_OK) return fail(ctx, st, "replica %d: %s", r, ctx->err.c_str());
@@ -1170,6 +1238,7 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     b->precision = descs[0].precision;
     b->model = descs[0].model;
     b->elt = b->precision == CH_FP64 ? 8 : 4;
+    b->compensation = compensation;
     // CH_BATCH_TB / CH_BATCH_SHAPE pin the configuration (no autotuning)
     const char *etb = std::getenv("CH_BATCH_TB"), *esh = std::getenv("CH_BATCH_SHAPE");
     if(etb || esh) {
@@ -1207,15 +1276,15 @@ extern "C" ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *desc
     const char *g = std::getenv("CH_GRAPH_STEPS");
     b->graph_steps = g ? std::atoi(g) : 64;
     if(b->graph_steps < 0) b->graph_steps = 0;
+    if(!b->tuned && tune_lookup(tune_key(b), b->shape, b->tb)) b->tuned = true;
     bool ok = hipMalloc(&b->buf[0], b->total * b->elt) == hipSuccess && hipMalloc(&b->buf[1], b->total * b->elt) == hipSuccess &&
               hipMalloc((void **)&b->reps, n * sizeof(chd::BatchRep)) == hipSuccess &&
               hipMalloc(&b->stage, maxbins * 8) == hipSuccess && hipMalloc((void **)&b->result, 2 * n * sizeof(double)) == hipSuccess;
-    const char *ecomp = std::getenv("CH_BATCH_COMP");
-    if(ok && b->precision == CH_FP32 && ecomp && std::atoi(ecomp) != 0)
-        ok = hipMalloc(&b->comp[0], b->total * b->elt) == hipSuccess &&
-             hipMalloc(&b->comp[1], b->total * b->elt) == hipSuccess &&
-             hipMemset(b->comp[0], 0, b->total * b->elt) == hipSuccess &&
-             hipMemset(b->comp[1], 0, b->total * b->elt) == hipSuccess;
+    if(ok && (compensation == CH_COMP_STORED || compensation == CH_COMP_LOW8)) {
+        const size_t bytes = b->total * b->comp_elt();
+        ok = hipMalloc(&b->comp[0], bytes) == hipSuccess && hipMalloc(&b->comp[1], bytes) == hipSuccess &&
+             hipMemset(b->comp[0], 0, bytes) == hipSuccess && hipMemset(b->comp[1], 0, bytes) == hipSuccess;
+    }
     for(int fam = 0; fam < chd::kBatchWidths; fam++)
         for(int i = 0; ok && i < kBatchDepths; i++) {
             const std::vector<int2> &v = items[fam][i];
@@ -1249,7 +1318,8 @@ extern "C" ch_status ch_batch_upload(ch_batch *b, int32_t r, const double *rho_h
     const int64_t bins = (int64_t)rep.p.N * rep.p.N;
     char *dst = (char *)b->buf[b->cur] + rep.off * b->elt;
     // a new state: nothing of the replica's earlier updates is owed to it
-    if(b->comp[0]) HIPCHK(c, hipMemsetAsync((char *)b->comp[b->cur] + rep.off * b->elt, 0, bins * b->elt, c->stream));
+    if(b->comp[0])
+        HIPCHK(c, hipMemsetAsync((char *)b->comp[b->cur] + rep.off * b->comp_elt(), 0, bins * b->comp_elt(), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if(b->precision == CH_FP64) {
         HIPCHK(c, hipMemcpyAsync(dst, rho_host, bins * 8, hipMemcpyHostToDevice, c->stream));
@@ -1282,7 +1352,7 @@ extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host)
 
 static hipError_t batch_launch(ch_batch *b, int shape, int K, int from) {
     const int i = log2i(K), fam = chd::batch_columns_per_lane(shape) - 1;
-    return chd::batch_launch(b->precision, b->model, shape, K, b->buf[from], b->buf[from ^ 1], b->reps,
+    return chd::batch_launch(b->precision, b->model, b->compensation, shape, K, b->buf[from], b->buf[from ^ 1], b->reps,
                              b->items[fam][i], b->nitems[fam][i], b->comp[from], b->comp[from ^ 1], b->ctx->stream);
 }
 
@@ -1304,8 +1374,12 @@ static ch_status batch_tune(ch_batch *b) {
     float best = 1e30f;
     int bs = b->shape, bk = b->tb;
     hipError_t err = hipSuccess;
+    // a compensated batch rounds psi once per launch, so its accuracy
+    // depends on K: its candidates keep K >= 4
+    const bool comp = b->compensation != CH_COMP_NONE;
     for(int shape = 0; shape < chd::kBatchShapes && err == hipSuccess; shape++) {
         for(int K : {2, 4, 8}) {
+            if(comp && K < 4) continue;
             constexpr int warm = 2, reps = 8, runs = 3;
             for(int r = 0; r < warm && err == hipSuccess; r++) err = batch_launch(b, shape, K, b->cur);
             float ms = 1e30f;
@@ -1340,6 +1414,7 @@ static ch_status batch_tune(ch_batch *b) {
     b->shape = bs;
     b->tb = bk;
     b->tuned = true;
+    tune_store(tune_key(b), bs, bk);
     return CH_OK;
 }
 

@@ -27,6 +27,7 @@ struct Params {
     double M, rho_min, rho_ref;     // mobility
     double k, two_k, dx, inv_dx, inv_dx2, dt, S;
     double Mdt;                     // M dt (constant-mobility Euler update)
+    double mu_lap, upd_lap;         // -2 k / dx^2 and M dt / dx^2 (folded FP32 cell arithmetic)
     double q2_unit;                 // (2 pi / (N dx))^2: q^2 of a unit wave vector
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
@@ -57,7 +58,10 @@ template <typename T, int MODEL>
 __device__ __forceinline__ T fprime(const Params &p, T r) {
 #pragma clang fp contract(off)
     if constexpr(MODEL == CH_LANDAU) {
-        return fma_t(r * r, r, -(T)p.eps * r);
+        // FP32: psi (psi^2 - eps), two operations; FP64 keeps the reference's
+        // -eps psi + psi^3
+        if constexpr(sizeof(T) == 4) return r * fma_t(r, r, -(T)p.eps);
+        else return fma_t(r * r, r, -(T)p.eps * r);
     }
     else {
         const double rd = (double)r;
@@ -145,18 +149,42 @@ __device__ __forceinline__ T laplacian(T c, const T (&m)[DIMS], const T (&q)[DIM
     return fma_t((T)(-2 * DIMS), c, s) * inv_dx2;
 }
 
+// The Laplacian without its 1 / dx^2: the neighbour sum minus 2 d times the
+// centre.  FP32 cells fold 1 / dx^2 into the constant that multiplies it
+// (-2 k / dx^2 for mu, M dt / dx^2 for the update): one multiplication less
+// per Laplacian; FP64 cells keep the reference's grouping.
+template <typename T, int DIMS>
+__device__ __forceinline__ T laplacian_sum(T c, const T (&m)[DIMS], const T (&q)[DIMS]) {
+#pragma clang fp contract(off)
+    T s = m[0] + q[0];
+#pragma unroll
+    for(int d = 1; d < DIMS; d++) {
+        s = s + m[d];
+        s = s + q[d];
+    }
+    return fma_t((T)(-2 * DIMS), c, s);
+}
+
 // mu = f'(psi) - 2 k lap(psi)  (EulerCPU.cpp:20-26)
 template <typename T, int DIMS, int MODEL>
 __device__ __forceinline__ T chem_pot(const Params &p, T c, const T (&m)[DIMS], const T (&q)[DIMS]) {
 #pragma clang fp contract(off)
-    return fma_t(-(T)p.two_k, laplacian<T, DIMS>(c, m, q, (T)p.inv_dx2), fprime<T, MODEL>(p, c));
+    if constexpr(sizeof(T) == 4) return fma_t((T)p.mu_lap, laplacian_sum<T, DIMS>(c, m, q), fprime<T, MODEL>(p, c));
+    else return fma_t(-(T)p.two_k, laplacian<T, DIMS>(c, m, q, (T)p.inv_dx2), fprime<T, MODEL>(p, c));
+}
+
+// dt M lap(mu), the increment of the constant-mobility Euler update
+template <typename T, int DIMS>
+__device__ __forceinline__ T euler_inc(const Params &p, T mu, const T (&m)[DIMS], const T (&q)[DIMS], T base) {
+#pragma clang fp contract(off)
+    if constexpr(sizeof(T) == 4) return fma_t((T)p.upd_lap, laplacian_sum<T, DIMS>(mu, m, q), base);
+    else return fma_t((T)p.Mdt, laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2), base);
 }
 
 // psi + dt M lap(mu)  (EulerCPU.cpp:29-35), constant mobility
 template <typename T, int DIMS>
 __device__ __forceinline__ T euler_update(const Params &p, T psi, T mu, const T (&m)[DIMS], const T (&q)[DIMS]) {
-#pragma clang fp contract(off)
-    return fma_t((T)p.Mdt, laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2), psi);
+    return euler_inc<T, DIMS>(p, mu, m, q, psi);
 }
 
 // the same update as a compensated sum: the FP32 rounding lost in psi + dt M
@@ -166,11 +194,30 @@ template <typename T, int DIMS>
 __device__ __forceinline__ T euler_update_comp(const Params &p, T psi, T mu, const T (&m)[DIMS], const T (&q)[DIMS],
                                                T &cmp) {
 #pragma clang fp contract(off)
-    const T inc = (T)p.Mdt * laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2);
-    const T y = inc - cmp;
+    const T y = euler_inc<T, DIMS>(p, mu, m, q, -cmp);
     const T t = psi + y;
     cmp = (t - psi) - y;
     return t;
+}
+
+// the update of a launch-compensated step: the increments of the launch so far
+// are summed in acc (small beside psi, so their sum keeps nearly all their
+// bits) and the field is psi0 + acc, psi0 being the value the launch loaded;
+// psi is rounded once from the launch's total increment instead of once per step
+template <typename T, int DIMS>
+__device__ __forceinline__ void euler_increment(const Params &p, T mu, const T (&m)[DIMS], const T (&q)[DIMS], T &acc) {
+    acc = euler_inc<T, DIMS>(p, mu, m, q, acc);
+}
+
+// An 8-bit low-order part of an FP32 value hi: lo = q 2^(E - 32) where
+// hi = m 2^E with 0.5 <= |m| < 1 (frexp), i.e. q in units of ulp(hi) / 256.
+// encode: the q nearest to a remainder e with |e| <= ulp(hi) / 2 (|q| <= 127).
+__device__ __forceinline__ float low8_decode(float hi, int q) {
+    return __builtin_ldexpf((float)q, __builtin_amdgcn_frexp_expf(hi) - 32);
+}
+__device__ __forceinline__ int low8_encode(float hi, float e) {
+    const int q = (int)__builtin_rintf(__builtin_ldexpf(e, 32 - __builtin_amdgcn_frexp_expf(hi)));
+    return q < -127 ? -127 : (q > 127 ? 127 : q);
 }
 
 // psi + dt div J with face fluxes J_d(i) = (M_i + M_{i+e_d})/2 (mu_{i+e_d} - mu_i)/dx

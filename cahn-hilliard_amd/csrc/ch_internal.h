@@ -137,8 +137,8 @@ constexpr int kBatchWidths = 3;
 int batch_columns_per_lane(int shape);
 int batch_tile_x(int shape, int K);
 int batch_tile_y(int K);
-hipError_t batch_launch(int precision, int model, int shape, int K, const void *a, void *b, const BatchRep *reps,
-                        const int2 *items, int nitems, const void *ca, void *cb, hipStream_t s);
+hipError_t batch_launch(int precision, int model, int upd, int shape, int K, const void *a, void *b,
+                        const BatchRep *reps, const int2 *items, int nitems, const void *ca, void *cb, hipStream_t s);
 // per-replica FP64 observables: result[2 r] = sum rho, result[2 r + 1] = sum f-density
 hipError_t batch_observables(int precision, int model, const void *buf, const BatchRep *reps, int n, double *result,
                              hipStream_t s);

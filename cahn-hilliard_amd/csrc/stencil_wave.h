@@ -91,13 +91,28 @@ __device__ __forceinline__ void row_neighbours(const T (&v)[C][RPW], int j, cons
 // which is the whole grid (WHOLE).
 // TAPE: `out` holds K consecutive fields and the state after every step is
 // stored (the tile interior is valid after each of the K steps).
-// COMP (Euler only): compensated updates, the lost low-order parts read from
-// cin and written to cout beside the field (euler_update_comp)
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false, bool COMP = false>
+// UPD (Euler only) selects how the update psi + dt M lap(mu) is rounded:
+//   UPD_PLAIN   rounded every step
+//   UPD_LAUNCH  the launch keeps each cell's loaded value psi0 and the sum of
+//               its increments in registers (euler_increment); the field used
+//               for mu is psi0 + sum, so the only rounding of psi that stays
+//               is the one at the store: one per launch instead of one per
+//               step, at one extra add per cell and step and no extra traffic
+//   UPD_STORED  compensated sums whose lost low-order parts are read from cin
+//               and written to cout beside the field (euler_update_comp)
+//   UPD_LOW8    UPD_LAUNCH over a field that carries an 8-bit low-order part
+//               beside psi (low8_decode / low8_encode, FP32 only): the launch
+//               starts its increment sum from the stored part, and the store
+//               keeps the rounding remainder of psi in it, so the field holds
+//               about 32 significant bits and no rounding accumulates
+enum { UPD_PLAIN = 0, UPD_LAUNCH = 1, UPD_STORED = 2, UPD_LOW8 = 3 };
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false, int UPD = UPD_PLAIN>
 __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restrict__ out, const Params &p, int gx0,
                                             int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex,
-                                            const T *__restrict__ cin = nullptr, T *__restrict__ cout = nullptr) {
-    static_assert(!COMP || (!EM && !TAPE), "compensated updates for the Euler integrator only");
+                                            const void *__restrict__ cin = nullptr, void *__restrict__ cout = nullptr) {
+    static_assert(UPD == UPD_PLAIN || (!EM && !TAPE), "compensated updates for the Euler integrator only");
+    static_assert(UPD != UPD_LOW8 || sizeof(T) == 4, "8-bit low-order parts of FP32 fields only");
+    constexpr bool COMP = UPD == UPD_STORED, LOW8 = UPD == UPD_LOW8, ACC = UPD == UPD_LAUNCH || LOW8;
     constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
     constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
     constexpr int H = WHOLE ? 0 : 2 * K;
@@ -109,15 +124,22 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
     const int wa = (w + NW - 1) % NW, wb = (w + 1) % NW;
 
     T psi[C][RPW], mu[C][RPW], mob[HASMOB ? C : 1][HASMOB ? RPW : 1], cmp[COMP ? C : 1][COMP ? RPW : 1];
+    T psi0[ACC ? C : 1][ACC ? RPW : 1], acc[ACC ? C : 1][ACC ? RPW : 1];
 #pragma unroll
     for(int j = 0; j < RPW; j++) {
         const size_t row = (size_t)((gy0 + w * RPW + j) & mask) * N;
 #pragma unroll
         for(int c = 0; c < C; c++) {
-            psi[c][j] = in[row + ((gx0 + C * lane + c) & mask)];
+            const size_t at = row + ((gx0 + C * lane + c) & mask);
+            psi[c][j] = in[at];
             mu[c][j] = 0;
             if constexpr(HASMOB) mob[c][j] = 0;
-            if constexpr(COMP) cmp[c][j] = cin[row + ((gx0 + C * lane + c) & mask)];
+            if constexpr(COMP) cmp[c][j] = static_cast<const T *>(cin)[at];
+            if constexpr(ACC) {
+                psi0[c][j] = psi[c][j];
+                if constexpr(LOW8) acc[c][j] = low8_decode(psi[c][j], static_cast<const int8_t *>(cin)[at]);
+                else acc[c][j] = 0;
+            }
         }
     }
     // wave w computes a phase whose results are read up to d rows beyond the interior
@@ -138,8 +160,20 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
                 const int lx = C * lane + c;
                 const int gx = gx0 + lx;
                 if(WHOLE || (lx >= H && lx < RW - H && gx < N)) {
-                    dst[(size_t)gy * N + gx] = psi[c][j];
-                    if constexpr(COMP) cout[(size_t)gy * N + gx] = cmp[c][j];
+                    const size_t at = (size_t)gy * N + gx;
+                    if constexpr(ACC) {
+                        const T v = psi0[c][j] + acc[c][j];
+                        dst[at] = v;
+                        // the remainder of the rounding (Fast2Sum: psi0 is the larger term
+                        // wherever the remainder is not negligible)
+                        if constexpr(LOW8)
+                            static_cast<int8_t *>(cout)[at] =
+                                (int8_t)low8_encode(v, acc[c][j] - (v - psi0[c][j]));
+                    }
+                    else {
+                        dst[at] = psi[c][j];
+                    }
+                    if constexpr(COMP) static_cast<T *>(cout)[at] = cmp[c][j];
                 }
             }
         }
@@ -149,6 +183,13 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
 #pragma unroll 1
     for(int s = 0; s < K; s++) {
         // ---- phase 1: mu (and M) from psi
+        if constexpr(ACC) {
+            // psi of this step from the launch's start value and increments
+#pragma unroll
+            for(int j = 0; j < RPW; j++)
+#pragma unroll
+                for(int c = 0; c < C; c++) psi[c][j] = psi0[c][j] + acc[c][j];
+        }
 #pragma unroll
         for(int c = 0; c < C; c++) {
             ex.first[0][w][c][lane] = psi[c][0];
@@ -211,6 +252,9 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
                     for(int c = 0; c < C; c++) {
                         if constexpr(COMP) {
                             psi[c][j] = euler_update_comp<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], cmp[c][j]);
+                        }
+                        else if constexpr(ACC) {
+                            euler_increment<T, 2>(p, mu[c][j], m[c], q[c], acc[c][j]);
                         }
                         else if constexpr(!EM) {
                             psi[c][j] = euler_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c]);

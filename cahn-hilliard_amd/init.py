@@ -47,3 +47,17 @@ def initial_density(dims, N, density=0.0, A=1e-2, N_peaks=0, seed=123456):
     if density != 0.0:
         return density * (1.0 + 2.0 * modulation * r)
     return 2.0 * modulation * r
+
+
+def replica_batch(sizes=(64, 128, 256, 512), per_size=64, seed=123456):
+    """The mixed-size Landau 2D replica workload: `per_size` replicas at each N
+    of `sizes`; replica r draws from drand48 seeded `seed` + r: eps ~ U[0.5, 1],
+    then c ~ U[-0.3, 0.3], then the cells psi0 = c + U[-0.01, 0.01] (x fastest).
+    Returns (Ns, eps, fields)."""
+    Ns, eps, fields = [], [], []
+    for i, N in enumerate([n for n in sizes for _ in range(per_size)]):
+        r = drand48_stream(seed + i, 2 + N * N)
+        Ns.append(N)
+        eps.append(0.5 + 0.5 * float(r[0]))
+        fields.append(-0.3 + 0.6 * r[1] + 0.02 * (r[2:] - 0.5))
+    return Ns, eps, fields

@@ -397,14 +397,24 @@ class System:
             pass
 
 
-class Batch:
-    """Independent 2D Euler replicas (mixed N and parameters) advanced together."""
+COMPENSATION = {"none": 0, "launch": 1, "stored": 2, "low8": 3}
 
-    def __init__(self, ctx, descs):
+
+class Batch:
+    """Independent 2D Euler replicas (mixed N and parameters) advanced together.
+
+    compensation: None for the library default (FP32: "launch", FP64: "none"),
+    or "none" / "launch" / "stored" / "low8" (ch_batch_create_ex)."""
+
+    def __init__(self, ctx, descs, compensation=None):
         self.ctx = ctx
         arr = (L.ChDesc * len(descs))(*descs)
         h = ctypes.c_void_p()
-        _check(ctx.handle, L.lib().ch_batch_create(ctx.handle, len(descs), arr, ctypes.byref(h)))
+        if compensation is None:
+            st = L.lib().ch_batch_create(ctx.handle, len(descs), arr, ctypes.byref(h))
+        else:
+            st = L.lib().ch_batch_create_ex(ctx.handle, len(descs), arr, COMPENSATION[compensation], ctypes.byref(h))
+        _check(ctx.handle, st)
         self.handle, self.n = h, len(descs)
         ctx._children.add(self)
         self.bins = [d.N ** d.dims for d in descs]
@@ -435,6 +445,11 @@ class Batch:
     @property
     def steps_per_launch(self):
         return int(L.lib().ch_batch_steps_per_launch(self.handle))
+
+    @property
+    def compensation(self):
+        c = int(L.lib().ch_batch_compensation(self.handle))
+        return {v: k for k, v in COMPENSATION.items()}[c]
 
     def autotune(self):
         """Time the kernel's region splits and launch depths on the current

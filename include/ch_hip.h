@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 6
+#define CH_ABI_VERSION 7
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -220,6 +220,28 @@ ch_status ch_sys_sq_loss(ch_sys *sys, const double *target, int32_t nshells, dou
  * Wertheim parameters), k, dt, dx and M; all share one model and precision.
  * One launch advances every replica by ch_batch_steps_per_launch steps. */
 ch_status ch_batch_create(ch_ctx *ctx, int32_t n, const ch_desc *descs, ch_batch **out);
+/* How an FP32 batch rounds its updates psi + dt M lap(mu).
+ *   CH_COMP_NONE    every step (the single-system FP32 arithmetic: each
+ *                   replica gets the bits of the same system run alone)
+ *   CH_COMP_LAUNCH  once per launch: a launch sums each cell's increments in
+ *                   registers and rounds psi once at its store (no extra
+ *                   memory traffic); the FP32 default of ch_batch_create.
+ *                   Results depend on the launch depth, not on the region split.
+ *   CH_COMP_STORED  every update is a compensated sum whose lost low-order
+ *                   parts are kept in a second FP32 field (twice the traffic);
+ *                   ch_batch_upload clears a replica's parts and
+ *                   ch_batch_download returns psi without them, so a download
+ *                   followed by an upload is not exact in this mode.
+ *   CH_COMP_LOW8    CH_COMP_LAUNCH over a field that keeps an 8-bit low-order
+ *                   part beside each FP32 value (ulp / 256 resolution: about
+ *                   32 significant bits, 2 more bytes per cell and launch): a
+ *                   launch starts from psi + its low-order part and stores the
+ *                   remainder of its final rounding there, so no rounding
+ *                   accumulates over launches.  Upload clears the parts.
+ * FP64 batches take CH_COMP_NONE only (CH_EINVAL otherwise). */
+enum { CH_COMP_NONE = 0, CH_COMP_LAUNCH = 1, CH_COMP_STORED = 2, CH_COMP_LOW8 = 3 };
+ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *descs, int32_t compensation, ch_batch **out);
+int32_t ch_batch_compensation(const ch_batch *batch);
 void ch_batch_destroy(ch_batch *batch);
 ch_status ch_batch_upload(ch_batch *batch, int32_t replica, const double *rho_host);
 ch_status ch_batch_download(ch_batch *batch, int32_t replica, double *rho_host);
@@ -232,10 +254,14 @@ int32_t ch_batch_steps_per_launch(const ch_batch *batch);
  * batch kernel (rows per wave x waves over 64-row regions: shape 0: 16 x 4,
  * 1: 8 x 8, 2: 4 x 16 with 64 columns; 3: 4 x 16, 4: 8 x 8 with 128 columns,
  * two per lane; 5: 4 x 16, 6: 8 x 8 with 192 columns, three per lane) at 2, 4
- * and 8 steps per launch on the batch's current fields and
- * keeps the fastest per time step; the fields are not changed.  Runs by
- * itself on the first ch_batch_evolve unless the environment pins the choice
- * (CH_BATCH_TB, CH_BATCH_SHAPE).  Synchronous; either output may be NULL. */
+ * and 8 steps per launch (4 and 8 for a compensated batch, whose accuracy
+ * depends on the depth) on the batch's current fields and keeps the fastest
+ * per time step; the fields are not changed.  Runs by itself on the first
+ * ch_batch_evolve unless the environment pins the choice (CH_BATCH_TB,
+ * CH_BATCH_SHAPE) or a batch of the same composition (device, precision,
+ * model, compensation, replicas per N) was tuned before in this process or
+ * is listed in the file named by CH_TUNE_CACHE, to which each tuning appends
+ * its result.  Synchronous; either output may be NULL. */
 ch_status ch_batch_autotune(ch_batch *batch, int32_t *shape, int32_t *steps_per_launch);
 
 /* ---- reverse mode (no reference counterpart: the reference has no

@@ -1,8 +1,11 @@
-"""Batched replicas (ch_batch_*): every replica of a mixed-size batch evolves
-exactly as the same system run on its own (bit-identical fields: the batch
-kernel shares the per-cell arithmetic), FP64 replicas match the oracle to
-1e-10, FP32 replicas stay within 1e-4 (relative L-inf, 200 steps) of FP64,
-and the per-replica FP64 observables match the oracle."""
+"""Batched replicas (ch_batch_*): with plain rounding every replica of a
+mixed-size batch evolves exactly as the same system run on its own
+(bit-identical fields: the batch kernel shares the per-cell arithmetic), FP64
+replicas match the oracle to 1e-10, FP32 replicas stay within 1e-4 (relative
+L-inf, 200 steps) of FP64, and the per-replica FP64 observables match the
+oracle.  The compensated FP32 modes give the same bits for every region split,
+and the default one keeps the 256-replica workload within 1e-5 of FP64 over
+its 5000 steps."""
 import numpy as np
 import pytest
 
@@ -33,8 +36,9 @@ def make_replicas(ch, precision, Ns, seed=0):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
-    """Every region split (CH_BATCH_SHAPE) and launch depth (CH_BATCH_TB) gives
-    each replica the bits of the same system run alone."""
+    """Plain rounding (CH_COMP_NONE): every region split (CH_BATCH_SHAPE) and
+    launch depth (CH_BATCH_TB) gives each replica the bits of the same system
+    run alone."""
     Ns = [64, 128, 64, 256, 128, 512, 64]
     descs, fields = make_replicas(ch, precision, Ns)
     for shape, tb in (("0", "1"), ("0", "2"), ("0", "4"), ("1", "4"), ("2", "4"), ("1", "8"), ("2", "8"),
@@ -42,7 +46,7 @@ def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
                       ("5", "8"), ("6", "1"), ("6", "4")):
         monkeypatch.setenv("CH_BATCH_TB", tb)
         monkeypatch.setenv("CH_BATCH_SHAPE", shape)
-        b = ch.Batch(ctx, descs)
+        b = ch.Batch(ctx, descs, compensation="none")
         assert b.steps_per_launch == int(tb)
         for r, f in enumerate(fields):
             b.upload(r, f)
@@ -92,12 +96,15 @@ def test_batch_rejects_unsupported(ch, ctx):
         ch.Batch(ctx, [ch.make_desc(dims=3, N=64)])
 
 
-def test_autotune_keeps_state_and_results(ch, ctx):
+def test_autotune_keeps_state_and_results(ch, ctx, tmp_path, monkeypatch):
     """The autotuner's trial launches leave the fields unchanged, it picks a
-    valid configuration, and the tuned batch gives the single-system bits."""
+    valid configuration, the tuned batch gives the single-system bits, and the
+    result is recorded (CH_TUNE_CACHE) for the next batch of the composition."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("CH_TUNE_CACHE", str(cache))
     Ns = [64, 256, 128]
     descs, fields = make_replicas(ch, "fp32", Ns, seed=4)
-    b = ch.Batch(ctx, descs)
+    b = ch.Batch(ctx, descs, compensation="none")
     for r, f in enumerate(fields):
         b.upload(r, f)
     shape, k = b.autotune()
@@ -114,21 +121,85 @@ def test_autotune_keeps_state_and_results(ch, ctx):
         s.evolve(45)
         assert np.array_equal(outs[r], s.download())
         s.close()
+    lines = cache.read_text().split("\n")
+    assert any(ln.endswith(f" {shape} {k}") and "/c0/" in ln and "/64x1/128x1/256x1" in ln for ln in lines), lines
+    b2 = ch.Batch(ctx, descs, compensation="none")
+    assert b2.steps_per_launch == k
+    b2.close()
 
 
-def c3_errors(ch, ctx, marks=(1000, 5000)):
-    """C3: 256 Landau replicas (64 each at N = 64, 128, 256, 512), eps ~ U[0.5, 1],
-    psi0 ~ U[-0.3, 0.3] + U[-0.01, 0.01]; the worst replica's FP32 relative L-inf
-    error against the FP64 batch after each of `marks` steps."""
-    rng = np.random.default_rng(123456)
-    Ns = [64] * 64 + [128] * 64 + [256] * 64 + [512] * 64
-    eps = 0.5 + 0.5 * rng.random(len(Ns))
+def test_compensation_modes(ch, ctx):
+    """FP32 batches default to per-launch compensation, FP64 batches take none;
+    invalid requests are refused."""
+    d32 = [ch.make_desc(dims=2, N=64, precision="fp32"), ch.make_desc(dims=2, N=128, precision="fp32")]
+    d64 = [ch.make_desc(dims=2, N=64), ch.make_desc(dims=2, N=128)]
+    b = ch.Batch(ctx, d32)
+    assert b.compensation == "launch" and b.steps_per_launch >= 4
+    b.close()
+    b = ch.Batch(ctx, d64)
+    assert b.compensation == "none"
+    b.close()
+    for mode in ("launch", "stored", "low8"):
+        with pytest.raises(ch.CHError):
+            ch.Batch(ctx, d64, compensation=mode)
+    import ctypes
+    from cahn_hilliard_amd import _lib as L
+    arr = (L.ChDesc * 2)(*d32)
+    h = ctypes.c_void_p()
+    assert L.lib().ch_batch_create_ex(ctx.handle, 2, arr, 7, ctypes.byref(h)) == 1  # CH_EINVAL
+
+
+@pytest.mark.parametrize("mode", ["launch", "low8", "stored"])
+def test_compensated_split_invariance(ch, ctx, monkeypatch, mode):
+    """At a fixed launch depth every region split gives the same bits (the
+    compensation is a per-cell computation), and after 141 steps the FP32
+    fields are within 1e-6 of FP64.  The stored mode does not depend on the
+    depth either."""
+    rng = np.random.default_rng(5)
+    Ns = [64, 128, 64, 256, 128, 512, 64]
+    eps = rng.uniform(0.5, 1.0, len(Ns))
     fields = [rng.uniform(-0.3, 0.3) + rng.uniform(-0.01, 0.01, N * N) for N in Ns]
+    d64 = [ch.make_desc(dims=2, N=N, integrator="euler", dt=0.01, epsilon=float(e)) for N, e in zip(Ns, eps)]
+    d32 = [ch.make_desc(dims=2, N=N, integrator="euler", precision="fp32", dt=0.01, epsilon=float(e))
+           for N, e in zip(Ns, eps)]
+    b = ch.Batch(ctx, d64)
+    for r, f in enumerate(fields):
+        b.upload(r, f)
+    b.evolve(141)
+    ref = [b.download(r) for r in range(len(Ns))]
+    b.close()
     outs = {}
-    for prec in ("fp64", "fp32"):
-        descs = [ch.make_desc(dims=2, N=N, integrator="euler", precision=prec, dt=0.01, dx=1.0, k=1.0,
-                              epsilon=float(e)) for N, e in zip(Ns, eps)]
-        b = ch.Batch(ctx, descs)
+    for tb in ("4", "8"):
+        for shape in ("0", "1", "2", "3", "4", "5", "6"):
+            monkeypatch.setenv("CH_BATCH_SHAPE", shape)
+            monkeypatch.setenv("CH_BATCH_TB", tb)
+            b = ch.Batch(ctx, d32, compensation=mode)
+            assert b.compensation == mode and b.steps_per_launch == int(tb)
+            for r, f in enumerate(fields):
+                b.upload(r, f)
+            b.evolve(77)
+            b.evolve(64)
+            outs[shape, tb] = [b.download(r) for r in range(len(Ns))]
+            b.close()
+    for (shape, tb), v in outs.items():
+        base = outs["0", "4" if mode == "stored" else tb]
+        assert all(np.array_equal(a, c) for a, c in zip(v, base)), (mode, shape, tb)
+        err = max(rel_linf(a, c) for a, c in zip(v, ref))
+        assert err < 1e-6, (mode, shape, tb, err)
+
+
+def c3_errors(ch, ctx, compensation=None, marks=(1000, 5000)):
+    """The 256-replica Landau workload (64 each at N = 64, 128, 256, 512; per
+    replica drand48 seeded 123456 + r: eps ~ U[0.5, 1], psi0 ~ c + U[-0.01, 0.01]
+    with c ~ U[-0.3, 0.3]): the worst replica's FP32 relative L-inf error
+    against the FP64 batch after each of `marks` steps."""
+    from cahn_hilliard_amd.init import replica_batch
+    Ns, eps, fields = replica_batch()
+    outs = {}
+    for prec, comp in (("fp64", None), ("fp32", compensation)):
+        descs = [ch.make_desc(dims=2, N=N, integrator="euler", precision=prec, dt=0.01, dx=1.0, k=1.0, epsilon=e)
+                 for N, e in zip(Ns, eps)]
+        b = ch.Batch(ctx, descs, compensation=comp)
         for r, f in enumerate(fields):
             b.upload(r, f)
         done = 0
@@ -137,45 +208,24 @@ def c3_errors(ch, ctx, marks=(1000, 5000)):
             done = t
             outs[prec, t] = [b.download(r) for r in range(len(Ns))]
         b.close()
-    return {t: max(float(np.abs(a - c).max() / np.abs(c).max()) for a, c in zip(outs["fp32", t], outs["fp64", t]))
-            for t in marks}
+    return {t: max(rel_linf(a, c) for a, c in zip(outs["fp32", t], outs["fp64", t])) for t in marks}
 
 
-def test_c3_full_length_fp32_error(ch, ctx):
-    """C3, plain FP32: within 1e-5 of the FP64 batch after 1000 steps; after the
-    full 5000 steps the FP32 rounding of each update has accumulated (measured
-    5.4e-5), bound here at 1e-4."""
-    errs = c3_errors(ch, ctx)
-    print(f"C3: fp32 rel L-inf vs fp64 = {errs[1000]:.3e} after 1000 steps, {errs[5000]:.3e} after 5000")
-    assert errs[1000] < 1e-5
-    assert errs[5000] < 1e-4
+@pytest.mark.parametrize("mode, bound", [(None, 1e-5), ("low8", 3e-6), ("stored", 3e-6)])
+def test_c3_full_length_fp32_error(ch, ctx, mode, bound):
+    """The 256-replica workload over its full 5000 steps: the default FP32 batch
+    (per-launch compensation) within 1e-5 of FP64 (measured 9.9e-6), the 8-bit
+    low-order and the stored-compensation modes within 3e-6 (1.4e-6)."""
+    errs = c3_errors(ch, ctx, mode)
+    print(f"C3 {mode or 'default'}: fp32 rel L-inf vs fp64 = {errs[1000]:.3e} after 1000 steps, "
+          f"{errs[5000]:.3e} after 5000")
+    assert errs[1000] < 1e-6
+    assert errs[5000] < bound
 
 
-def test_compensated_fp32_batch(ch, ctx, monkeypatch):
-    """CH_BATCH_COMP=1: FP32 updates as compensated sums.  Every region split and
-    launch depth gives the same bits, and C3 stays within 1e-5 of the FP64
-    batch over the full 5000 steps (measured 1.6e-6)."""
-    monkeypatch.setenv("CH_BATCH_COMP", "1")
-    rng = np.random.default_rng(5)
-    Ns = [64, 128, 64, 256, 128, 512, 64]
-    descs = [ch.make_desc(dims=2, N=N, integrator="euler", precision="fp32", dt=0.01, epsilon=0.7) for N in Ns]
-    fields = [rng.uniform(-0.3, 0.3) + rng.uniform(-0.01, 0.01, N * N) for N in Ns]
-    outs = {}
-    for shape, tb in (("0", "1"), ("0", "4"), ("2", "8"), ("4", "4"), ("6", "2"), ("5", "8")):
-        monkeypatch.setenv("CH_BATCH_SHAPE", shape)
-        monkeypatch.setenv("CH_BATCH_TB", tb)
-        b = ch.Batch(ctx, descs)
-        for r, f in enumerate(fields):
-            b.upload(r, f)
-        b.evolve(77)
-        b.evolve(64)
-        outs[shape, tb] = [b.download(r) for r in range(len(Ns))]
-        b.close()
-    ref = outs["0", "1"]
-    for key, v in outs.items():
-        assert all(np.array_equal(a, c) for a, c in zip(v, ref)), key
-    monkeypatch.delenv("CH_BATCH_SHAPE")
-    monkeypatch.delenv("CH_BATCH_TB")
-    errs = c3_errors(ch, ctx)
-    print(f"C3 compensated: fp32 rel L-inf vs fp64 = {errs[1000]:.3e} after 1000 steps, {errs[5000]:.3e} after 5000")
-    assert errs[1000] < 1e-5 and errs[5000] < 1e-5
+@pytest.mark.xfail(strict=True, reason="plain FP32 rounding of every update accumulates to 5.6e-5 over the "
+                                       "5000 steps (the compensated modes exist for this)")
+def test_c3_plain_fp32_error(ch, ctx):
+    errs = c3_errors(ch, ctx, "none")
+    print(f"C3 plain: fp32 rel L-inf vs fp64 = {errs[1000]:.3e} after 1000 steps, {errs[5000]:.3e} after 5000")
+    assert errs[5000] < 1e-5
