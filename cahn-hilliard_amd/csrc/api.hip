@@ -127,7 +127,9 @@ extern "C" ch_status ch_ctx_create(int device, ch_ctx **out) {
     if(device < 0 || device >= n) return CH_EINVAL;
     ch_ctx *c = new ch_ctx();
     c->device = device;
-    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    // the scratch address of ch_observables_device_noalloc, once per device
+    if(hipSetDevice(device) != hipSuccess || chd::noalloc_scratch_init(device) != hipSuccess ||
+       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return CH_EHIP;
     }
@@ -976,6 +978,38 @@ static ch_status observables_result(ch_sys *s, const double *pair, double *mass,
     if(free_energy) *free_energy = f;
     if(std::isnan(h[0])) return fail(c, CH_ENAN, "Encountered a nan while computing the total mass");
     if(std::isnan(h[1])) return fail(c, CH_ENAN, "Encountered a nan while computing the total free energy");
+    return CH_OK;
+}
+
+// The reductions of ch_sys_observables on a caller-held device field, with
+// no allocation: the partial sums go to the library's fixed per-device
+// scratch array (reduce.hip), the final pair is copied to the stack.
+extern "C" ch_status ch_observables_device_noalloc(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev,
+                                                   double *mass, double *free_energy) {
+    if(!ctx) return CH_EINVAL;
+    if(!desc || !rho_dev) return fail(ctx, CH_EINVAL, "null descriptor or field");
+    ch_status st = validate(ctx, desc);
+    if(st != CH_OK) return st;
+    if(desc->model != CH_LANDAU && desc->model != CH_SIMPLE_WERTHEIM)
+        return fail(ctx, CH_EUNSUPPORTED, "allocation-free observables take single-species models only");
+    int64_t bins = 1;
+    for(int i = 0; i < desc->dims; i++) bins *= desc->N;
+    if(bins > chd::kNoAllocMaxCells)
+        return fail(ctx, CH_EINVAL, "field of %lld cells exceeds the %lld the allocation-free reduction takes",
+                    (long long)bins, (long long)chd::kNoAllocMaxCells);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double *scr = chd::noalloc_scratch(ctx->device);
+    if(!scr) return fail(ctx, CH_ESTATE, "no scratch address for device %d", ctx->device);
+    const chd::Params p = make_params(desc);
+    double *result = scr + 2 * chd::kObsParts;
+    HIPCHK(ctx, chd::observables(p, desc->precision, desc->model, rho_dev, scr, chd::kObsParts, result, ctx->stream));
+    double h[2];
+    HIPCHK(ctx, hipMemcpyAsync(h, result, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if(mass) *mass = h[0] * p.vbin_over_bins;
+    if(free_energy) *free_energy = h[1] * p.vbin_over_bins;
+    if(std::isnan(h[0])) return fail(ctx, CH_ENAN, "Encountered a nan while computing the total mass");
+    if(std::isnan(h[1])) return fail(ctx, CH_ENAN, "Encountered a nan while computing the total free energy");
     return CH_OK;
 }
 

@@ -122,6 +122,12 @@ hipError_t finalize_pairs(double *partials, int nparts, double *result, hipStrea
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
                        double *result, hipStream_t s);
 constexpr int kObsParts = 1024;
+// allocation-free observables: the device address of the fixed scratch array
+// (2 kObsParts + 2 doubles), looked up per device by noalloc_scratch_init
+constexpr int kMaxDevices = 64;
+constexpr int64_t kNoAllocMaxCells = (int64_t)1 << 31;
+hipError_t noalloc_scratch_init(int device);
+double *noalloc_scratch(int device);
 
 // batched replicas (batch.hip)
 struct BatchRep {

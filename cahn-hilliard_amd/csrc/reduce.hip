@@ -7,6 +7,8 @@
 // in a fixed order (deterministic, no atomics).  Only two doubles return to
 // the host, instead of the reference's blocking copy of the whole field and a
 // serial host loop (CUDAIntegrator.cu:55-70).
+#include <mutex>
+
 #include "ch_internal.h"
 
 namespace chd {
@@ -97,6 +99,36 @@ hipError_t observables(const Params &p, int precision, int model, const void *rh
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, result);
     return hipGetLastError();
+}
+
+// ---- the allocation-free variant of the observables (ch_observables_device_noalloc)
+// One fixed array per device holds the kObsParts partial pairs and the final
+// pair: 2 kObsParts + 2 doubles (16,400 bytes), kept until the process ends.
+// Host code never passes the array's name to a kernel: noalloc_scratch_init
+// looks its device address up once per device (with that device current) and
+// noalloc_scratch returns that address.  Every call on a device shares the
+// array, so calls that overlap on one device race.
+static __device__ __align__(256) double scratch[2 * kObsParts + 2];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t noalloc_scratch_init(int device) {
+    if(device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if(g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if(e != hipSuccess) return e;
+    if(!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+double *noalloc_scratch(int device) {
+    if(device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return static_cast<double *>(g_scratch_ptr[device]);
 }
 
 hipError_t finalize_pairs(double *partials, int nparts, double *result, hipStream_t s) {

@@ -470,6 +470,15 @@ class Batch:
             pass
 
 
+def observables_device_noalloc(ctx, desc, rho_ptr):
+    """(mass, free energy) of a device field (int: address of N^dims elements of
+    desc's precision) without any allocation (ch_observables_device_noalloc)."""
+    m, f = ctypes.c_double(), ctypes.c_double()
+    _check(ctx.handle, L.lib().ch_observables_device_noalloc(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(rho_ptr),
+                                                             ctypes.byref(m), ctypes.byref(f)))
+    return m.value, f.value
+
+
 def der_bulk_free_energy(ctx, desc, rho_ptr, out_ptr, n):
     """FreeEnergyModel::der_bulk_free_energy on device buffers (ints: device addresses)."""
     _check(ctx.handle, L.lib().ch_der_bulk_free_energy(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(rho_ptr),

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 7
+#define CH_ABI_VERSION 8
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -203,6 +203,19 @@ ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 ch_status ch_sys_observables_queue(ch_sys *sys);
 ch_status ch_sys_observables_fetch(ch_sys *sys, double *mass, double *free_energy);
 int32_t ch_sys_observables_pending(const ch_sys *sys);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The reductions of ch_sys_observables (same kernels, same
+ * bits) on a field the caller holds on the device: `rho_dev` is N^dims
+ * elements of desc->precision, desc a single-species model (CH_EUNSUPPORTED
+ * otherwise).  No allocation: the partial sums live in one fixed array of
+ * 16,400 bytes per device, set up by ch_ctx_create; fields of more than 2^31
+ * cells are refused (CH_EINVAL).  Calls on one device share that array, so
+ * calls that overlap on one device race.  Synchronous; CH_ENAN as
+ * ch_sys_observables. */
+ch_status ch_observables_device_noalloc(ch_ctx *ctx, const ch_desc *desc, const void *rho_dev, double *mass,
+                                        double *free_energy);
 
 /* ---- structure factor (2D): the circular average of utils/sq_from_traj.py:72-103
  * (mean removed, |FFT|^2 / N^2, shell = rint(|k| / dq)), computed with the
