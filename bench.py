@@ -103,9 +103,18 @@ def cpu_baseline(N, budget_s=12.0):
     t0 = time.perf_counter()
     O.run(O.EULER, p, m, rho, steps, fast=True)
     dt = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(N * N * steps / dt / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{steps} FP64 Euler steps of the Landau 2D N={N} field (oracle, -O3 -ffast-math, "
-                      f"1 thread), {dt:.1f} s"}
+                      f"1 thread), {dt:.1f} s",
+            "host": {"cpus_visible": os.cpu_count(), "cpus_usable": len(os.sched_getaffinity(0)), "cpu_model": model},
+            "calibration": "the oracle's Euler matches the reference's own EulerCPU bit for bit and runs at "
+                           "0.84-1.10 x its speed on one core (profiles/cpu_baseline_calibration.md)"}
 
 
 # ------------------------------------------------------------------ workloads
@@ -239,57 +248,64 @@ def fp32_error(ch, ctx, integrator, dt, N, steps):
 
 
 def w_adjoint(ch, ctx, args):
-    """C4 shape: Landau 2D N=256, dt=0.01, eps=0.9, k=1.2; T steps forward, then
-    record (checkpoint every 100) + reverse sweep.  The loss here is
-    1/2 |psi_T|^2 (the adjoint's cost does not depend on the loss)."""
+    """C4 as written: Landau 2D N=256 Euler, dt = 0.01, T = 5000 steps from
+    psi0 = U[-0.01, 0.01) (drand48 seed 123456); loss = sum over the radial
+    shells of (S(q) - S_target(q))^2 on the final field (utils/sq_from_traj.py:72-103,
+    ch_sys_sq_loss), S_target from the same start at eps = 0.8, k = 1.0; the
+    gradients wrt eps, k and psi0 at eps = 0.9, k = 1.2, checkpoints every 100
+    steps, windows recomputed.  Timed (device-synchronised, best of 3): the
+    forward alone (T steps) against record + loss and seed + reverse sweep.  In
+    FP64 and FP32; the FP32 gradients are compared with the FP64 ones."""
     from cahn_hilliard_amd.init import initial_density
-    N, T = 256, 5000 if args.steps >= 50 else 1000
+    N, T, ckpt = 256, 5000, 100
     rho0 = initial_density(2, N, 0.0, 0.01, 0, 123456)
-    d = ch.make_desc(dims=2, N=N, integrator="euler", precision="fp64", dt=0.01, dx=1.0, k=1.2, epsilon=0.9)
-    import torch
-    s = ch.System(ctx, d)
-    s.upload(rho0)
-    s.evolve(T)
-    sync(ctx)
-    t_fwd = float("inf")
-    for _ in range(3):
-        t0 = time.perf_counter()
-        s.evolve(T)
-        sync(ctx)
-        t_fwd = min(t_fwd, time.perf_counter() - t0)
-    # the loss seed dL/dpsi_T = psi_T stays on the device (as in a device-side
-    # loss); the host-seed variant copies psi_T out and the seed back in
-    lam = torch.empty(N * N, dtype=torch.float64, device="cuda")
-    dpsi0 = torch.empty_like(lam)
-    torch.cuda.synchronize()
-    out, out_host = {}, {}
-    for ckpt in (100, 0):
+    st = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator="euler", dt=0.01, dx=1.0, k=1.0, epsilon=0.8))
+    st.upload(rho0)
+    st.evolve(T)
+    target = st.structure_factor()
+    st.close()
+    res, grads = {}, {}
+    for prec in ("fp64", "fp32"):
+        s = ch.System(ctx, ch.make_desc(dims=2, N=N, integrator="euler", precision=prec, dt=0.01, dx=1.0, k=1.2,
+                                        epsilon=0.9))
+        s.upload(rho0)
+        s.evolve(T)  # graphs
+        t_fwd = float("inf")
+        for _ in range(3):
+            s.upload(rho0)
+            sync(ctx)
+            t0 = time.perf_counter()
+            s.evolve(T)
+            sync(ctx)
+            t_fwd = min(t_fwd, time.perf_counter() - t0)
         # an untimed first call allocates the checkpoints, tapes and the
         # recompute stream (set-up, like an FFT plan)
         s.upload(rho0)
         s.record(T, ckpt)
-        s.adjoint(s.download())
-        t_dev = t_host = float("inf")
-        for _ in range(2):
+        s.adjoint(s.sq_loss(target)[1])
+        t_both = float("inf")
+        for _ in range(3):
             s.upload(rho0)
             sync(ctx)
             t0 = time.perf_counter()
             s.record(T, ckpt)
-            s.download_device(lam.data_ptr())
-            s.adjoint_device(lam.data_ptr(), dpsi0.data_ptr())
-            t_dev = min(t_dev, time.perf_counter() - t0)
-            s.upload(rho0)
-            sync(ctx)
-            t0 = time.perf_counter()
-            s.record(T, ckpt)
-            psiT = s.download()
-            s.adjoint(psiT)
-            t_host = min(t_host, time.perf_counter() - t0)
-        out[f"ckpt_{ckpt}"] = round(t_dev / t_fwd, 3)
-        out_host[f"ckpt_{ckpt}"] = round(t_host / t_fwd, 3)
-    s.close()
-    return {"fwd_plus_adjoint_over_fwd": out, "with_host_seed_copies": out_host, "N": N, "time_steps": T,
-            "fwd_ms": round(t_fwd * 1e3, 3), "timing": "best of 3 forward and of 2 forward + adjoint runs"}
+            loss, seed = s.sq_loss(target)
+            d0, g = s.adjoint(seed)
+            t_both = min(t_both, time.perf_counter() - t0)
+        s.close()
+        grads[prec] = (d0, g)
+        res[prec] = {"fwd_plus_adjoint_over_fwd": round(t_both / t_fwd, 3), "fwd_ms": round(t_fwd * 1e3, 3),
+                     "fwd_plus_adjoint_ms": round(t_both * 1e3, 3), "loss": float(f"{loss:.9e}"),
+                     "dL_depsilon": float(f"{g[ch.GRAD_EPSILON]:.9e}"), "dL_dk": float(f"{g[ch.GRAD_K]:.9e}")}
+    (d64, g64), (d32, g32) = grads["fp64"], grads["fp32"]
+    res["fp32"]["rel_err_vs_fp64"] = {
+        "dL_dpsi0": float(f"{np.abs(d32 - d64).max() / np.abs(d64).max():.3e}"),
+        "dL_depsilon": float(f"{abs(g32[ch.GRAD_EPSILON] / g64[ch.GRAD_EPSILON] - 1):.3e}"),
+        "dL_dk": float(f"{abs(g32[ch.GRAD_K] / g64[ch.GRAD_K] - 1):.3e}")}
+    return {"N": N, "time_steps": T, "checkpoint_every": ckpt, "fwd_plus_adjoint_over_fwd": res["fp64"][
+        "fwd_plus_adjoint_over_fwd"], "fp64": res["fp64"], "fp32": res["fp32"],
+            "timing": "best of 3; forward + adjoint = record + S(q) loss and its gradient + reverse sweep, "
+                      "seed and dL/dpsi0 through host memory"}
 
 
 def w_wertheim(ch, ctx, args):
@@ -302,7 +318,7 @@ def w_wertheim(ch, ctx, args):
     restatement of EulerMobilityCPU gives the same values and the same NaN),
     so the workload runs 400 steps."""
     from cahn_hilliard_amd.init import initial_density
-    N, nst = 128, 400 if args.steps >= 100 else 200
+    N, nst = 128, 400
     d = ch.make_desc(dims=3, N=N, integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
                      valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, dx=1.0, k=1.0)
     tot = tot_grad = 0.0

@@ -56,6 +56,10 @@ struct ch_sys {
     int64_t ckpt_hat_cap = 0;
     int graph_steps = 0;
     int tb = 1;  // time steps per stencil launch (temporal blocking depth)
+    // region shape of the 2D wave kernel's shorter launches (K < tb), which
+    // close a step count that tb does not divide: smaller regions give such a
+    // launch enough blocks to fill the chip
+    int tail_cfg = 0;
     int64_t *noise_step = nullptr;  // euler_mobility noise: index of the next step (device)
     // pseudospectral_mobility: spectra, fields, reductions and the GMRES workspace
     void *psm_spec[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -193,6 +197,17 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     if(d->integrator == CH_EULER_MOBILITY && d->with_noise && !std::isfinite(d->noise_rescale_factor))
         return fail(ctx, CH_EINVAL, "noise_rescale_factor must be finite");
     return CH_OK;
+}
+
+// the observables' kObsParts partial pairs and the reduction's block counter
+// (2 doubles' room, 0 before the first launch; chd::observables keeps it at 0)
+static bool alloc_partials(double **p) {
+    const size_t bytes = (chd::kObsParts * 2 + 2) * sizeof(double);
+    if(hipMalloc((void **)p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return false;
+    }
+    return hipMemset(*p, 0, bytes) == hipSuccess;
 }
 
 static bool has_noise(const ch_sys *s) { return s->d.integrator == CH_EULER_MOBILITY && s->d.with_noise; }
@@ -361,6 +376,11 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     s->tb = 1;
     while(s->tb * 2 <= max_k && s->tb * 2 <= want) s->tb *= 2;
     if(const char *r = std::getenv("CH_WAVE_CFG")) s->p.wave_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
+    // N >= 1024: 128 x 80 regions for the shorter launches (measured at N = 1024
+    // with 8-step launches and a 4-step tail: 0.2635 ms per 100 steps against
+    // 0.2674 with the 192 x 64 regions of the full launches)
+    s->tail_cfg = N >= 1024 ? 7 : s->p.wave_cfg;
+    if(const char *r = std::getenv("CH_WAVE_CFG_TAIL")) s->tail_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
     if(const char *r = std::getenv("CH_TILE3D")) s->p.tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_ADJ_TILE3D")) s->p.adj_tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_NOISE_TILE")) s->p.noise_tile = std::max(0, std::min(2, std::atoi(r)));
@@ -377,7 +397,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
 
     auto alloc = [&](void **p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
     bool ok = alloc(&s->psi[0], s->bins * s->elt) && alloc(&s->stage, s->bins * 8) &&
-              alloc(&s->stage2, s->bins * s->elt) && alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
+              alloc(&s->stage2, s->bins * s->elt) && alloc_partials(&s->partials) &&
               alloc((void **)&s->result, 2 * sizeof(double));
     if(ok && !is_spectral(desc->integrator)) ok = alloc(&s->psi[1], s->bins * s->elt);
     if(ok && noisy) ok = alloc((void **)&s->noise_step, sizeof(int64_t)) &&
@@ -487,7 +507,7 @@ extern "C" ch_status ch_sys_create_species(ch_ctx *ctx, const ch_desc *desc, con
     const int P = std::max(1, s->spm_host->P);
     auto alloc = [&](void **p, size_t bytes) -> bool { return hipMalloc(p, bytes) == hipSuccess; };
     bool ok = alloc(&s->psi[0], SB * s->elt) && alloc(&s->stage, SB * 8) &&
-              alloc((void **)&s->partials, chd::kObsParts * 2 * sizeof(double)) &&
+              alloc_partials(&s->partials) &&
               alloc((void **)&s->result, 2 * sizeof(double)) && alloc((void **)&s->spm_dev, sizeof(chd::SpModel)) &&
               alloc((void **)&s->sp_X, (size_t)P * s->bins * sizeof(double)) && alloc(&s->sp_mu, SB * s->elt) &&
               alloc(&s->sp_mob, SB * s->elt);
@@ -667,9 +687,11 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K, int noise_off = 0) {
         cur ^= 1;
         return e;
     }
+    chd::Params p2 = s->p;
+    if(K < s->tb) p2.wave_cfg = s->tail_cfg;
     switch(d.dims) {
     case 1: e = chd::stencil_step<1>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
-    case 2: e = chd::stencil_step<2>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
+    case 2: e = chd::stencil_step<2>(p2, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     default: e = chd::stencil_step<3>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     }
     cur ^= 1;
@@ -1001,7 +1023,7 @@ extern "C" ch_status ch_observables_device_noalloc(ch_ctx *ctx, const ch_desc *d
     double *scr = chd::noalloc_scratch(ctx->device);
     if(!scr) return fail(ctx, CH_ESTATE, "no scratch address for device %d", ctx->device);
     const chd::Params p = make_params(desc);
-    double *result = scr + 2 * chd::kObsParts;
+    double *result = scr + 2 * chd::kObsParts + 2;  // after the partials and the block counter
     HIPCHK(ctx, chd::observables(p, desc->precision, desc->model, rho_dev, scr, chd::kObsParts, result, ctx->stream));
     double h[2];
     HIPCHK(ctx, hipMemcpyAsync(h, result, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

@@ -27,7 +27,7 @@ struct Params {
     double M, rho_min, rho_ref;     // mobility
     double k, two_k, dx, inv_dx, inv_dx2, dt, S;
     double Mdt;                     // M dt (constant-mobility Euler update)
-    double mu_lap, upd_lap;         // -2 k / dx^2 and M dt / dx^2 (folded FP32 cell arithmetic)
+    double mu_lap, upd_lap;         // -2 k / dx^2 and M dt / dx^2 (the folded cell arithmetic)
     double q2_unit;                 // (2 pi / (N dx))^2: q^2 of a unit wave vector
     double vbin_over_bins;          // V_bin / bins
     double inv_bins;                // 1 / N^dims (c2r normalisation)
@@ -58,10 +58,9 @@ template <typename T, int MODEL>
 __device__ __forceinline__ T fprime(const Params &p, T r) {
 #pragma clang fp contract(off)
     if constexpr(MODEL == CH_LANDAU) {
-        // FP32: psi (psi^2 - eps), two operations; FP64 keeps the reference's
-        // -eps psi + psi^3
-        if constexpr(sizeof(T) == 4) return r * fma_t(r, r, -(T)p.eps);
-        else return fma_t(r * r, r, -(T)p.eps * r);
+        // psi (psi^2 - eps): two operations and two roundings for the
+        // reference's -eps psi + psi^3
+        return r * fma_t(r, r, -(T)p.eps);
     }
     else {
         const double rd = (double)r;
@@ -150,9 +149,9 @@ __device__ __forceinline__ T laplacian(T c, const T (&m)[DIMS], const T (&q)[DIM
 }
 
 // The Laplacian without its 1 / dx^2: the neighbour sum minus 2 d times the
-// centre.  FP32 cells fold 1 / dx^2 into the constant that multiplies it
+// centre.  The cells fold 1 / dx^2 into the constant that multiplies it
 // (-2 k / dx^2 for mu, M dt / dx^2 for the update): one multiplication less
-// per Laplacian; FP64 cells keep the reference's grouping.
+// per Laplacian, and the same bits as lap = sum / dx^2 when dx = 1.
 template <typename T, int DIMS>
 __device__ __forceinline__ T laplacian_sum(T c, const T (&m)[DIMS], const T (&q)[DIMS]) {
 #pragma clang fp contract(off)
@@ -169,16 +168,14 @@ __device__ __forceinline__ T laplacian_sum(T c, const T (&m)[DIMS], const T (&q)
 template <typename T, int DIMS, int MODEL>
 __device__ __forceinline__ T chem_pot(const Params &p, T c, const T (&m)[DIMS], const T (&q)[DIMS]) {
 #pragma clang fp contract(off)
-    if constexpr(sizeof(T) == 4) return fma_t((T)p.mu_lap, laplacian_sum<T, DIMS>(c, m, q), fprime<T, MODEL>(p, c));
-    else return fma_t(-(T)p.two_k, laplacian<T, DIMS>(c, m, q, (T)p.inv_dx2), fprime<T, MODEL>(p, c));
+    return fma_t((T)p.mu_lap, laplacian_sum<T, DIMS>(c, m, q), fprime<T, MODEL>(p, c));
 }
 
 // dt M lap(mu), the increment of the constant-mobility Euler update
 template <typename T, int DIMS>
 __device__ __forceinline__ T euler_inc(const Params &p, T mu, const T (&m)[DIMS], const T (&q)[DIMS], T base) {
 #pragma clang fp contract(off)
-    if constexpr(sizeof(T) == 4) return fma_t((T)p.upd_lap, laplacian_sum<T, DIMS>(mu, m, q), base);
-    else return fma_t((T)p.Mdt, laplacian<T, DIMS>(mu, m, q, (T)p.inv_dx2), base);
+    return fma_t((T)p.upd_lap, laplacian_sum<T, DIMS>(mu, m, q), base);
 }
 
 // psi + dt M lap(mu)  (EulerCPU.cpp:29-35), constant mobility

@@ -210,7 +210,7 @@ int32_t ch_sys_observables_pending(const ch_sys *sys);
  * bits) on a field the caller holds on the device: `rho_dev` is N^dims
  * elements of desc->precision, desc a single-species model (CH_EUNSUPPORTED
  * otherwise).  No allocation: the partial sums live in one fixed array of
- * 16,400 bytes per device, set up by ch_ctx_create; fields of more than 2^31
+ * 16,416 bytes per device, set up by ch_ctx_create; fields of more than 2^31
  * cells are refused (CH_EINVAL).  Calls on one device share that array, so
  * calls that overlap on one device race.  Synchronous; CH_ENAN as
  * ch_sys_observables. */

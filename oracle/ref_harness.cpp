@@ -22,6 +22,25 @@
 #include <memory>
 #include <sstream>
 
+// mobility.type = "linear" (M rho / rho_ref) has no class in the reference: the
+// harness supplies it through the reference's own IMobility interface
+// (src/mobility/IMobility.h), which EulerMobilityCPU reads through
+// SimulationState::mobility like every other rule.
+template <int dims>
+class HarnessLinearMobility : public ch::IMobility<dims> {
+public:
+    HarnessLinearMobility(ch::SimulationState<dims> &state, double M, double rho_ref)
+        : ch::IMobility<dims>(state), _M(M), _rho_ref(rho_ref) {}
+    void update_mobility() override {
+        for(unsigned int idx = 0; idx < this->_sim_state.rho.bins(); idx++)
+            for(int s = 0; s < this->_sim_state.rho.species(); s++)
+                this->_sim_state.mobility(idx, s) = _M * this->_sim_state.rho(idx, s) / _rho_ref;
+    }
+
+private:
+    double _M, _rho_ref;
+};
+
 int main(int argc, char **argv) {
     if(argc < 5) {
         std::fprintf(stderr, "usage: %s config.toml in.bin steps out.bin\n", argv[0]);
@@ -55,7 +74,12 @@ int main(int argc, char **argv) {
     in.read(reinterpret_cast<char *>(state.rho.data()), sizeof(double) * bins);
     if(!in) { std::fprintf(stderr, "short input\n"); return 2; }
 
-    std::unique_ptr<ch::IMobility<DIM>> mobility(ch::build_mobility(config, state));
+    std::unique_ptr<ch::IMobility<DIM>> mobility;
+    if(config["mobility"]["type"].value_or(std::string("constant")) == "linear")
+        mobility = std::make_unique<HarnessLinearMobility<DIM>>(state, config["mobility"]["M"].value_or(1.0),
+                                                                config["mobility"]["rho_ref"].value_or(1.0));
+    else
+        mobility.reset(ch::build_mobility(config, state));
     std::string integ = config["integrator"].value_or(std::string("euler"));
     std::unique_ptr<ch::Integrator<DIM>> integrator;
     if(integ == "euler") integrator = std::make_unique<ch::EulerCPU<DIM>>(state, state.model.get(), config);

@@ -11,7 +11,8 @@ plus tests/golden/cases.json with the parameters.  The initial fields come
 from glibc's srand48/drand48 through the reference's recipe
 (src/CahnHilliard.cpp:133-151), computed here independently of the oracle.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py            (the small cases)
+       python tests/golden/make_golden.py --large    (the full-size cases, ~2 min)
 """
 import ctypes
 import json
@@ -59,6 +60,28 @@ CASES = [
 ]
 
 
+# Full-size configurations: the Landau 2D N=1024 Euler workload after 1000 steps
+# (stored as every 8th cell in x and y plus sums over the whole field, to keep
+# the fixture small) and the N=128 workload after 1000 and 10,000 steps (whole
+# field).  Both start from psi0 = U[-0.01, 0.01) (initial_density 0,
+# initial_A 0.01, seed 123456) with eps = 0.9, k = 1, dt = 0.01, dx = 1, M = 1.
+LARGE_CASES = [
+    dict(name="large_landau_2d_N1024", integrator="euler", dims=2, N=1024, marks=[1000], stride=8, dt=0.01,
+         k=1.0, init=(0.0, 0.01, 123456), model=dict(free_energy="landau", epsilon=0.9),
+         mobility=dict(type="constant", M=1.0)),
+    dict(name="large_landau_2d_N128", integrator="euler", dims=2, N=128, marks=[1000, 10000], stride=1, dt=0.01,
+         k=1.0, init=(0.0, 0.01, 123456), model=dict(free_energy="landau", epsilon=0.9),
+         mobility=dict(type="constant", M=1.0)),
+    # SimpleWertheim 3D N=128 euler_mobility with M = rho / 0.05 (the harness's
+    # linear mobility through the reference's IMobility), rho0 = 0.05 (1 +
+    # U[-0.05, 0.05)), 400 steps (rho turns negative near step 450 on these
+    # inputs); every 4th cell in x, y and z
+    dict(name="large_wertheim_3d_N128", integrator="euler_mobility", dims=3, N=128, marks=[400], stride=4, dt=1e-4,
+         k=1.0, init=(0.05, 0.05, 123456), model=dict(free_energy="simple_wertheim", valence=4, B2=1.0, delta=1e4),
+         mobility=dict(type="linear", M=1.0, rho_ref=0.05)),
+]
+
+
 def glibc_init(dims, N, density, A, seed):
     """src/CahnHilliard.cpp:133-151 with N_peaks = 0, using glibc's drand48."""
     libc = ctypes.CDLL("libc.so.6")
@@ -99,9 +122,40 @@ def run_ref(c, rho0, steps, tmp):
     return np.fromfile(out, dtype="<f8")
 
 
+def make_large(tmp):
+    """LARGE_CASES -> tests/golden/<name>.npz: for each mark t the field after t
+    steps at the cells whose coordinates are all multiples of `stride`
+    (rho<t>_sub, x fastest), and over the whole field sum(rho<t>) and
+    sum(rho<t>^2); plus
+    tests/golden/large_cases.json."""
+    meta = []
+    for c in LARGE_CASES:
+        N, st = c["N"], c["stride"]
+        rho0 = glibc_init(c["dims"], N, *c["init"])
+        out, prev, done = {}, rho0, 0
+        for t in c["marks"]:
+            cfg = dict(c, steps=t - done)
+            prev = run_ref(cfg, prev, t - done, tmp)
+            done = t
+            assert np.all(np.isfinite(prev)), c["name"]
+            f = prev.reshape((N,) * c["dims"])
+            out[f"rho{t}_sub"] = f[(slice(None, None, st),) * c["dims"]].copy()
+            out[f"sum{t}"] = np.array(prev.sum())
+            out[f"sumsq{t}"] = np.array((prev * prev).sum())
+            print(f"{c['name']} t={t}: max|rho|={np.abs(prev).max():.4g} sum={prev.sum():.6e}")
+        np.savez_compressed(os.path.join(HERE, c["name"] + ".npz"), **out)
+        meta.append({k: v for k, v in c.items()})
+    with open(os.path.join(HERE, "large_cases.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 def main():
     if not os.path.exists(os.path.join(REF_BIN, "ref_2D")):
         sys.exit("oracle/_ref is not built: run `make -C oracle ref`")
+    if "--large" in sys.argv:
+        with tempfile.TemporaryDirectory() as tmp:
+            make_large(tmp)
+        return
     meta = []
     with tempfile.TemporaryDirectory() as tmp:
         for c in CASES:

@@ -8,6 +8,7 @@ import oracle as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = json.load(open(os.path.join(GOLD, "cases.json")))
+LARGE = {c["name"]: c for c in json.load(open(os.path.join(GOLD, "large_cases.json")))}
 MOB = {"constant": O.MOB_CONSTANT, "free_energy": O.MOB_FREE_ENERGY, "regularised": O.MOB_REGULARISED,
        "linear": O.MOB_LINEAR}
 

@@ -84,6 +84,27 @@ def lap_eig(N, dx, m):
     return -(4.0 / dx ** 2) * sum(np.sin(np.pi * mi / N) ** 2 for mi in m)
 
 
+def test_c1_full_length_against_reference_build():
+    """The N=128 workload (Landau 2D, eps 0.9, k 1, dt 0.01, psi0 = U[-0.01, 0.01)
+    from drand48 seed 123456): the oracle against the reference's own EulerCPU
+    (tests/golden/large_landau_2d_N128.npz) after 1000 and 10,000 steps."""
+    from helpers import LARGE, golden
+    c = LARGE["large_landau_2d_N128"]
+    g = golden(c["name"])
+    p = O.Params(dims=2, N=128, dt=0.01, dx=1.0, k=1.0)
+    m = O.Model(epsilon=0.9)
+    rho = O.init_density(2, 128, 0.0, 0.01, 0, 123456)
+    done = 0
+    for t in c["marks"]:
+        rho = O.run(O.EULER, p, m, rho, t - done)
+        done = t
+        ref = g[f"rho{t}_sub"].ravel()
+        err = float(np.abs(rho - ref).max() / np.abs(ref).max())
+        print(f"C1 oracle vs reference after {t} steps: rel L-inf {err:.2e}")
+        assert err < 1e-10
+        assert abs(rho.sum() - float(g[f"sum{t}"])) < 1e-12 * rho.size
+
+
 @pytest.mark.parametrize("dims,m", [(1, (3,)), (2, (2, 5)), (3, (1, 2, 3))])
 def test_euler_linear_eigenmode(dims, m):
     """psi = a cos(2 pi m.x / N), a = 1e-7: one Euler step multiplies it by
