@@ -60,11 +60,12 @@ def timed_run(ctx, sysm, steps, warmup, inner, observe=True):
     read inside the timed region."""
     def run(n):
         for _ in range(n):
-            sysm.evolve(inner)
             if observe:
-                sysm.queue_observables()
+                sysm.evolve_observe(inner)  # the steps and the queued reductions, one graph
                 if sysm.observables_pending > 1:
                     sysm.fetch_observables()
+            else:
+                sysm.evolve(inner)
         while observe and sysm.observables_pending:
             sysm.fetch_observables()
     run(warmup)

@@ -119,6 +119,7 @@ def lib():
         "ch_sys_observables_queue": (ctypes.c_int, [P]),
         "ch_sys_observables_fetch": (ctypes.c_int, [P, DD, DD]),
         "ch_sys_observables_pending": (ctypes.c_int32, [P]),
+        "ch_sys_evolve_observe": (ctypes.c_int, [P, I64]),
         "ch_observables_device_noalloc": (ctypes.c_int, [P, DP, P, DD, DD]),
         "ch_sq_nshells": (I32, [I32]),
         "ch_sys_structure_factor": (ctypes.c_int, [P, P, I32]),

@@ -85,6 +85,12 @@ struct ch_sys {
     // the remainder nsteps % G of a longer run (per starting buffer), captured
     // when the same count recurs
     hipGraphExec_t rexec[2] = {nullptr, nullptr};
+    // ch_sys_evolve_observe: per starting buffer, the captured steps + queued
+    // reductions of a recurring count (osteps), its end buffer (ocur), and the
+    // count seen once (captured on its second occurrence)
+    hipGraphExec_t oexec[2] = {nullptr, nullptr};
+    int64_t osteps[2] = {0, 0}, oseen[2] = {-1, -1};
+    int ocur[2] = {0, 0};
     int64_t rsteps[2] = {0, 0};
     int rcur[2] = {0, 0};       // the buffer holding the field after those steps
     int64_t rseen[2] = {0, 0};  // the last such count run by direct launches
@@ -199,8 +205,9 @@ static ch_status validate(ch_ctx *ctx, const ch_desc *d) {
     return CH_OK;
 }
 
-// the observables' kObsParts partial pairs and the reduction's block counter
-// (2 doubles' room, 0 before the first launch; chd::observables keeps it at 0)
+// the observables' kObsParts partial pairs, then (2 doubles' room) the device
+// count of queued observables, which picks each queued pair's ring slot
+// (chd::observables_ring); zeroed here, with the host's count of queued pairs
 static bool alloc_partials(double **p) {
     const size_t bytes = (chd::kObsParts * 2 + 2) * sizeof(double);
     if(hipMalloc((void **)p, bytes) != hipSuccess) {
@@ -325,6 +332,7 @@ static void free_sys(ch_sys *s) {
     for(int i = 0; i < 2; i++) {
         if(s->gexec[i]) (void)hipGraphExecDestroy(s->gexec[i]);
         if(s->rexec[i]) (void)hipGraphExecDestroy(s->rexec[i]);
+        if(s->oexec[i]) (void)hipGraphExecDestroy(s->oexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
     void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
@@ -939,19 +947,29 @@ extern "C" int32_t ch_sys_region_shape(const ch_sys *s) {
     return s->p.wave_cfg;
 }
 
+static hipError_t ensure_obs_host(ch_sys *s) {
+    if(s->obs_host) return hipSuccess;
+    return hipHostMalloc((void **)&s->obs_host, 2 * (1 + CH_OBS_QUEUE) * sizeof(double));
+}
+
 // the FP64 sums of the current field into pinned host memory, straight from
 // the finalize kernel (no copy behind the reduction): the synchronous pair
-// (slot -1) or queue slot `slot`
-static hipError_t enqueue_observables(ch_sys *s, int slot) {
+// (queued = false) or the next queue slot, which the device's count of queued
+// pairs selects (so the same launches can be captured once and replayed)
+static hipError_t enqueue_observables(ch_sys *s, bool queued) {
     ch_ctx *c = s->ctx;
-    if(!s->obs_host) {
-        hipError_t e = hipHostMalloc((void **)&s->obs_host, 2 * (1 + CH_OBS_QUEUE) * sizeof(double));
-        if(e != hipSuccess) return e;
-    }
-    double *dst = s->obs_host + 2 * (1 + slot);
-    if(is_species(s)) return chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, dst, c->stream);
-    return chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts, dst,
-                            c->stream);
+    hipError_t e = ensure_obs_host(s);
+    if(e != hipSuccess) return e;
+    double *ring = s->obs_host + 2;
+    unsigned *count = reinterpret_cast<unsigned *>(s->partials + 2 * chd::kObsParts);
+    if(is_species(s))
+        return queued ? chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, nullptr, c->stream, ring, count)
+                      : chd::sp_observables(sp_args(s), s->partials, chd::kObsParts, s->obs_host, c->stream);
+    if(queued)
+        return chd::observables_ring(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts,
+                                     ring, count, c->stream);
+    return chd::observables(s->p, s->d.precision, s->d.model, s->psi[s->cur], s->partials, chd::kObsParts,
+                            s->obs_host, c->stream);
 }
 static ch_status observables_result(ch_sys *s, const double *pair, double *mass, double *free_energy);
 
@@ -959,7 +977,7 @@ extern "C" ch_status ch_sys_observables(ch_sys *s, double *mass, double *free_en
     if(!s) return CH_EINVAL;
     ch_ctx *c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, enqueue_observables(s, -1));
+    HIPCHK(c, enqueue_observables(s, false));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return observables_result(s, s->obs_host, mass, free_energy);
 }
@@ -972,7 +990,66 @@ extern "C" ch_status ch_sys_observables_queue(ch_sys *s) {
     HIPCHK(c, hipSetDevice(c->device));
     const int slot = (s->obs_head + s->obs_count) % CH_OBS_QUEUE;
     if(!s->obs_ev[slot]) HIPCHK(c, hipEventCreateWithFlags(&s->obs_ev[slot], hipEventDisableTiming));
-    HIPCHK(c, enqueue_observables(s, slot));
+    HIPCHK(c, enqueue_observables(s, true));
+    HIPCHK(c, hipEventRecord(s->obs_ev[slot], c->stream));
+    s->obs_count++;
+    return CH_OK;
+}
+
+// nsteps of ch_sys_evolve, then ch_sys_observables_queue.  For the stencil
+// integrators (no noise) a count that recurs from the same buffer is one
+// captured graph of its launches and of the reductions: the host issues one
+// graph launch per print interval, with no launch gap before the reductions.
+extern "C" ch_status ch_sys_evolve_observe(ch_sys *s, int64_t nsteps) {
+    if(!s || nsteps < 0) return CH_EINVAL;
+    ch_ctx *c = s->ctx;
+    if(s->obs_count == CH_OBS_QUEUE)
+        return fail(c, CH_EINVAL, "%d observables already queued: fetch before queueing more", CH_OBS_QUEUE);
+    const bool fusable = !is_species(s) && !is_spectral(s->d.integrator) && !has_noise(s) && s->graph_steps > 0 &&
+                         nsteps >= s->tb && nsteps <= 4 * (int64_t)s->graph_steps;
+    const int par = s->cur;
+    if(!fusable || (!(s->oexec[par] && s->osteps[par] == nsteps) && s->oseen[par] != nsteps)) {
+        // first sight of this count (or not capturable): issue it directly
+        if(fusable) s->oseen[par] = nsteps;
+        ch_status st = ch_sys_evolve(s, nsteps);
+        if(st != CH_OK) return st;
+        return ch_sys_observables_queue(s);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    if(!s->oexec[par] || s->osteps[par] != nsteps) {
+        if(s->oexec[par]) {
+            HIPCHK(c, hipGraphExecDestroy(s->oexec[par]));
+            s->oexec[par] = nullptr;
+        }
+        HIPCHK(c, ensure_obs_host(s));
+        hipGraph_t graph;
+        int cur = par;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        int64_t n = nsteps;
+        for(; n >= s->tb && e == hipSuccess; n -= s->tb) e = launch_steps(s, cur, s->tb);
+        for(int K = s->tb / 2; K >= 1 && e == hipSuccess; K /= 2)
+            if(n >= K) {
+                e = launch_steps(s, cur, K);
+                n -= K;
+            }
+        const int saved = s->cur;
+        s->cur = cur;  // the reductions read the chain's last buffer
+        if(e == hipSuccess) e = enqueue_observables(s, true);
+        s->cur = saved;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+        if(e != hipSuccess) return fail(c, CH_EHIP, "launch during capture failed: %s", hipGetErrorString(e));
+        if(e2 != hipSuccess) return fail(c, CH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e2));
+        e = hipGraphInstantiate(&s->oexec[par], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if(e != hipSuccess) return fail(c, CH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        s->osteps[par] = nsteps;
+        s->ocur[par] = cur;
+    }
+    const int slot = (s->obs_head + s->obs_count) % CH_OBS_QUEUE;
+    if(!s->obs_ev[slot]) HIPCHK(c, hipEventCreateWithFlags(&s->obs_ev[slot], hipEventDisableTiming));
+    HIPCHK(c, hipGraphLaunch(s->oexec[par], c->stream));
+    s->cur = s->ocur[par];
     HIPCHK(c, hipEventRecord(s->obs_ev[slot], c->stream));
     s->obs_count++;
     return CH_OK;
@@ -1023,7 +1100,7 @@ extern "C" ch_status ch_observables_device_noalloc(ch_ctx *ctx, const ch_desc *d
     double *scr = chd::noalloc_scratch(ctx->device);
     if(!scr) return fail(ctx, CH_ESTATE, "no scratch address for device %d", ctx->device);
     const chd::Params p = make_params(desc);
-    double *result = scr + 2 * chd::kObsParts + 2;  // after the partials and the block counter
+    double *result = scr + 2 * chd::kObsParts;  // after the partials
     HIPCHK(ctx, chd::observables(p, desc->precision, desc->model, rho_dev, scr, chd::kObsParts, result, ctx->stream));
     double h[2];
     HIPCHK(ctx, hipMemcpyAsync(h, result, sizeof h, hipMemcpyDeviceToHost, ctx->stream));

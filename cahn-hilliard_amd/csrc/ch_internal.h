@@ -115,18 +115,23 @@ hipError_t sp_step_stencil(const SpArgs &a, int integrator, hipStream_t s);
 hipError_t sp_step_spectral(const SpArgs &a, const SpectralArgs &sa, hipStream_t s);
 hipError_t sp_init_spectral(const SpArgs &a, const SpectralArgs &sa, hipStream_t s);
 hipError_t sp_mobility_field(const SpArgs &a, double *out, hipStream_t s);
-hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double *result, hipStream_t s);
+// ring != nullptr: the pair goes to the ring slot of *count (observables_ring)
+hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double *result, hipStream_t s,
+                          double *ring = nullptr, unsigned *count = nullptr);
 hipError_t finalize_pairs(double *partials, int nparts, double *result, hipStream_t s);
 
-// FP64 observables: partials -> result[0] = sum rho, result[1] = sum f-density.
-// `partials` holds 2 nparts + 2 doubles: the pairs, then the block counter of
-// the reduction, which must be 0 when first used (the kernel leaves it at 0).
+// FP64 observables: partials (2 nparts doubles) -> result[0] = sum rho,
+// result[1] = sum f-density.  The _ring forms write the pair to slot
+// (*count % CH_OBS_QUEUE) of `ring` and increment *count on the device.
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
                        double *result, hipStream_t s);
+hipError_t observables_ring(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
+                            double *ring, unsigned *count, hipStream_t s);
+hipError_t finalize_pairs_ring(double *partials, int nparts, double *ring, unsigned *count, hipStream_t s);
 constexpr int kObsParts = 1024;
 // allocation-free observables: the device address of the fixed scratch array
-// (2 kObsParts + 4 doubles: partials, block counter, result), looked up per
-// device by noalloc_scratch_init
+// (2 kObsParts + 2 doubles: partials, result), looked up per device by
+// noalloc_scratch_init
 constexpr int kMaxDevices = 64;
 constexpr int64_t kNoAllocMaxCells = (int64_t)1 << 31;
 hipError_t noalloc_scratch_init(int device);

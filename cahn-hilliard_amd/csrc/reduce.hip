@@ -48,9 +48,10 @@ __device__ __forceinline__ void obs_cell(const T *__restrict__ rho, const Params
     f += bulk<MODEL>(p, r) + inter;
 }
 
-// result[0 .. 1] = the sums of n (mass, f) pairs, one block, in a fixed order
-__device__ __forceinline__ void sum_pairs_block(const double *__restrict__ partials, int n, double *__restrict__ result) {
-    double m = 0, f = 0;
+// the sums of n (mass, f) pairs, one block, in a fixed order
+__device__ __forceinline__ void sum_pairs_block(const double *__restrict__ partials, int n, double &m, double &f) {
+    m = 0;
+    f = 0;
     for(int i = threadIdx.x; i < n; i += 256) {
         m += partials[2 * i];
         f += partials[2 * i + 1];
@@ -61,25 +62,17 @@ __device__ __forceinline__ void sum_pairs_block(const double *__restrict__ parti
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if(lane == 0) { sm[w] = m; sf[w] = f; }
     __syncthreads();
-    if(threadIdx.x == 0) {
-        result[0] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
-        result[1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
-    }
+    m = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+    f = (sf[0] + sf[1]) + (sf[2] + sf[3]);
 }
 
-// Per-block (mass, f) partials; the block that finishes last (counted in
-// *done, which it resets to 0 for the next launch) sums them in block order,
-// so the result has the bits of a separate single-block pass and the
-// reduction is one launch.
 template <typename T, int MODEL>
-__global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, Params p, double *__restrict__ partials,
-                                                     unsigned *__restrict__ done, double *__restrict__ result) {
+__global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, Params p, double *__restrict__ partials) {
     const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
     double m = 0, f = 0;
     for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256)
         obs_cell<T, MODEL>(rho, p, i, m, f);
     __shared__ double sm[4], sf[4];
-    __shared__ int last;
     m = wave_sum(m);
     f = wave_sum(f);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -88,48 +81,82 @@ __global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, 
     if(threadIdx.x == 0) {
         partials[2 * blockIdx.x] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
         partials[2 * blockIdx.x + 1] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
-        __threadfence();  // the partials are visible device-wide before the count
-        last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if(!last) return;
-    __threadfence();  // and every other block's partials are seen here
-    sum_pairs_block(partials, gridDim.x, result);
-    if(threadIdx.x == 0) *done = 0;
 }
 
 __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nparts,
                                                   double *__restrict__ result) {
-    sum_pairs_block(partials, nparts, result);
+    double m, f;
+    sum_pairs_block(partials, nparts, m, f);
+    if(threadIdx.x == 0) {
+        result[0] = m;
+        result[1] = f;
+    }
+}
+
+// the same sums into slot (*count % CH_OBS_QUEUE) of a ring of pairs (host
+// memory the device writes), then ++*count: a captured graph that queues
+// observables needs no argument of its own per replay, and the host, which
+// counts the queued pairs in the same order, knows each pair's slot
+__global__ __launch_bounds__(256) void k_finalize_ring(const double *__restrict__ partials, int nparts,
+                                                       double *__restrict__ ring, unsigned *__restrict__ count) {
+    double m, f;
+    sum_pairs_block(partials, nparts, m, f);
+    if(threadIdx.x == 0) {
+        const unsigned c = *count;
+        const unsigned slot = c % CH_OBS_QUEUE;
+        ring[2 * slot] = m;
+        ring[2 * slot + 1] = f;
+        *count = c + 1;
+    }
+}
+
+static hipError_t obs_partials(const Params &p, int precision, int model, const void *rho, double *partials, int nb,
+                               hipStream_t s) {
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<double, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
+        else hipLaunchKernelGGL((k_observables<double, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
+    }
+    else {
+        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<float, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
+        else hipLaunchKernelGGL((k_observables<float, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
+    }
+    return hipGetLastError();
 }
 
 hipError_t observables(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
                        double *result, hipStream_t s) {
     const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
-    int nb = (int)std::min<int64_t>(nparts, (bins + 255) / 256);
-    // the block counter sits after the nparts pairs (kept at 0 between launches)
-    unsigned *done = reinterpret_cast<unsigned *>(partials + 2 * nparts);
-    if(precision == CH_FP64) {
-        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<double, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials, done, result);
-        else hipLaunchKernelGGL((k_observables<double, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials, done, result);
-    }
-    else {
-        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<float, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials, done, result);
-        else hipLaunchKernelGGL((k_observables<float, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials, done, result);
-    }
+    const int nb = (int)std::min<int64_t>(nparts, (bins + 255) / 256);
+    hipError_t e = obs_partials(p, precision, model, rho, partials, nb, s);
+    if(e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, result);
+    return hipGetLastError();
+}
+
+hipError_t observables_ring(const Params &p, int precision, int model, const void *rho, double *partials, int nparts,
+                            double *ring, unsigned *count, hipStream_t s) {
+    const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    const int nb = (int)std::min<int64_t>(nparts, (bins + 255) / 256);
+    hipError_t e = obs_partials(p, precision, model, rho, partials, nb, s);
+    if(e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize_ring, dim3(1), dim3(256), 0, s, partials, nb, ring, count);
+    return hipGetLastError();
+}
+
+hipError_t finalize_pairs_ring(double *partials, int nparts, double *ring, unsigned *count, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize_ring, dim3(1), dim3(256), 0, s, partials, nparts, ring, count);
     return hipGetLastError();
 }
 
 // ---- the allocation-free variant of the observables (ch_observables_device_noalloc)
-// One fixed array per device holds the kObsParts partial pairs, the block
-// counter of k_observables (zero at load, reset by every launch's last block)
-// and the final pair: 2 kObsParts + 4 doubles (16,416 bytes), kept until the
-// process ends.
+// One fixed array per device holds the kObsParts partial pairs and the final
+// pair: 2 kObsParts + 2 doubles (16,400 bytes), kept until the process ends.
 // Host code never passes the array's name to a kernel: noalloc_scratch_init
 // looks its device address up once per device (with that device current) and
 // noalloc_scratch returns that address.  Every call on a device shares the
 // array, so calls that overlap on one device race.
-static __device__ __align__(256) double scratch[2 * kObsParts + 4];
+static __device__ __align__(256) double scratch[2 * kObsParts + 2];
 
 static std::mutex g_scratch_mu;
 static void *g_scratch_ptr[kMaxDevices] = {};

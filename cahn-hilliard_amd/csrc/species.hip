@@ -494,7 +494,8 @@ hipError_t sp_mobility_field(const SpArgs &a, double *out, hipStream_t s) {
     });
 }
 
-hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double *result, hipStream_t s) {
+hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double *result, hipStream_t s, double *ring,
+                          unsigned *count) {
     return by_precision<1>(a.precision, [&](auto tag) -> hipError_t {
         using T = decltype(tag);
         const int nb = (int)std::min<int64_t>(nparts, (a.bins + 255) / 256);
@@ -508,7 +509,7 @@ hipError_t sp_observables(const SpArgs &a, double *partials, int nparts, double 
                                      : (a.p.dims == 2 ? go(std::integral_constant<int, 2>{})
                                                       : go(std::integral_constant<int, 3>{}));
         if(e != hipSuccess) return e;
-        return finalize_pairs(partials, nb, result, s);
+        return ring ? finalize_pairs_ring(partials, nb, ring, count, s) : finalize_pairs(partials, nb, result, s);
     });
 }
 

@@ -127,6 +127,11 @@ public:
     // average_free_energy and average_mass of the current state, read later
     // with fetch_observables (first in, first out)
     void queue_observables() { integrator->queue_observables(); }
+    // evolve(n), then queue_observables() (one captured graph for a recurring n)
+    void evolve_observe(long long n) {
+        mobility->update_mobility();
+        integrator->evolve_observe(n);
+    }
     void fetch_observables(double &F, double &m) {
         if(!integrator->fetch_observables(m, F)) critical("Encountered a nan while computing the total free energy");
     }

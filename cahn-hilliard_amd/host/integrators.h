@@ -123,6 +123,11 @@ public:
     }
     // the same pair, queued behind the issued steps and read later (FIFO)
     void queue_observables() { _check(ch_sys_observables_queue(_sys), "ch_sys_observables_queue"); }
+    // evolve(n), then queue_observables(): one captured graph for a recurring n
+    void evolve_observe(long long n) {
+        _check(ch_sys_evolve_observe(_sys, n), "ch_sys_evolve_observe");
+        _output_ready = false;
+    }
     bool fetch_observables(double &mass, double &fe) {
         ch_status st = ch_sys_observables_fetch(_sys, &mass, &fe);
         if(st == CH_ENAN) return false;

@@ -89,6 +89,7 @@ public:
                 pending.pop_front();
             }
         };
+        bool queued = false;  // the observables of t were queued with the steps that led to t
         while(t < t_end) {
             if(_printer->should_print_last(t)) _print_current_state("last_", t);
             if(_printer->should_print_traj(t)) {
@@ -104,7 +105,7 @@ public:
                     double F = _system->average_free_energy(), m = _system->average_mass();
                     energy_line(t, F, m);
                 }
-                else {
+                else if(!queued) {
                     write_pending(CH_OBS_QUEUE - 1);
                     _system->queue_observables();
                     pending.push_back(t);
@@ -114,7 +115,16 @@ public:
             long long next = _printer->next_event(t, t_end);
             if(_print_mass_every > 0) next = std::min(next, (t / _print_mass_every + 1) * _print_mass_every);
             next = std::max(next, t + 1);
-            _system->evolve(next - t);
+            // an energy line due at `next`: its reductions go with the steps
+            queued = _print_mass_every > 0 && !_print_pressure && next < t_end && next % _print_mass_every == 0;
+            if(queued) {
+                write_pending(CH_OBS_QUEUE - 1);
+                _system->evolve_observe(next - t);
+                pending.push_back(next);
+            }
+            else {
+                _system->evolve(next - t);
+            }
             t = next;
             write_pending(1);
         }

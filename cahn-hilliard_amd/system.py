@@ -324,6 +324,11 @@ class System:
         fetch_observables (first in, first out)."""
         self._chk(L.lib().ch_sys_observables_queue(self.handle))
 
+    def evolve_observe(self, nsteps):
+        """evolve(nsteps) then queue_observables(), as one captured graph when
+        the count recurs (ch_sys_evolve_observe)."""
+        self._chk(L.lib().ch_sys_evolve_observe(self.handle, int(nsteps)))
+
     def fetch_observables(self):
         m, f = ctypes.c_double(), ctypes.c_double()
         self._chk(L.lib().ch_sys_observables_fetch(self.handle, ctypes.byref(m), ctypes.byref(f)))

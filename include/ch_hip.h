@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CH_ABI_VERSION 8
+#define CH_ABI_VERSION 9
 
 typedef enum ch_status {
     CH_OK = 0,
@@ -203,6 +203,13 @@ ch_status ch_sys_observables(ch_sys *sys, double *mass, double *free_energy);
 ch_status ch_sys_observables_queue(ch_sys *sys);
 ch_status ch_sys_observables_fetch(ch_sys *sys, double *mass, double *free_energy);
 int32_t ch_sys_observables_pending(const ch_sys *sys);
+/* ch_sys_evolve(sys, nsteps) followed by ch_sys_observables_queue(sys): one
+ * print interval of the reference's run loop (src/main.cpp:120-128, the
+ * reductions of CahnHilliard.cpp:285-319 after CahnHilliard::evolve).  For the
+ * stencil integrators a count that recurs from the same buffer becomes one
+ * captured graph of the steps and the reductions (no launch gap between
+ * them); same values as the two calls.  No reference counterpart. */
+ch_status ch_sys_evolve_observe(ch_sys *sys, int64_t nsteps);
 
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under stream capture, tight serving loops, harnesses that
@@ -210,7 +217,7 @@ int32_t ch_sys_observables_pending(const ch_sys *sys);
  * bits) on a field the caller holds on the device: `rho_dev` is N^dims
  * elements of desc->precision, desc a single-species model (CH_EUNSUPPORTED
  * otherwise).  No allocation: the partial sums live in one fixed array of
- * 16,416 bytes per device, set up by ch_ctx_create; fields of more than 2^31
+ * 16,400 bytes per device, set up by ch_ctx_create; fields of more than 2^31
  * cells are refused (CH_EINVAL).  Calls on one device share that array, so
  * calls that overlap on one device race.  Synchronous; CH_ENAN as
  * ch_sys_observables. */

@@ -28,7 +28,7 @@ def test_every_header_in_include_is_exported(ch):
 
 def test_abi_version_and_status_strings_without_gpu(ch):
     L = ch._lib.lib()
-    assert L.ch_abi_version() == 8
+    assert L.ch_abi_version() == 9
     assert L.ch_status_string(3) == b"NaN encountered"
 
 

@@ -259,6 +259,48 @@ def test_queued_observables_equal_synchronous_ones(ch, ctx):
     b.close()
 
 
+@pytest.mark.parametrize("kw", [dict(dims=2, N=1024), dict(dims=2, N=128, integrator="euler_mobility",
+                                                             mobility="regularised", rho_min=0.5),
+                                dict(dims=3, N=32, dt=0.001), dict(dims=2, N=64, integrator="pseudospectral", dt=0.05)],
+                         ids=["2d_1024", "2d_em", "3d", "spectral"])
+def test_evolve_observe_equals_evolve_then_queue(ch, ctx, kw):
+    """ch_sys_evolve_observe: the fields and the queued pairs of evolve + queue,
+    for a recurring count (direct on first sight, captured on the second,
+    replayed after), mixed with other counts, the queue running past its 8
+    slots, and the synchronous observables in between."""
+    kw = dict(dict(dt=0.01), **kw)
+    N, dims = kw["N"], kw["dims"]
+    rho0 = O.init_density(dims, N, 0.5 if kw.get("mobility") else 0.0, 0.3, 0, 8)
+    counts = [100, 100, 100, 37, 100, 100, 36, 100, 100, 100, 100, 100]
+    a = ch.System(ctx, ch.make_desc(**kw))
+    a.upload(rho0)
+    want, sync_vals = [], []
+    for i, n in enumerate(counts):
+        a.evolve(n)
+        a.queue_observables()
+        want.append(a.fetch_observables())
+        if i == 5:
+            sync_vals.append(a.observables())
+    fa = a.download()
+    a.close()
+    b = ch.System(ctx, ch.make_desc(**kw))
+    b.upload(rho0)
+    got = []
+    for i, n in enumerate(counts):
+        b.evolve_observe(n)
+        if b.observables_pending > 2:
+            got.append(b.fetch_observables())
+        if i == 5:
+            sync_vals.append(b.observables())
+    while b.observables_pending:
+        got.append(b.fetch_observables())
+    fb = b.download()
+    b.close()
+    assert got == want
+    assert sync_vals[0] == sync_vals[1]
+    assert np.array_equal(fa, fb)
+
+
 def test_observables_nan_status(ch, ctx):
     s = ch.System(ctx, ch.make_desc(dims=2, N=16))
     rho = np.zeros(256)
