@@ -1259,8 +1259,8 @@ struct ch_batch {
     std::vector<chd::BatchRep> host_reps;
     chd::BatchRep *reps = nullptr;
     // work list per region width (64, 128 or 192 columns) and K = 1, 2, 4, 8
-    int2 *items[chd::kBatchWidths][kBatchDepths] = {};
-    int nitems[chd::kBatchWidths][kBatchDepths] = {};
+    int2 *items[chd::kBatchFamilies][kBatchDepths] = {};
+    int nitems[chd::kBatchFamilies][kBatchDepths] = {};
     void *buf[2] = {nullptr, nullptr};
     int compensation = CH_COMP_NONE;  // CH_COMP_* (FP32 only)
     // CH_COMP_STORED / CH_COMP_LOW8: the low-order parts of each replica's
@@ -1382,7 +1382,7 @@ extern "C" ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *d
         }
         if(esh) b->shape = std::max(0, std::min(chd::kBatchShapes - 1, std::atoi(esh)));
     }
-    std::vector<int2> items[chd::kBatchWidths][kBatchDepths];
+    std::vector<int2> items[chd::kBatchFamilies][kBatchDepths];
     int64_t maxbins = 0;
     for(int r = 0; r < n; r++) {
         chd::BatchRep rep{};
@@ -1390,14 +1390,14 @@ extern "C" ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *d
         rep.off = b->total;
         const int N = descs[r].N;
         rep.whole = N == chd::kBatchRegion ? 1 : 0;
-        for(int fam = 0; fam < chd::kBatchWidths; fam++)
+        for(int fam = 0; fam < chd::kBatchFamilies; fam++)
             for(int i = 0; i < kBatchDepths; i++) {
                 if(rep.whole) {
                     items[fam][i].push_back(int2{r, 0});
                     continue;
                 }
-                // shapes 0, 3 and 5 stand for the 64-, 128- and 192-column families
-                const int Tx = chd::batch_tile_x(fam == 0 ? 0 : (fam == 1 ? 3 : 5), 1 << i), Ty = chd::batch_tile_y(1 << i);
+                const int fs = chd::batch_family_shape(fam);
+                const int Tx = chd::batch_tile_x(fs, 1 << i), Ty = chd::batch_tile_y(fs, 1 << i);
                 const int ntx = (N + Tx - 1) / Tx, nty = (N + Ty - 1) / Ty;
                 for(int ty = 0; ty < nty; ty++)
                     for(int tx = 0; tx < ntx; tx++) items[fam][i].push_back(int2{r, tx | (ty << 16)});
@@ -1418,7 +1418,7 @@ extern "C" ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *d
         ok = hipMalloc(&b->comp[0], bytes) == hipSuccess && hipMalloc(&b->comp[1], bytes) == hipSuccess &&
              hipMemset(b->comp[0], 0, bytes) == hipSuccess && hipMemset(b->comp[1], 0, bytes) == hipSuccess;
     }
-    for(int fam = 0; fam < chd::kBatchWidths; fam++)
+    for(int fam = 0; fam < chd::kBatchFamilies; fam++)
         for(int i = 0; ok && i < kBatchDepths; i++) {
             const std::vector<int2> &v = items[fam][i];
             b->nitems[fam][i] = (int)v.size();
@@ -1484,7 +1484,7 @@ extern "C" ch_status ch_batch_download(ch_batch *b, int32_t r, double *rho_host)
 }
 
 static hipError_t batch_launch(ch_batch *b, int shape, int K, int from) {
-    const int i = log2i(K), fam = chd::batch_columns_per_lane(shape) - 1;
+    const int i = log2i(K), fam = chd::batch_family(shape);
     return chd::batch_launch(b->precision, b->model, b->compensation, shape, K, b->buf[from], b->buf[from ^ 1], b->reps,
                              b->items[fam][i], b->nitems[fam][i], b->comp[from], b->comp[from ^ 1], b->ctx->stream);
 }

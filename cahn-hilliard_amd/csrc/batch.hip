@@ -28,9 +28,9 @@ template <typename T, int MODEL, int K, int C, int RPW, int NW, int UPD = UPD_PL
 __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T *__restrict__ bufB,
                                                   const BatchRep *__restrict__ reps, const int2 *__restrict__ items,
                                                   const void *__restrict__ compA, void *__restrict__ compB) {
-    static_assert(RPW * NW == kBatchRegion, "batch regions are 64 rows high");
+    static_assert(RPW * NW >= kBatchRegion && kBatchRegion % NW == 0, "whole items split 64 rows over the waves");
     using HALO = WaveCfg<C, RPW, NW>;
-    using WHOLE = WaveCfg<1, RPW, NW>;
+    using WHOLE = WaveCfg<1, kBatchRegion / NW, NW>;
     constexpr int W = HALO::RW - 4 * K, TH = HALO::RH - 4 * K;
     // one exchange buffer, seen with the whole region's (narrower) layout too
     __shared__ WaveExchange<T, 2, NW, C> ex;
@@ -54,13 +54,16 @@ __global__ __launch_bounds__(64 * NW) void k_batch(const T *__restrict__ bufA, T
     }
 }
 
-int batch_columns_per_lane(int shape) { return shape >= 5 ? 3 : (shape >= 3 ? 2 : 1); }
+int batch_family(int shape) { return shape == 7 ? 3 : (shape >= 5 ? 2 : (shape >= 3 ? 1 : 0)); }
+int batch_family_shape(int family) { return family == 3 ? 7 : (family == 2 ? 5 : (family == 1 ? 3 : 0)); }
+static int batch_columns_per_lane(int shape) { return shape >= 5 ? 3 : (shape >= 3 ? 2 : 1); }
 int batch_tile_x(int shape, int K) { return 64 * batch_columns_per_lane(shape) - 4 * K; }
-int batch_tile_y(int K) { return kBatchRegion - 4 * K; }
+int batch_tile_y(int shape, int K) { return (shape == 7 ? 80 : kBatchRegion) - 4 * K; }
 
 // shape s (rows per wave x waves; halo regions of 64, 128 or 192 columns):
 //   0 = 16 x 4, 1 = 8 x 8, 2 = 4 x 16 (64 columns);  3 = 4 x 16, 4 = 8 x 8 (128 columns);
-//   5 = 4 x 16, 6 = 8 x 8 (192 columns)
+//   5 = 4 x 16, 6 = 8 x 8 (192 columns);  7 = 5 x 16 (192 columns, 80 rows: at K = 4
+//   the 64-row interiors tile every power-of-two N >= 64 exactly)
 template <typename T, int MODEL, int K, int UPD>
 static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchRep *reps, const int2 *items,
                                  int nitems, const void *ca, void *cb, hipStream_t s) {
@@ -76,6 +79,7 @@ static hipError_t launch_batch_k(int shape, const void *a, void *b, const BatchR
     case 4: hipLaunchKernelGGL((k_batch<T, MODEL, K, 2, 8, 8, UPD>), dim3(nitems), dim3(512), 0, s, A, Bf, reps, items, CA, CB); break;
     case 5: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 4, 16, UPD>), dim3(nitems), dim3(1024), 0, s, A, Bf, reps, items, CA, CB); break;
     case 6: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 8, 8, UPD>), dim3(nitems), dim3(512), 0, s, A, Bf, reps, items, CA, CB); break;
+    case 7: hipLaunchKernelGGL((k_batch<T, MODEL, K, 3, 5, 16, UPD>), dim3(nitems), dim3(1024), 0, s, A, Bf, reps, items, CA, CB); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -143,14 +143,17 @@ struct BatchRep {
     int64_t off;   // element offset of the replica inside each ping-pong buffer
     int whole;     // 1: N == 64, one periodic item
 };
-// batch regions are kBatchRegion rows high and 64, 128 or 192 columns wide
-// (shape); interior of a K-step halo item
+// batch regions (shape): 64 rows high and 64, 128 or 192 columns wide
+// (shapes 0-6), or 80 rows and 192 columns (shape 7); a whole N = 64 replica
+// is one 64 x 64 item.  The items of a K-step launch cover each replica with
+// the interiors of one family (region size) of halo regions.
 constexpr int kBatchRegion = 64;
-constexpr int kBatchShapes = 7;
-constexpr int kBatchWidths = 3;
-int batch_columns_per_lane(int shape);
-int batch_tile_x(int shape, int K);
-int batch_tile_y(int K);
+constexpr int kBatchShapes = 8;
+constexpr int kBatchFamilies = 4;
+int batch_family(int shape);         // 0: 64 x 64, 1: 128 x 64, 2: 192 x 64, 3: 192 x 80
+int batch_family_shape(int family);  // a shape of the family (its region size)
+int batch_tile_x(int shape, int K);  // interior of a K-step halo item
+int batch_tile_y(int shape, int K);
 hipError_t batch_launch(int precision, int model, int upd, int shape, int K, const void *a, void *b,
                         const BatchRep *reps, const int2 *items, int nitems, const void *ca, void *cb, hipStream_t s);
 // per-replica FP64 observables: result[2 r] = sum rho, result[2 r + 1] = sum f-density
