@@ -2,7 +2,9 @@
 //
 // Follows src/utils/Printer.cpp:13-262: trajectory cadence "linear"
 // (print_trajectory_every, print_last_every defaulting to it) or "log"
-// (output.log_n0, output.log_fact); native files start with
+// (output.log_n0, output.log_fact); with output.print_pressure, pressure
+// frames every output.print_pressure_every steps or on the log schedule
+// (output.print_pressure_strategy); native files start with
 //   # step = S, t = T, size = NXxNYxNZ | obs
 // followed by the values at the default ostream precision (6 significant
 // digits), a newline every N values; VTK as STRUCTURED_POINTS ASCII.
@@ -18,20 +20,11 @@
 #include <set>
 #include <string>
 
-#include "SimulationState.h"
 #include "Object.h"
+#include "SimulationState.h"
+#include "fmt_general.h"
 
 namespace ch {
-
-// fmt's "{:.P}" for a double: general format with P significant digits and
-// at least one digit after the point when no exponent is used.
-inline std::string fmt_general(double v, int prec) {
-    char buf[64];
-    std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
-    std::string s(buf);
-    if(s.find_first_of(".eEni") == std::string::npos) s += ".0";
-    return s;
-}
 
 template <int dims>
 class Printer : public Object {
@@ -51,6 +44,21 @@ public:
             _last_every = _config_value<long long>(config, "print_last_every");
         }
         else critical("Unsupported trajectory printing strategy '{}'", _strategy);
+
+        // pressure frames (output.print_pressure): every print_pressure_every
+        // steps, or on the log schedule round(log_n0 log_fact^n) (Printer.cpp:33-46)
+        _print_pressure = _config_optional_value<bool>(config, "output.print_pressure", false);
+        if(_print_pressure) {
+            _pressure_strategy = _config_optional_value<std::string>(config, "output.print_pressure_strategy", "linear");
+            if(_pressure_strategy == "linear") {
+                _pressure_every = _config_value<long long>(config, "output.print_pressure_every");
+            }
+            else if(_pressure_strategy == "log") {
+                _log_n0 = _config_value<int>(config, "output.log_n0");
+                _log_fact = _config_value<double>(config, "output.log_fact");
+            }
+            else critical("Unsupported pressure printing strategy '{}'", _pressure_strategy);
+        }
 
         std::string outp = _config_optional_value<std::string>(config, "output.path", ".");
         _out = std::filesystem::path(outp);
@@ -72,6 +80,14 @@ public:
     }
 
     bool should_print_last(long long t) const { return _last_every > 0 && t % _last_every == 0; }
+    // Printer.cpp:157-170; the log schedule counts pressure frames on their own
+    // (pressure_frame_done), so it is the trajectory's schedule
+    bool should_print_pressure(long long t) const {
+        if(!_print_pressure) return false;
+        if(_pressure_strategy == "linear") return _pressure_every > 0 && t % _pressure_every == 0;
+        return (long long)std::round(_log_n0 * std::pow(_log_fact, _pressure_printed)) == t;
+    }
+    void pressure_frame_done() { _pressure_printed++; }
     bool should_print_traj(long long t) const {
         if(_strategy == "linear") return _traj_every > 0 && t % _traj_every == 0;
         long long next_t = (long long)std::round(_log_n0 * std::pow(_log_fact, _traj_printed));
@@ -83,12 +99,17 @@ public:
         auto upd = [&](long long every) {
             if(every > 0) best = std::min(best, (t / every + 1) * every);
         };
-        upd(_last_every);
-        if(_strategy == "linear") upd(_traj_every);
-        else {
-            long long nt = (long long)std::round(_log_n0 * std::pow(_log_fact, _traj_printed));
+        auto upd_log = [&](int printed) {
+            long long nt = (long long)std::round(_log_n0 * std::pow(_log_fact, printed));
             if(nt > t) best = std::min(best, nt);
             else best = std::min(best, t + 1);
+        };
+        upd(_last_every);
+        if(_strategy == "linear") upd(_traj_every);
+        else upd_log(_traj_printed);
+        if(_print_pressure) {
+            if(_pressure_strategy == "linear") upd(_pressure_every);
+            else upd_log(_pressure_printed);
         }
         return best;
     }
@@ -167,9 +188,10 @@ public:
 
 private:
     SimulationState<dims> &_state;
-    std::string _strategy;
-    long long _traj_every = 0, _last_every = 0;
-    int _log_n0 = 0, _traj_printed = 0;
+    std::string _strategy, _pressure_strategy;
+    long long _traj_every = 0, _last_every = 0, _pressure_every = 0;
+    int _log_n0 = 0, _traj_printed = 0, _pressure_printed = 0;
+    bool _print_pressure = false;
     double _log_fact = 1.0;
     bool _print_vtk = false;
     bool _lossless = false;

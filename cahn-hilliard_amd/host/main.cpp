@@ -3,7 +3,7 @@
 // Same input file, run loop and outputs as the reference driver
 // (src/main.cpp:20-200): srand48(seed); model factory on `free_energy`;
 // init_* snapshots; for t in [t0, t0 + steps): last_* every print_last_every,
-// trajectory frames, the energy line "t*dt F mass t" every print_every
+// trajectory and pressure frames, the energy line "t*dt F mass t" every print_every
 // (printed *before* the step, as the reference does), then evolve; last_*
 // at the end.  Steps between two output events are queued as one batch
 // (CahnHilliard::evolve(n)), which the library replays as captured graphs.
@@ -97,6 +97,13 @@ public:
                 for(int i = 0; i < _state.model->N_species(); i++)
                     _printer->add_to_trajectory("traj", "density", _state.rho, rho_factor, i, t);
                 _printer->trajectory_frame_done();
+            }
+            // pressure frames, one file per species (src/main.cpp:114-119)
+            if(_printer->should_print_pressure(t)) {
+                auto p = _system->pressure();
+                for(int i = 0; i < _state.model->N_species(); i++)
+                    _printer->add_to_trajectory("pressure", "pressure", p, rho_factor, i, t);
+                _printer->pressure_frame_done();
             }
             if(_print_mass_every > 0 && t % _print_mass_every == 0) {
                 if(_print_pressure) {

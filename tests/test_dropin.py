@@ -71,9 +71,10 @@ def test_reference_regression_on_dropin_binary(tmp_path, integrator, dims):
     assert abs(phi0.sum() - phi1.sum()) < 1e-4
     ref = np.load(os.path.join(GOLD, "reference_landau_fixtures.npz"))[f"{integrator}_{dims}D"]
     np.testing.assert_allclose(phi1.ravel(), ref, rtol=1e-12, atol=1e-14)
-    # the header line the reference writes (src/utils/Printer.cpp:183)
+    # the header line the reference writes (src/utils/Printer.cpp:183), as in its
+    # own fixture tests/data/reference/landau_euler_2D.dat: "# step = 100, t = 0.10000, size = 64x64x1"
     head = (tmp_path / "last_0.dat").read_text().split("\n", 1)[0]
-    assert head.startswith("# step = 100, t = 0.1, size = 64x") and head.endswith("| density")
+    assert head.startswith("# step = 100, t = 0.10000, size = 64x") and head.endswith("| density")
 
 
 @pytest.mark.gpu
@@ -87,7 +88,9 @@ def test_energy_file_and_restart(tmp_path):
     lines = (tmp_path / "energy.dat").read_text().split("\n")
     lines = [l for l in lines if l]
     assert [l.split()[3] for l in lines] == ["0", "100", "200"]
-    assert lines[0].split()[0] == "0.0" and lines[1].split()[0] == "1.0"
+    # "{:.5}" of fmt 7.1.3 (src/main.cpp:121): trailing zeros kept (tests/golden/fmt_vectors.json)
+    assert lines[0].split()[0] == "0.0000" and lines[1].split()[0] == "1.0000"
+    assert len(lines[0].split()[1].replace("-", "").replace(".", "").split("e")[0]) == 8
     assert r.stdout.split("\n")[0] == lines[0]
     import sys
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -227,3 +230,106 @@ def test_generic_wertheim_on_dropin_binary(tmp_path):
         assert np.abs(a - b).max() > 0
     e = np.loadtxt(tmp_path / "energy.dat")
     assert e.shape[0] == 4 and np.isfinite(e).all()  # t = 0, 50, 100, 150 (main.cpp:104-127)
+
+
+LANDAU_BASE = ("integrator = \"euler\"\nN = 32\ndt = 0.01\nk = 1.0\nfree_energy = \"landau\"\n"
+               "initial_density = 0.0\ninitial_A = 0.3\nseed = 7\n[landau]\nepsilon = 0.9\n")
+
+
+def native_frames(path):
+    """(step, header, values) of each frame of a native trajectory file."""
+    frames, cur = [], None
+    for line in path.read_text().split("\n"):
+        if line.startswith("# step = "):
+            cur = [int(line.split()[3].rstrip(",")), line, []]
+            frames.append(cur)
+        elif line.strip():
+            cur[2].extend(float(v) for v in line.split())
+    return [(s, h, np.array(v)) for s, h, v in frames]
+
+
+@pytest.mark.gpu
+def test_vtk_output(tmp_path):
+    """output.print_vtk (src/utils/Printer.cpp:209-262): init_/last_ VTK files and
+    one trajectory file per frame, STRUCTURED_POINTS ASCII, values in
+    std::scientific equal to the native files' values."""
+    traj = tmp_path / "traj"
+    traj.mkdir()
+    text = ("steps = 40\nprint_trajectory_every = 20\n" + LANDAU_BASE.replace("N = 32", "N = 16") +
+            f"[output]\nprint_vtk = true\ntrajectory_path = \"{traj}\"\n")
+    run_bin(2, text, tmp_path)
+    for name in ("init_0.vtk", "init_density.vtk", "last_0.vtk", "last_density.vtk"):
+        assert (tmp_path / name).exists(), name
+    assert sorted(p.name for p in traj.iterdir()) == ["traj_0_0.vtk", "traj_0_20.vtk"]
+    lines = (tmp_path / "last_0.vtk").read_text().split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0"
+    assert lines[1] == "step = 40, t = 0.40000, size = 16x16x1 | density"
+    assert lines[2:9] == ["ASCII", "DATASET STRUCTURED_POINTS", "DIMENSIONS 16 16 1", "ORIGIN 0 0 0",
+                          "SPACING 1 1 1", "POINT_DATA 256", "SCALARS species_0 double 1"]
+    assert lines[9] == "LOOKUP_TABLE default"
+    vals = [l for l in lines[10:] if l]
+    assert len(vals) == 256 and all("e" in v for v in vals)
+    native = np.loadtxt(tmp_path / "last_0.dat").ravel()
+    np.testing.assert_allclose(np.array([float(v) for v in vals]), native, rtol=1e-5)
+    # the 2D trajectory frame at t = 20 names its step in the header
+    assert (traj / "traj_0_20.vtk").read_text().split("\n")[1].startswith("step = 20, t = 0.20000,")
+
+
+@pytest.mark.gpu
+def test_log_trajectory_cadence(tmp_path):
+    """print_trajectory_strategy = "log" (Printer.cpp:145-155): frames at
+    t = round(log_n0 log_fact^n) for n = 0, 1, 2, ... (the counter advances
+    per frame; the reference never advances it and writes only t = log_n0)."""
+    text = ("steps = 50\nprint_trajectory_strategy = \"log\"\nprint_last_every = 25\n" + LANDAU_BASE +
+            "[output]\nlog_n0 = 2\nlog_fact = 1.5\n")
+    run_bin(2, text, tmp_path)
+    steps = [s for s, _, _ in native_frames(tmp_path / "traj_0.dat")]
+    want = []
+    n = 0
+    while round(2 * 1.5 ** n) < 50:
+        want.append(round(2 * 1.5 ** n))
+        n += 1
+    assert steps == want
+
+
+def landau_pressure(rho, N, k=1.0, eps=0.9, dx=1.0):
+    """p = psi f'(psi) - f(psi) - k/2 sum_d (forward difference)^2 (CahnHilliard.cpp:322-361)."""
+    f = -0.5 * eps * rho ** 2 + 0.25 * rho ** 4
+    fp = -eps * rho + rho ** 3
+    r = rho.reshape(N, N)
+    g2 = ((np.roll(r, -1, 1) - r) / dx) ** 2 + ((np.roll(r, -1, 0) - r) / dx) ** 2
+    return (rho * fp - f) - 0.5 * k * g2.ravel()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strategy", ["linear", "log"])
+def test_pressure_trajectory_cadence(tmp_path, strategy):
+    """output.print_pressure (Printer.cpp:33-46, 157-170; main.cpp:114-128):
+    pressure frames every print_pressure_every steps (linear) or on the log
+    schedule, one pressure_<species>.dat per species holding the pressure
+    field (the reference's add_to_trajectory writes the density there
+    instead); the energy lines gain the average pressure column."""
+    out = ("[output]\nprint_pressure = true\nprint_pressure_every = 20\n" if strategy == "linear" else
+           "[output]\nprint_pressure = true\nprint_pressure_strategy = \"log\"\nlog_n0 = 1\nlog_fact = 3\n")
+    text = "steps = 60\nprint_every = 20\n" + LANDAU_BASE + out
+    run_bin(2, text, tmp_path)
+    frames = native_frames(tmp_path / "pressure_0.dat")
+    steps = [s for s, _, _ in frames]
+    assert steps == ([0, 20, 40] if strategy == "linear" else [1, 3, 9, 27])
+    assert frames[0][1].endswith("| pressure")
+    if strategy == "linear":
+        rho0 = np.loadtxt(tmp_path / "init_0.dat").ravel()
+        np.testing.assert_allclose(frames[0][2], landau_pressure(rho0, 32), rtol=2e-5, atol=1e-9)
+    lines = [l for l in (tmp_path / "energy.dat").read_text().split("\n") if l]
+    assert [len(l.split()) for l in lines] == [5, 5, 5]
+    if strategy == "linear":
+        assert float(lines[0].split()[4]) == pytest.approx(frames[0][2].mean(), rel=1e-4, abs=1e-9)
+
+
+@pytest.mark.gpu
+def test_pressure_strategy_rejected(tmp_path):
+    """The Printer reads its keys after the system (and its device context) is
+    built, as the reference's Manager does (src/main.cpp:20-89)."""
+    text = "steps = 1\n" + LANDAU_BASE + "[output]\nprint_pressure = true\nprint_pressure_strategy = \"cubic\"\n"
+    r = run_bin(2, text, tmp_path, check=False)
+    assert r.returncode != 0 and "Unsupported pressure printing strategy 'cubic'" in (r.stdout + r.stderr)

@@ -13,7 +13,12 @@ restatement is pinned by:
     rfftn / irfftn, whose c2r keeps only the real part of the DC and Nyquist
     entries of each last-axis line, as FFTW's does), the GMRES scheme against
     a dense direct solve of A x = b;
-  * total mass unchanged (the divergence has no k = 0 component).
+  * total mass unchanged (the divergence has no k = 0 component);
+  * consistency, fixed by the mathematics rather than by a restatement: on a
+    band-limited field with varying mobility, (rho_1 - rho_0) / dt of each
+    scheme converges at first order to div(M grad mu_0) (mu_0 in closed form,
+    the operator pinned by closed-form derivatives), which a wrong sign of the
+    (M - M0) term cannot reach, and the two schemes' steps differ by O(dt^2).
 """
 import numpy as np
 import pytest
@@ -146,3 +151,60 @@ def test_mass_is_conserved(use_gmres):
     out = O.run_psm(p, m, rho0, 20, use_gmres=use_gmres)
     assert abs(out.sum() - rho0.sum()) < 1e-12 * rho0.sum()
     assert np.abs(out - rho0).max() > 1e-4
+
+
+def smooth_state(N, dx, eps, kap):
+    """A band-limited field rho0 (a few low modes around 0.5) with its chemical
+    potential mu0 = f'(rho0) - 2 kap lap(rho0) in closed form (f' = -eps rho + rho^3
+    has three times the modes, still resolved at N = 32)."""
+    L = N * dx
+    k = 2 * np.pi / L
+    x = np.arange(N) * dx
+    Y, X = np.meshgrid(x, x, indexing="ij")  # [y, x]: x fastest, as idx = x + N y
+    rho = 0.5 + 0.1 * np.cos(k * X) + 0.05 * np.sin(2 * k * Y) + 0.03 * np.cos(k * X + k * Y)
+    lap = -k * k * 0.1 * np.cos(k * X) - 4 * k * k * 0.05 * np.sin(2 * k * Y) - 2 * k * k * 0.03 * np.cos(k * X + k * Y)
+    mu = -eps * rho + rho ** 3 - 2 * kap * lap
+    return rho.ravel(), mu.ravel()
+
+
+@pytest.mark.parametrize("use_gmres", [False, True])
+def test_step_converges_to_div_mobility_grad_mu(use_gmres):
+    """Consistency of each scheme with d rho / dt = div(M(rho) grad mu): as dt -> 0,
+    (rho_1 - rho_0) / dt tends to div(M grad mu_0) at first order (the error halves
+    with dt), mu_0 in closed form and div(M grad .) the operator pinned above by
+    closed-form derivatives.  With the sign of the (M - M0) term flipped the limit
+    would be div((2 M0 - M) grad mu_0), which this check tells apart."""
+    N, dx, eps, kap, S, Mc, rho_min = 32, 1.0, 0.9, 1.0, 0.5, 1.0, 0.2
+    rho0, mu0 = smooth_state(N, dx, eps, kap)
+    m = O.Model(O.LANDAU, epsilon=eps)
+    base = O.Params(dims=2, N=N, dx=dx, k=kap, mobility=O.MOB_REGULARISED, M=Mc, rho_min=rho_min, S=S)
+    mob = Mc * rho0 / (rho0 + rho_min)
+    want = O.psm_div_mob_grad(base, mob, mu0)
+    flipped = O.psm_div_mob_grad(base, 2 * mob.max() - mob, mu0)
+    errs = []
+    for dt in (4e-3, 2e-3, 1e-3):
+        p = O.Params(dims=2, N=N, dt=dt, dx=dx, k=kap, mobility=O.MOB_REGULARISED, M=Mc, rho_min=rho_min, S=S)
+        rho1 = O.run_psm(p, m, rho0, 1, use_gmres=use_gmres, restart=40, max_iter=400, tol=1e-14)
+        errs.append(np.abs((rho1 - rho0) / dt - want).max())
+    print(f"use_gmres={use_gmres}: errors {errs}, |limit| {np.abs(want).max():.3e}, "
+          f"|flipped - limit| {np.abs(flipped - want).max():.3e}")
+    assert 0.4 < errs[1] / errs[0] < 0.6 and 0.4 < errs[2] / errs[1] < 0.6
+    assert errs[2] < 0.02 * np.abs(want).max()
+    assert errs[2] < 0.01 * np.abs(flipped - want).max()
+
+
+def test_splitting_and_gmres_agree_to_second_order():
+    """Both schemes are first-order consistent with the same equation, so one
+    step of each from the same state differs by O(dt^2): the difference drops
+    by four when dt halves."""
+    N, dx, eps, kap, S, Mc, rho_min = 32, 1.0, 0.9, 1.0, 0.5, 1.0, 0.2
+    rho0, _ = smooth_state(N, dx, eps, kap)
+    m = O.Model(O.LANDAU, epsilon=eps)
+    diffs = []
+    for dt in (4e-3, 2e-3, 1e-3):
+        p = O.Params(dims=2, N=N, dt=dt, dx=dx, k=kap, mobility=O.MOB_REGULARISED, M=Mc, rho_min=rho_min, S=S)
+        a = O.run_psm(p, m, rho0, 1)
+        b = O.run_psm(p, m, rho0, 1, use_gmres=True, restart=40, max_iter=400, tol=1e-14)
+        diffs.append(np.abs(a - b).max())
+    print("splitting - gmres:", diffs)
+    assert 0.2 < diffs[1] / diffs[0] < 0.3 and 0.2 < diffs[2] / diffs[1] < 0.3
