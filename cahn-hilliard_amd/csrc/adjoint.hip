@@ -550,8 +550,9 @@ static hipError_t tape_tiles(const Params &p, int K, const void *in, void *out, 
     }
     else {
         if(N >= 16) {
-            // the forward's 3D tile (Params::tile3d)
-            if(p.tile3d == 1) return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1, 512>(p, in, out, s);
+            // the forward's 3D tile (Params::tile3d; the two-launch forward, 3,
+            // records its states with the 512-thread tile: the same bits)
+            if(p.tile3d == 1 || p.tile3d == 3) return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1, 512>(p, in, out, s);
             if(p.tile3d == 2) return launch_tape<T, 3, MODEL, MOB, EM, 16, 16, 4, 1, 512>(p, in, out, s);
             return launch_tape<T, 3, MODEL, MOB, EM, 16, 8, 8, 1>(p, in, out, s);
         }

@@ -259,6 +259,10 @@ static chd::Params make_params(const ch_desc *d) {
     // measured on C5 (SimpleWertheim 3D N=128 euler_mobility): 16x8x8 with 512
     // threads and the 16x8x4 adjoint tile beat the other shapes
     p.tile3d = 1;
+    // SimpleWertheim 3D: the two-launch step evaluates each cell's FP64 f' once
+    // (C5, N=128 linear mobility: 34.7 against 41.1 us per step with the tile);
+    // Landau's cheap f' keeps the fused tile ahead (18.3 against 21.7 us)
+    if(d->dims == 3 && d->model == CH_SIMPLE_WERTHEIM && chd::split3d_ok(p, d->integrator, d->mobility)) p.tile3d = 3;
     p.adj_tile3d = 1;
     // noise step: 64 x 32 (2D) and 16 x 8 x 8 (3D) tiles of 512 threads measured
     // 10 % and 30 % faster than 64 x 16 and 8 x 8 x 8 with 256
@@ -389,7 +393,7 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
     // 0.2674 with the 192 x 64 regions of the full launches)
     s->tail_cfg = N >= 1024 ? 7 : s->p.wave_cfg;
     if(const char *r = std::getenv("CH_WAVE_CFG_TAIL")) s->tail_cfg = std::max(0, std::min(chd::kWaveCfgs - 1, std::atoi(r)));
-    if(const char *r = std::getenv("CH_TILE3D")) s->p.tile3d = std::max(0, std::min(2, std::atoi(r)));
+    if(const char *r = std::getenv("CH_TILE3D")) s->p.tile3d = std::max(0, std::min(3, std::atoi(r)));
     if(const char *r = std::getenv("CH_ADJ_TILE3D")) s->p.adj_tile3d = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_NOISE_TILE")) s->p.noise_tile = std::max(0, std::min(2, std::atoi(r)));
     if(const char *r = std::getenv("CH_PSM_FUSE")) s->psm_fuse = std::atoi(r) != 0;
@@ -700,7 +704,12 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K, int noise_off = 0) {
     switch(d.dims) {
     case 1: e = chd::stencil_step<1>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     case 2: e = chd::stencil_step<2>(p2, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
-    default: e = chd::stencil_step<3>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
+    default:
+        if(s->p.tile3d == 3 && chd::split3d_ok(s->p, d.integrator, d.mobility))
+            e = chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out, s->stage2, st);
+        else
+            e = chd::stencil_step<3>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st);
+        break;
     }
     cur ^= 1;
     return e;

@@ -19,7 +19,7 @@ struct Params {
     int use_dealias;
     int wave_cfg;          // region shape of the 2D wave kernel (0 .. kWaveCfgs - 1)
     int adj_cfg;           // region shape of the 2D wave adjoint (0, 3, 4 or 5)
-    int tile3d;            // 3D LDS tile of the forward step (0: 16x8x8 / 256 threads, 1 (default): the same / 512, 2: 16x16x4 / 512)
+    int tile3d;            // 3D forward step: LDS tile 0: 16x8x8 / 256 threads, 1: the same / 512, 2: 16x16x4 / 512; 3: two launches (stencil_split.hip)
     int adj_tile3d;        // 3D tile of the euler_mobility adjoint (0: 16x4x4 / 256, 1 (default): 16x8x4 / 512, 2: 16x8x8 / 512)
     int noise_tile;        // tile of the noise step (0: 2D 64x16, 3D 8x8x8 / 256 threads; 1: the same / 512; 2 (default): 2D 64x32, 3D 16x8x8 / 512)
     double eps;            // Landau epsilon

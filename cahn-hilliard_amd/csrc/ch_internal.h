@@ -10,6 +10,12 @@ namespace chd {
 template <int DIMS>
 hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, int K, const void *in,
                         void *out, hipStream_t s);
+// the 3D euler / euler_mobility step as two launches (mu of every cell into
+// `work`, N^3 elements, then the update): Params::tile3d == 3; not for
+// free-energy mobility (split3d_ok)
+bool split3d_ok(const Params &p, int integrator, int mob);
+hipError_t stencil_split3d(const Params &p, int precision, int integrator, int model, int mob, const void *in,
+                           void *out, void *work, hipStream_t s);
 inline int stencil_max_k(int dims, int N) { return (dims == 2 && N >= 64) ? 8 : 1; }
 // one euler_mobility step with conserved noise (noise.hip), in -> out; the
 // step index is *step_base + step_off.  noise_step_advance adds n to *step_base.

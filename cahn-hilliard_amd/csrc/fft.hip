@@ -597,25 +597,39 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
     }
 }
 
+// entry k of the c2r pre-processing: the packed n-point inverse transform
+// input Z_k from X_k and X_{n-k} of a row of n + 1 half-spectrum entries
+// (table W_N^m, N = 2 n)
+template <typename T2>
+__device__ __forceinline__ T2 c2r_entry(const T2 *__restrict__ in, long ks, int k, int n, const T2 *__restrict__ tw) {
+    T2 xk = in[(size_t)k * ks];
+    T2 xnk = in[(size_t)(n - k) * ks];
+    if(k == 0) { xk.y = 0; xnk.y = 0; }
+    // E = X_k + conj X_{n-k}; O = (X_k - conj X_{n-k}) conj(W_N^k); Z = E + i O
+    const T2 Ev = T2{xk.x + xnk.x, xk.y - xnk.y};
+    const T2 Ov = cmul(T2{xk.x - xnk.x, xk.y + xnk.y}, conj_(tw[k]));
+    return T2{Ev.x - Ov.y, Ev.y + Ov.x};
+}
+
+// entry k of the r2c post-processing: X_k from Z_k and Z_{n-k} and w = W_N^k
+template <typename T, typename T2>
+__device__ __forceinline__ T2 r2c_entry(T2 zk, T2 znk, T2 w) {
+    // E = (Z_k + conj Z_{n-k}) / 2, O = (Z_k - conj Z_{n-k}) / (2i), X_k = E + W_N^k O
+    const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
+    const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
+    const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
+    return cadd(Ev, cmul(w, Ov));
+}
+
 // c2r pre-processing of a row of n + 1 half-spectrum entries into the packed
 // n-point inverse transform input, in distribution D (k = lane + 64 m)
 template <typename T, int E>
 __device__ __forceinline__ void c2r_pre(const typename Complex<T>::type *__restrict__ in, long ks, int lane,
                                         const typename Complex<T>::type *__restrict__ tw,
                                         typename Complex<T>::type (&z)[E]) {
-    using T2 = typename Complex<T>::type;
     constexpr int n = 64 * E;
 #pragma unroll
-    for(int m = 0; m < E; m++) {
-        const int k = lane + 64 * m;
-        T2 xk = in[(size_t)k * ks];
-        T2 xnk = in[(size_t)(n - k) * ks];
-        if(k == 0) { xk.y = 0; xnk.y = 0; }
-        // E = X_k + conj X_{n-k}; O = (X_k - conj X_{n-k}) conj(W_N^k); Z = E + i O
-        const T2 Ev = T2{xk.x + xnk.x, xk.y - xnk.y};
-        const T2 Ov = cmul(T2{xk.x - xnk.x, xk.y + xnk.y}, conj_(tw[k]));
-        z[m] = T2{Ev.x - Ov.y, Ev.y + Ov.x};
-    }
+    for(int m = 0; m < E; m++) z[m] = c2r_entry(in, ks, lane + 64 * m, n, tw);
 }
 
 // r2c post-processing: Z (distribution D) -> n + 1 half-spectrum entries
@@ -628,13 +642,7 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
 #pragma unroll
     for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
     __syncthreads();
-    auto emit = [&](int k, T2 zk, T2 znk) {
-        // E = (Z_k + conj Z_{n-k}) / 2, O = (Z_k - conj Z_{n-k}) / (2i), X_k = E + W_N^k O
-        const T2 Ev = T2{(zk.x + znk.x) * (T)0.5, (zk.y - znk.y) * (T)0.5};
-        const T2 dd = T2{zk.x - znk.x, zk.y + znk.y};
-        const T2 Ov = T2{dd.y * (T)0.5, -dd.x * (T)0.5};
-        out[(size_t)k * ks] = cadd(Ev, cmul(tw[k], Ov));
-    };
+    auto emit = [&](int k, T2 zk, T2 znk) { out[(size_t)k * ks] = r2c_entry<T>(zk, znk, tw[k]); };
 #pragma unroll
     for(int m = 0; m < E; m++) {
         const int k = lane + 64 * m;

@@ -5,6 +5,7 @@ the same TOML input, the same output files, the same comparisons.
 CPU tests cover configuration errors that stop before any device work; the
 GPU tests rerun the reference's regression and mass tests on our binaries.
 """
+import math
 import os
 import shutil
 import subprocess
@@ -90,7 +91,7 @@ def test_energy_file_and_restart(tmp_path):
     assert [l.split()[3] for l in lines] == ["0", "100", "200"]
     # "{:.5}" of fmt 7.1.3 (src/main.cpp:121): trailing zeros kept (tests/golden/fmt_vectors.json)
     assert lines[0].split()[0] == "0.0000" and lines[1].split()[0] == "1.0000"
-    assert len(lines[0].split()[1].replace("-", "").replace(".", "").split("e")[0]) == 8
+    assert len(lines[0].split()[1].replace("-", "").split("e")[0].replace(".", "").lstrip("0")) == 8
     assert r.stdout.split("\n")[0] == lines[0]
     import sys
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -286,8 +287,9 @@ def test_log_trajectory_cadence(tmp_path):
     steps = [s for s, _, _ in native_frames(tmp_path / "traj_0.dat")]
     want = []
     n = 0
-    while round(2 * 1.5 ** n) < 50:
-        want.append(round(2 * 1.5 ** n))
+    # C's round(): halves away from zero (round(4.5) = 5), unlike Python's round
+    while math.floor(2 * 1.5 ** n + 0.5) < 50:
+        want.append(math.floor(2 * 1.5 ** n + 0.5))
         n += 1
     assert steps == want
 
