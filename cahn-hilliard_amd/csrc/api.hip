@@ -1400,17 +1400,9 @@ extern "C" ch_status ch_batch_create_ex(ch_ctx *ctx, int32_t n, const ch_desc *d
         const int N = descs[r].N;
         rep.whole = N == chd::kBatchRegion ? 1 : 0;
         for(int fam = 0; fam < chd::kBatchFamilies; fam++)
-            for(int i = 0; i < kBatchDepths; i++) {
-                if(rep.whole) {
-                    items[fam][i].push_back(int2{r, 0});
-                    continue;
-                }
-                const int fs = chd::batch_family_shape(fam);
-                const int Tx = chd::batch_tile_x(fs, 1 << i), Ty = chd::batch_tile_y(fs, 1 << i);
-                const int ntx = (N + Tx - 1) / Tx, nty = (N + Ty - 1) / Ty;
-                for(int ty = 0; ty < nty; ty++)
-                    for(int tx = 0; tx < ntx; tx++) items[fam][i].push_back(int2{r, tx | (ty << 16)});
-            }
+            for(int i = 0; i < kBatchDepths; i++)
+                for(int code : chd::batch_items(chd::batch_family_shape(fam), N, 1 << i))
+                    items[fam][i].push_back(int2{r, code});
         b->total += (int64_t)N * N;
         maxbins = std::max<int64_t>(maxbins, (int64_t)N * N);
         b->host_reps.push_back(rep);

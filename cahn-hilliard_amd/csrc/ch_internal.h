@@ -1,6 +1,8 @@
 // ch_internal.h -- host-side declarations shared by the library's .hip files.
 #pragma once
 
+#include <vector>
+
 #include "ch_device.h"
 
 namespace chd {
@@ -154,12 +156,13 @@ struct BatchRep {
 // is one 64 x 64 item.  The items of a K-step launch cover each replica with
 // the interiors of one family (region size) of halo regions.
 constexpr int kBatchRegion = 64;
-constexpr int kBatchShapes = 8;
-constexpr int kBatchFamilies = 4;
-int batch_family(int shape);         // 0: 64 x 64, 1: 128 x 64, 2: 192 x 64, 3: 192 x 80
+constexpr int kBatchShapes = 9;
+constexpr int kBatchFamilies = 5;
+int batch_family(int shape);         // 0: 64 x 64, 1: 128 x 64, 2: 192 x 64, 3: 192 x 80, 4: fitted per N (shape 8)
 int batch_family_shape(int family);  // a shape of the family (its region size)
 int batch_tile_x(int shape, int K);  // interior of a K-step halo item
 int batch_tile_y(int shape, int K);
+std::vector<int> batch_items(int shape, int N, int K);  // tx | ty << 16 per item of one replica
 hipError_t batch_launch(int precision, int model, int upd, int shape, int K, const void *a, void *b,
                         const BatchRep *reps, const int2 *items, int nitems, const void *ca, void *cb, hipStream_t s);
 // per-replica FP64 observables: result[2 r] = sum rho, result[2 r + 1] = sum f-density

@@ -105,17 +105,23 @@ __device__ __forceinline__ void row_neighbours(const T (&v)[C][RPW], int j, cons
 //               starts its increment sum from the stored part, and the store
 //               keeps the rounding remainder of psi in it, so the field holds
 //               about 32 significant bits and no rounding accumulates
+// XW (x-whole): the region spans the whole periodic x axis (N == 64 C, gx0 =
+// 0): x neighbours use DPP wave rotates and carry no halo, y keeps its 2K halo.
 enum { UPD_PLAIN = 0, UPD_LAUNCH = 1, UPD_STORED = 2, UPD_LOW8 = 3 };
-template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false, int UPD = UPD_PLAIN>
+template <typename T, int MODEL, int MOB, bool EM, int K, class CFG, bool WHOLE, bool TAPE = false, int UPD = UPD_PLAIN,
+          bool XW = false>
 __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restrict__ out, const Params &p, int gx0,
                                             int gy0, WaveExchange<T, exchange_fields<MOB, EM>(), CFG::NW, CFG::C> &ex,
                                             const void *__restrict__ cin = nullptr, void *__restrict__ cout = nullptr) {
     static_assert(UPD == UPD_PLAIN || (!EM && !TAPE), "compensated updates for the Euler integrator only");
     static_assert(UPD != UPD_LOW8 || sizeof(T) == 4, "8-bit low-order parts of FP32 fields only");
+    static_assert(!(XW && WHOLE), "x-whole regions keep a y halo");
     constexpr bool COMP = UPD == UPD_STORED, LOW8 = UPD == UPD_LOW8, ACC = UPD == UPD_LAUNCH || LOW8;
     constexpr int C = CFG::C, RPW = CFG::RPW, NW = CFG::NW, RW = CFG::RW, RH = CFG::RH;
     constexpr bool HASMOB = EM && MOB != CH_MOB_CONSTANT;
-    constexpr int H = WHOLE ? 0 : 2 * K;
+    constexpr int H = WHOLE ? 0 : 2 * K;          // y halo
+    constexpr int HX = (WHOLE || XW) ? 0 : 2 * K;  // x halo
+    constexpr bool ROTX = WHOLE || XW;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int N = p.N, mask = p.mask;
     // waves above and below, wrapping: in halo mode the first and last waves'
@@ -159,7 +165,7 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
             for(int c = 0; c < C; c++) {
                 const int lx = C * lane + c;
                 const int gx = gx0 + lx;
-                if(WHOLE || (lx >= H && lx < RW - H && gx < N)) {
+                if(WHOLE || XW || (lx >= HX && lx < RW - HX && gx < N)) {
                     const size_t at = (size_t)gy * N + gx;
                     if constexpr(ACC) {
                         const T v = psi0[c][j] + acc[c][j];
@@ -206,7 +212,7 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
 #pragma unroll
             for(int j = 0; j < RPW; j++) {
                 T m[C][2], q[C][2];
-                row_neighbours<T, C, RPW, WHOLE>(psi, j, above, below, m, q);
+                row_neighbours<T, C, RPW, ROTX>(psi, j, above, below, m, q);
 #pragma unroll
                 for(int c = 0; c < C; c++) {
                     mu[c][j] = chem_pot<T, 2, MODEL>(p, psi[c][j], m[c], q[c]);
@@ -239,10 +245,10 @@ __device__ __forceinline__ void wave_region(const T *__restrict__ in, T *__restr
 #pragma unroll
             for(int j = 0; j < RPW; j++) {
                 T m[C][2], q[C][2];
-                row_neighbours<T, C, RPW, WHOLE>(mu, j, mu_a, mu_b, m, q);
+                row_neighbours<T, C, RPW, ROTX>(mu, j, mu_a, mu_b, m, q);
                 if constexpr(HASMOB) {
                     T Mm[C][2], Mq[C][2];
-                    row_neighbours<T, C, RPW, WHOLE>(mob, j, M_a, M_b, Mm, Mq);
+                    row_neighbours<T, C, RPW, ROTX>(mob, j, M_a, M_b, Mm, Mq);
 #pragma unroll
                     for(int c = 0; c < C; c++)
                         psi[c][j] = mobility_update<T, 2>(p, psi[c][j], mu[c][j], m[c], q[c], mob[c][j], Mm[c], Mq[c]);

@@ -274,7 +274,9 @@ int32_t ch_batch_steps_per_launch(const ch_batch *batch);
  * batch kernel (rows per wave x waves over 64-row regions: shape 0: 16 x 4,
  * 1: 8 x 8, 2: 4 x 16 with 64 columns; 3: 4 x 16, 4: 8 x 8 with 128 columns,
  * two per lane; 5: 4 x 16, 6: 8 x 8 with 192 columns, three per lane; 7: 5 x 16
- * over 192 columns and 80 rows) at 2, 4
+ * over 192 columns and 80 rows; 8: regions fitted per replica size -- whole
+ * N = 64 and 128 replicas, N = 256 in regions of all 256 columns, others as 7)
+ * at 2, 4
  * and 8 steps per launch (4 and 8 for a compensated batch, whose accuracy
  * depends on the depth) on the batch's current fields and keeps the fastest
  * per time step; the fields are not changed.  Runs by itself on the first

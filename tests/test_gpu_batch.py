@@ -43,7 +43,8 @@ def test_batch_equals_single_systems(ch, ctx, monkeypatch, precision):
     descs, fields = make_replicas(ch, precision, Ns)
     for shape, tb in (("0", "1"), ("0", "2"), ("0", "4"), ("1", "4"), ("2", "4"), ("1", "8"), ("2", "8"),
                       ("3", "1"), ("3", "4"), ("3", "8"), ("4", "2"), ("4", "8"), ("5", "2"), ("5", "4"),
-                      ("5", "8"), ("6", "1"), ("6", "4"), ("7", "2"), ("7", "4"), ("7", "8")):
+                      ("5", "8"), ("6", "1"), ("6", "4"), ("7", "2"), ("7", "4"), ("7", "8"),
+                      ("8", "1"), ("8", "2"), ("8", "4"), ("8", "8")):
         monkeypatch.setenv("CH_BATCH_TB", tb)
         monkeypatch.setenv("CH_BATCH_SHAPE", shape)
         b = ch.Batch(ctx, descs, compensation="none")
@@ -109,7 +110,7 @@ def test_autotune_keeps_state_and_results(ch, ctx, tmp_path, monkeypatch):
         b.upload(r, f)
     shape, k = b.autotune()
     print(f"autotuned: shape {shape}, {k} steps per launch")
-    assert shape in range(8) and k in (2, 4, 8) and b.steps_per_launch == k
+    assert shape in range(9) and k in (2, 4, 8) and b.steps_per_launch == k
     for r, f in enumerate(fields):
         assert np.array_equal(b.download(r), f.astype(np.float32).astype(np.float64))
     b.evolve(45)
@@ -170,7 +171,7 @@ def test_compensated_split_invariance(ch, ctx, monkeypatch, mode):
     b.close()
     outs = {}
     for tb in ("4", "8"):
-        for shape in ("0", "1", "2", "3", "4", "5", "6", "7"):
+        for shape in ("0", "1", "2", "3", "4", "5", "6", "7", "8"):
             monkeypatch.setenv("CH_BATCH_SHAPE", shape)
             monkeypatch.setenv("CH_BATCH_TB", tb)
             b = ch.Batch(ctx, d32, compensation=mode)
