@@ -163,7 +163,7 @@ __global__ __launch_bounds__(NT) void k_adjoint_em(const T *__restrict__ lam_in,
         if constexpr(VARM) Mb[c] = mobility<T, MOB>(p, ps);
     }
     __syncthreads();
-    const T inv_dx2 = (T)p.inv_dx2, half = (T)0.5, Mc = (T)p.M;
+    const T inv_dx2 = (T)p.inv_dx2, Mc = (T)p.M;
     auto cell = [&](int lo, int c) {
         const int EX = RX - 2 * lo, EY = DIMS >= 2 ? RY - 2 * lo : 1;
         const int ex = c % EX, ey = (c / EX) % EY, ez = c / (EX * EY);
@@ -180,13 +180,15 @@ __global__ __launch_bounds__(NT) void k_adjoint_em(const T *__restrict__ lam_in,
         Mu[li] = chem_pot<T, DIMS, MODEL>(p, P[li], m, q);
         const T l = Lm[li];
         const T Mi = VARM ? Mb[li] : Mc;
-        T s = 0;
+        T lm[DIMS], lq[DIMS], Mm[DIMS], Mq[DIMS];
 #pragma unroll
         for(int d = 0; d < DIMS; d++) {
-            const T Mm = VARM ? Mb[li - off[d]] : Mc, Mq = VARM ? Mb[li + off[d]] : Mc;
-            s = s + half * (Mi + Mq) * (Lm[li + off[d]] - l) - half * (Mm + Mi) * (l - Lm[li - off[d]]);
+            Mm[d] = VARM ? Mb[li - off[d]] : Mc;
+            Mq[d] = VARM ? Mb[li + off[d]] : Mc;
+            lm[d] = Lm[li - off[d]];
+            lq[d] = Lm[li + off[d]];
         }
-        Nu[li] = s * inv_dx2;
+        Nu[li] = adj_nu<T, DIMS>(l, Mi, lm, lq, Mm, Mq, inv_dx2);
     }
     __syncthreads();
     double ge = 0, gk = 0, gm = 0, gd = 0;
@@ -199,18 +201,22 @@ __global__ __launch_bounds__(NT) void k_adjoint_em(const T *__restrict__ lam_in,
             pm[d] = P[li - off[d]]; pq[d] = P[li + off[d]];
         }
         const T nu = Nu[li], l = Lm[li], mu = Mu[li];
-        T se = 0;
+        T um[DIMS], uq[DIMS], lm[DIMS], lq[DIMS];
 #pragma unroll
-        for(int d = 0; d < DIMS; d++)
-            se = se + (Mu[li + off[d]] - mu) * (Lm[li + off[d]] - l) + (mu - Mu[li - off[d]]) * (l - Lm[li - off[d]]);
-        const T eta = -half * se * inv_dx2;
+        for(int d = 0; d < DIMS; d++) {
+            um[d] = Mu[li - off[d]];
+            uq[d] = Mu[li + off[d]];
+            lm[d] = Lm[li - off[d]];
+            lq[d] = Lm[li + off[d]];
+        }
+        const T eta = adj_eta<T, DIMS>(mu, l, um, uq, lm, lq, inv_dx2);
         const T lnu = laplacian<T, DIMS>(nu, nm, nq, inv_dx2);
         const double ps = (double)P[li];
         const MobDeriv<MOB> md = mob_deriv<MOB>(p, ps);
-        const T upd = (T)fsecond<MODEL>(p, ps) * nu - (T)p.two_k * lnu + (T)md.drho * eta;
         const int lx = li % RX, ly = (li / RX) % RY, lz = li / RXY;
         const int gx = tx * TX + lx - H, gy = DIMS >= 2 ? ty * TY + ly - H : 0, gz = DIMS == 3 ? tz * TZ + lz - H : 0;
-        lam_out[((size_t)gz * N + gy) * N + gx] = l + (T)p.dt * upd;
+        lam_out[((size_t)gz * N + gy) * N + gx] =
+            adj_em_step<T>(p, l, (T)fsecond<MODEL>(p, ps), nu, lnu, (T)md.drho, eta);
         const double lps = (double)laplacian<T, DIMS>(P[li], pm, pq, inv_dx2);
         ge += (double)nu * ps;
         gk += (double)nu * lps;

@@ -78,6 +78,41 @@ __device__ __forceinline__ MobDeriv<MOB> mob_deriv(const Params &p, double r) {
     return m;
 }
 
+// Per-cell terms of the euler_mobility reverse step (EulerMobilityCPU.cpp:44-82
+// reversed), written without contraction so that every kernel that shares
+// them (the LDS tile and the three-launch form) gives the same bits.
+// nu = D(Mob, lambda): the face-flux divergence of lambda with the mobility
+// Mi of the cell and Mm / Mq of its -e_d / +e_d neighbours
+template <typename T, int DIMS>
+__device__ __forceinline__ T adj_nu(T l, T Mi, const T (&Lm)[DIMS], const T (&Lq)[DIMS], const T (&Mm)[DIMS],
+                                    const T (&Mq)[DIMS], T inv_dx2) {
+#pragma clang fp contract(off)
+    const T half = (T)0.5;
+    T s = 0;
+#pragma unroll
+    for(int d = 0; d < DIMS; d++) s = s + half * (Mi + Mq[d]) * (Lq[d] - l) - half * (Mm[d] + Mi) * (l - Lm[d]);
+    return s * inv_dx2;
+}
+// eta = -1/2 sum_d [(mu_+ - mu)(lambda_+ - lambda) + (mu - mu_-)(lambda - lambda_-)] / dx^2:
+// the mobility's share of the reverse step
+template <typename T, int DIMS>
+__device__ __forceinline__ T adj_eta(T mu, T l, const T (&Um)[DIMS], const T (&Uq)[DIMS], const T (&Lm)[DIMS],
+                                     const T (&Lq)[DIMS], T inv_dx2) {
+#pragma clang fp contract(off)
+    const T half = (T)0.5;
+    T se = 0;
+#pragma unroll
+    for(int d = 0; d < DIMS; d++) se = se + (Uq[d] - mu) * (Lq[d] - l) + (mu - Um[d]) * (l - Lm[d]);
+    return -half * se * inv_dx2;
+}
+// lambda_n = lambda + dt (f'' nu - 2k lap nu + Mob' eta)
+template <typename T>
+__device__ __forceinline__ T adj_em_step(const Params &p, T l, T fpp, T nu, T lnu, T drho, T eta) {
+#pragma clang fp contract(off)
+    const T upd = fpp * nu - (T)p.two_k * lnu + drho * eta;
+    return l + (T)p.dt * upd;
+}
+
 __device__ __forceinline__ double wave_sum64(double v) {
 #pragma unroll
     for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -40,6 +40,10 @@ struct ch_sys {
     void *work = nullptr, *rho_hat = nullptr, *twiddles = nullptr;
     void *stage = nullptr;   // bins doubles: host transfers of FP32 fields, mobility
     void *stage2 = nullptr;  // bins elements of T: mobility
+    void *split_mob = nullptr;  // bins elements of T: the cells' mobilities of the two-launch 3D step
+    // the three-launch 3D euler_mobility reverse step: nu and eta, and the mu /
+    // mobility fields of the window recompute on the auxiliary stream
+    void *adj_split[4] = {nullptr, nullptr, nullptr, nullptr};
     double *partials = nullptr, *result = nullptr;
     // reverse mode: checkpoints, the window tape, lambda ping-pong, gradient slots
     void *ckpt = nullptr, *tape = nullptr, *lam[2] = {nullptr, nullptr};
@@ -339,9 +343,10 @@ static void free_sys(ch_sys *s) {
         if(s->oexec[i]) (void)hipGraphExecDestroy(s->oexec[i]);
         if(s->psi[i]) (void)hipFree(s->psi[i]);
     }
-    void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->partials, s->result,
+    void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->split_mob, s->partials, s->result,
                     s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc, s->ckpt_hat, s->ps_rh, s->ps_w,
-                    s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1]};
+                    s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1],
+                    s->adj_split[0], s->adj_split[1], s->adj_split[2], s->adj_split[3]};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
     for(void *b : s->psm_spec)
@@ -412,6 +417,9 @@ extern "C" ch_status ch_sys_create(ch_ctx *ctx, const ch_desc *desc, ch_sys **ou
               alloc(&s->stage2, s->bins * s->elt) && alloc_partials(&s->partials) &&
               alloc((void **)&s->result, 2 * sizeof(double));
     if(ok && !is_spectral(desc->integrator)) ok = alloc(&s->psi[1], s->bins * s->elt);
+    if(ok && s->p.tile3d == 3 && chd::split3d_ok(s->p, desc->integrator, desc->mobility) &&
+       chd::split3d_needs_mobility(desc->integrator, desc->mobility))
+        ok = alloc(&s->split_mob, s->bins * s->elt);
     if(ok && noisy) ok = alloc((void **)&s->noise_step, sizeof(int64_t)) &&
                          hipMemset(s->noise_step, 0, sizeof(int64_t)) == hipSuccess;
     if(ok && desc->integrator == CH_PSEUDOSPECTRAL_MOBILITY) {
@@ -706,7 +714,8 @@ static hipError_t launch_steps(ch_sys *s, int &cur, int K, int noise_off = 0) {
     case 2: e = chd::stencil_step<2>(p2, d.precision, d.integrator, d.model, d.mobility, K, in, out, st); break;
     default:
         if(s->p.tile3d == 3 && chd::split3d_ok(s->p, d.integrator, d.mobility))
-            e = chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out, s->stage2, st);
+            e = chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out, s->stage2,
+                                     s->split_mob, st);
         else
             e = chd::stencil_step<3>(s->p, d.precision, d.integrator, d.model, d.mobility, K, in, out, st);
         break;
@@ -1635,14 +1644,30 @@ static bool grow(void **buf, int64_t &cap, int64_t fields, size_t field_bytes) {
 
 static char *field_at(void *base, int64_t i, size_t field_bytes) { return (char *)base + (size_t)i * field_bytes; }
 
+// the 3D euler_mobility reverse sweep in three launches per step
+// (chd::adjoint_split3d) and its window recompute in two
+static bool adj_split(const ch_sys *s) {
+    return s->d.dims == 3 && s->d.integrator == CH_EULER_MOBILITY && chd::adjoint_split3d_ok(s->p, s->d.mobility);
+}
+
+// one recorded state: `aux` picks the work fields of the auxiliary stream
+static hipError_t tape_step1(ch_sys *s, const void *in, void *out, hipStream_t st, bool aux) {
+    const ch_desc &d = s->d;
+    if(adj_split(s))
+        return chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out,
+                                    aux ? s->adj_split[2] : s->stage2, aux ? s->adj_split[3] : s->split_mob, st);
+    return chd::tape_launch(s->p, d.precision, d.model, d.mobility, d.integrator == CH_EULER_MOBILITY, 1, in, out, st);
+}
+
 // `count` states after tape slot `from` of `tape` into slots from+1 .. from+count
-static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count, hipStream_t st) {
+static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count, hipStream_t st, bool aux) {
     const size_t fb = s->bins * s->elt;
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
     while(count > 0) {
         const int K = chd::tape_depth(s->d.dims, s->d.N, em, count);
-        hipError_t e = chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,
-                                        field_at(tape, from, fb), field_at(tape, from + 1, fb), st);
+        hipError_t e = K == 1 ? tape_step1(s, field_at(tape, from, fb), field_at(tape, from + 1, fb), st, aux)
+                              : chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,
+                                                 field_at(tape, from, fb), field_at(tape, from + 1, fb), st);
         if(e != hipSuccess) return e;
         from += K;
         count -= K;
@@ -1668,6 +1693,12 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     // checkpointed: two window tapes, one recomputed while the other is swept;
     // the pseudospectral full tape also keeps the final state
     const int64_t tape_fields = s->rec_full_tape ? nsteps + (ps ? 1 : 0) : 2 * s->rec_window;
+    if(adj_split(s))
+        for(void *&b : s->adj_split)
+            if(!b && hipMalloc(&b, fb) != hipSuccess) {
+                b = nullptr;
+                return fail(c, CH_ENOMEM, "no device memory for the reverse-sweep work fields");
+            }
     if(!grow(&s->tape, s->tape_cap, tape_fields, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)) ||
        (ps && !s->rec_full_tape && !grow(&s->ckpt_hat, s->ckpt_hat_cap, nck, hb)))
         return fail(c, CH_ENOMEM, "no device memory for %lld checkpoints / a %lld-state tape", (long long)nck,
@@ -1696,10 +1727,8 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     if(s->rec_full_tape) {
         // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
         HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, tape_steps(s, s->tape, 0, nsteps - 1, c->stream));
-        HIPCHK(c, chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility,
-                                   s->d.integrator == CH_EULER_MOBILITY, 1, field_at(s->tape, nsteps - 1, fb),
-                                   s->psi[s->cur], c->stream));
+        HIPCHK(c, tape_steps(s, s->tape, 0, nsteps - 1, c->stream, false));
+        HIPCHK(c, tape_step1(s, field_at(s->tape, nsteps - 1, fb), s->psi[s->cur], c->stream, false));
     }
     else {
         for(int64_t k = 0; k < nck; k++) {
@@ -1754,7 +1783,7 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
                 HIPCHK(c, chd::ps_steps_tape(a, s->aux, len - 1, tape));
             }
             else {
-                HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux));
+                HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux, true));
             }
             HIPCHK(c, hipEventRecord(s->ev_rec[b], s->aux));
             HIPCHK(c, hipStreamWaitEvent(ms, s->ev_rec[b], 0));
@@ -1767,7 +1796,15 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
                 li ^= 1;
             }
         }
-        for(int64_t t = ps ? -1 : len - 1; t >= 0;) {
+        if(adj_split(s)) {
+            void *const w[4] = {s->stage2, s->split_mob, s->adj_split[0], s->adj_split[1]};
+            for(int64_t t = len - 1; t >= 0; t--) {
+                HIPCHK(c, chd::adjoint_split3d(s->p, s->d.precision, s->d.model, s->d.mobility, s->lam[li],
+                                               s->lam[li ^ 1], field_at(tape, t, fb), w, s->gacc, ms));
+                li ^= 1;
+            }
+        }
+        for(int64_t t = (ps || adj_split(s)) ? -1 : len - 1; t >= 0;) {
             const int ka = chd::adjoint_depth(s->d.dims, s->d.N, em, t + 1);
             HIPCHK(c, chd::adjoint_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, ka, s->lam[li],
                                           s->lam[li ^ 1], tape, (int)t, s->gacc, ms));

@@ -13,11 +13,21 @@ template <int DIMS>
 hipError_t stencil_step(const Params &p, int precision, int integrator, int model, int mob, int K, const void *in,
                         void *out, hipStream_t s);
 // the 3D euler / euler_mobility step as two launches (mu of every cell into
-// `work`, N^3 elements, then the update): Params::tile3d == 3; not for
+// `work`, N^3 elements, and a variable mobility into `mob_work`, N^3 elements,
+// when split3d_needs_mobility; then the update): Params::tile3d == 3; not for
 // free-energy mobility (split3d_ok)
 bool split3d_ok(const Params &p, int integrator, int mob);
+bool split3d_needs_mobility(int integrator, int mob);
 hipError_t stencil_split3d(const Params &p, int precision, int integrator, int model, int mob, const void *in,
-                           void *out, void *work, hipStream_t s);
+                           void *out, void *work, void *mob_work, hipStream_t s);
+// the 3D euler_mobility reverse step as three launches (Params::tile3d == 3,
+// mobility not free_energy): lambda_in -> lambda_out with psi = psi_n; work =
+// four N^3 fields (mu, mobility, nu, eta); gradient partials into the first
+// adjoint_split3d_blocks(p) slot groups of gacc
+bool adjoint_split3d_ok(const Params &p, int mob);
+int adjoint_split3d_blocks(const Params &p);
+hipError_t adjoint_split3d(const Params &p, int precision, int model, int mob, const void *lam_in, void *lam_out,
+                           const void *psi, void *const work[4], double *gacc, hipStream_t s);
 inline int stencil_max_k(int dims, int N) { return (dims == 2 && N >= 64) ? 8 : 1; }
 // one euler_mobility step with conserved noise (noise.hip), in -> out; the
 // step index is *step_base + step_off.  noise_step_advance adds n to *step_base.

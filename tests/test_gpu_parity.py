@@ -443,6 +443,37 @@ def test_temporal_blocking_variants_bitwise(ch, ctx, monkeypatch, integrator, mo
     assert rel_linf(base, ref) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["wertheim_em_linear", "wertheim_euler", "landau_em_regularised", "landau_em_constant"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_split3d_equals_fused_tile(ch, ctx, monkeypatch, case, precision):
+    """The 3D two-launch step (mu and the cells' mobilities of every cell, then
+    the update: the default for SimpleWertheim) against the fused LDS tile
+    kernel, bit for bit, at N = 32 and N = 128 over 24 steps; both evaluate each
+    cell with the same arithmetic."""
+    from cahn_hilliard_amd.init import initial_density
+    kw = {"wertheim_em_linear": dict(integrator="euler_mobility", free_energy="simple_wertheim", B2=1.0, delta=1e4,
+                                     valence=4, mobility="linear", M=1.0, rho_ref=0.05, dt=1e-4, k=1.0),
+          "wertheim_euler": dict(integrator="euler", free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4,
+                                 dt=1e-4, k=1.0),
+          "landau_em_regularised": dict(integrator="euler_mobility", mobility="regularised", rho_min=0.5, dt=1e-3,
+                                        k=1.0, epsilon=0.9),
+          "landau_em_constant": dict(integrator="euler_mobility", mobility="constant", M=1.0, dt=1e-3, k=1.0,
+                                     epsilon=0.9)}[case]
+    for N in (32, 128):
+        rho0 = initial_density(3, N, 0.05 if "wertheim" in case else 0.3, 0.05, 0, 4242 + N)
+        outs = {}
+        for t3 in ("1", "3"):
+            monkeypatch.setenv("CH_TILE3D", t3)
+            s = ch.System(ctx, ch.make_desc(dims=3, N=N, precision=precision, **kw))
+            s.upload(rho0)
+            s.evolve(24)
+            outs[t3] = s.download()
+            s.close()
+        print(f"{case} {precision} N={N}: max |rho - rho0| {np.abs(outs['3'] - rho0.ravel()).max():.3e}")
+        assert np.all(np.isfinite(outs["3"])) and not np.array_equal(outs["3"], rho0.ravel())
+        assert np.array_equal(outs["1"], outs["3"]), (case, precision, N)
+
+
 @pytest.mark.parametrize("name", ["large_landau_2d_N1024", "large_landau_2d_N128", "large_wertheim_3d_N128"])
 def test_full_size_against_reference_build(ch, ctx, name):
     """Full-size workloads against the reference's own integrators (oracle/_ref,
