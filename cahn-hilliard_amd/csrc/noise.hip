@@ -10,7 +10,8 @@
 // eta_d(i) is the d-th standard normal of cell i at step t: Philox4x32-10
 // (Salmon, Moraes, Dror, Shaw, SC 2011) of the counter (i, 0, t lo, t hi)
 // under the key (seed lo, seed hi), the four 32-bit words mapped to uniforms
-// u = (x + 1/2) 2^-32 and Box-Muller on (u0, u1), (u2, u3).  A face's value
+// u = (x + 1/2) 2^-32 and Box-Muller on (u0, u1), (u2, u3) (in FP64 with the
+// series logarithm of ch_device.h and sincos_2pi below).  A face's value
 // is a pure function of (seed, cell, step), so any tile computes it without
 // coordination and a run does not depend on the launch shape.  The
 // reference draws from one mt19937 stream (CPU) or per-cell curand states
@@ -43,6 +44,44 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
     return c;
 }
 
+// sin(2 pi u) and cos(2 pi u) for u in [0, 1): with 8 u = k + 1/2 + f (k the
+// octant, |f| <= 1/2, both exact), 2 pi u = (2k + 1) pi / 8 + r, r = f pi / 4
+// (pi / 4 as head + tail), |r| <= pi / 8; sin r and cos r by their Maclaurin
+// series through r^13 and r^14 (first omitted terms below 1e-18), then the
+// angle sum with sin and cos of (2k + 1) pi / 8, which are +-sin(pi/8) or
+// +-cos(pi/8).  Within 2e-16 of sin / cos of the exact 2 pi u, with no
+// general argument reduction.
+__device__ __forceinline__ void sincos_2pi(double u, double &sn, double &cs) {
+#pragma clang fp contract(off)
+    const double v = 8.0 * u;
+    const int k = (int)v;
+    const double f = (v - (double)k) - 0.5;
+    const double r = __builtin_fma(f, 0.7853981633974483, f * 3.061616997868383e-17);
+    const double t = r * r;
+    double s = -1.0 / 6227020800.0;  // -1/13!
+    s = __builtin_fma(s, t, 1.0 / 39916800.0);
+    s = __builtin_fma(s, t, -1.0 / 362880.0);
+    s = __builtin_fma(s, t, 1.0 / 5040.0);
+    s = __builtin_fma(s, t, -1.0 / 120.0);
+    s = __builtin_fma(s, t, 1.0 / 6.0);
+    const double sr = __builtin_fma(-r * t, s, r);  // r - r^3 (1/3! - r^2/5! + ...)
+    double c = 1.0 / 87178291200.0;                 // 1/14!
+    c = __builtin_fma(c, t, -1.0 / 479001600.0);
+    c = __builtin_fma(c, t, 1.0 / 3628800.0);
+    c = __builtin_fma(c, t, -1.0 / 40320.0);
+    c = __builtin_fma(c, t, 1.0 / 720.0);
+    c = __builtin_fma(c, t, -1.0 / 24.0);
+    c = __builtin_fma(c, t, 0.5);
+    const double cr = __builtin_fma(-t, c, 1.0);  // 1 - r^2 (1/2! - r^2/4! + ...)
+    constexpr double S1 = 0.3826834323650898, C1 = 0.9238795325112867;  // sin, cos of pi/8
+    const bool swap = k == 1 || k == 2 || k == 5 || k == 6;
+    double sk = swap ? C1 : S1, ck = swap ? S1 : C1;
+    if(k >= 4) sk = -sk;
+    if(k >= 2 && k <= 5) ck = -ck;
+    sn = __builtin_fma(sk, cr, ck * sr);
+    cs = __builtin_fma(ck, cr, -(sk * sr));
+}
+
 // standard normals DIMS of cell `cell` at step `step`, in T
 template <typename T, int DIMS>
 __device__ __forceinline__ void cell_normals(uint64_t seed, uint32_t cell, uint64_t step, T (&eta)[DIMS]) {
@@ -55,9 +94,9 @@ __device__ __forceinline__ void cell_normals(uint64_t seed, uint32_t cell, uint6
         if constexpr(sizeof(T) == 8) {
             const double u1 = ((double)w[2 * h] + 0.5) * 0x1p-32;
             const double u2 = ((double)w[2 * h + 1] + 0.5) * 0x1p-32;
-            const double r = sqrt(-2.0 * log(u1)), th = (2.0 * 3.14159265358979323846) * u2;
+            const double r = sqrt(-2.0 * log_series(u1));
             double sn, cs;
-            sincos(th, &sn, &cs);
+            sincos_2pi(u2, sn, cs);
             eta[2 * h] = r * cs;
             if(2 * h + 1 < DIMS) eta[2 * h + 1] = r * sn;
         }
