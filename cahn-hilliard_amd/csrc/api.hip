@@ -41,9 +41,8 @@ struct ch_sys {
     void *stage = nullptr;   // bins doubles: host transfers of FP32 fields, mobility
     void *stage2 = nullptr;  // bins elements of T: mobility
     void *split_mob = nullptr;  // bins elements of T: the cells' mobilities of the two-launch 3D step
-    // the three-launch 3D euler_mobility reverse step: nu and eta, and the mu /
-    // mobility fields of the window recompute on the auxiliary stream
-    void *adj_split[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the three-launch 3D euler_mobility reverse step: nu and eta
+    void *adj_split[2] = {nullptr, nullptr};
     double *partials = nullptr, *result = nullptr;
     // reverse mode: checkpoints, the window tape, lambda ping-pong, gradient slots
     void *ckpt = nullptr, *tape = nullptr, *lam[2] = {nullptr, nullptr};
@@ -346,7 +345,7 @@ static void free_sys(ch_sys *s) {
     void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->split_mob, s->partials, s->result,
                     s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc, s->ckpt_hat, s->ps_rh, s->ps_w,
                     s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1],
-                    s->adj_split[0], s->adj_split[1], s->adj_split[2], s->adj_split[3]};
+                    s->adj_split[0], s->adj_split[1]};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
     for(void *b : s->psm_spec)
@@ -1650,24 +1649,31 @@ static bool adj_split(const ch_sys *s) {
     return s->d.dims == 3 && s->d.integrator == CH_EULER_MOBILITY && chd::adjoint_split3d_ok(s->p, s->d.mobility);
 }
 
-// one recorded state: `aux` picks the work fields of the auxiliary stream
-static hipError_t tape_step1(ch_sys *s, const void *in, void *out, hipStream_t st, bool aux) {
+// Tape slots: psi_t, and with the three-launch reverse step also mu_t and the
+// mobilities of psi_t behind it (the recompute's two-launch step computes
+// them on its way to psi_{t+1}, so the reverse step reads them back).
+static size_t tape_slot(const ch_sys *s) { return s->bins * s->elt * (adj_split(s) ? 3 : 1); }
+
+// one recorded state: psi_{t+1} = step(psi_t) from the slot at `in` into `out`
+static hipError_t tape_step1(ch_sys *s, const void *in, void *out, hipStream_t st) {
     const ch_desc &d = s->d;
-    if(adj_split(s))
-        return chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out,
-                                    aux ? s->adj_split[2] : s->stage2, aux ? s->adj_split[3] : s->split_mob, st);
+    if(adj_split(s)) {
+        const size_t fb = s->bins * s->elt;
+        return chd::stencil_split3d(s->p, d.precision, d.integrator, d.model, d.mobility, in, out, (char *)in + fb,
+                                    (char *)in + 2 * fb, st);
+    }
     return chd::tape_launch(s->p, d.precision, d.model, d.mobility, d.integrator == CH_EULER_MOBILITY, 1, in, out, st);
 }
 
 // `count` states after tape slot `from` of `tape` into slots from+1 .. from+count
-static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count, hipStream_t st, bool aux) {
-    const size_t fb = s->bins * s->elt;
+static hipError_t tape_steps(ch_sys *s, void *tape, int64_t from, int64_t count, hipStream_t st) {
+    const size_t sb = tape_slot(s);
     const bool em = s->d.integrator == CH_EULER_MOBILITY;
     while(count > 0) {
         const int K = chd::tape_depth(s->d.dims, s->d.N, em, count);
-        hipError_t e = K == 1 ? tape_step1(s, field_at(tape, from, fb), field_at(tape, from + 1, fb), st, aux)
+        hipError_t e = K == 1 ? tape_step1(s, field_at(tape, from, sb), field_at(tape, from + 1, sb), st)
                               : chd::tape_launch(s->p, s->d.precision, s->d.model, s->d.mobility, em, K,
-                                                 field_at(tape, from, fb), field_at(tape, from + 1, fb), st);
+                                                 field_at(tape, from, sb), field_at(tape, from + 1, sb), st);
         if(e != hipSuccess) return e;
         from += K;
         count -= K;
@@ -1699,7 +1705,8 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
                 b = nullptr;
                 return fail(c, CH_ENOMEM, "no device memory for the reverse-sweep work fields");
             }
-    if(!grow(&s->tape, s->tape_cap, tape_fields, fb) || (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)) ||
+    if(!grow(&s->tape, s->tape_cap, tape_fields, tape_slot(s)) ||
+       (!s->rec_full_tape && !grow(&s->ckpt, s->ckpt_cap, nck, fb)) ||
        (ps && !s->rec_full_tape && !grow(&s->ckpt_hat, s->ckpt_hat_cap, nck, hb)))
         return fail(c, CH_ENOMEM, "no device memory for %lld checkpoints / a %lld-state tape", (long long)nck,
                     (long long)tape_fields);
@@ -1727,8 +1734,8 @@ extern "C" ch_status ch_sys_record(ch_sys *s, int64_t nsteps, int32_t ckpt_every
     if(s->rec_full_tape) {
         // every state: psi_0 .. psi_{T-1} on the tape, psi_T in the field buffer
         HIPCHK(c, hipMemcpyAsync(s->tape, s->psi[s->cur], fb, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, tape_steps(s, s->tape, 0, nsteps - 1, c->stream, false));
-        HIPCHK(c, tape_step1(s, field_at(s->tape, nsteps - 1, fb), s->psi[s->cur], c->stream, false));
+        HIPCHK(c, tape_steps(s, s->tape, 0, nsteps - 1, c->stream));
+        HIPCHK(c, tape_step1(s, field_at(s->tape, nsteps - 1, tape_slot(s)), s->psi[s->cur], c->stream));
     }
     else {
         for(int64_t k = 0; k < nck; k++) {
@@ -1769,7 +1776,7 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
     for(int64_t k = nck - 1; k >= 0; k--) {
         const int64_t start = k * W, len = std::min<int64_t>(W, T - start);
         const int b = (int)(k & 1);
-        void *tape = recompute ? field_at(s->tape, b * W, fb) : s->tape;
+        void *tape = recompute ? field_at(s->tape, b * W, tape_slot(s)) : s->tape;
         if(recompute) {
             if(used[b]) HIPCHK(c, hipStreamWaitEvent(s->aux, s->ev_adj[b], 0));
             HIPCHK(c, hipMemcpyAsync(tape, field_at(s->ckpt, k, fb), fb, hipMemcpyDeviceToDevice, s->aux));
@@ -1783,7 +1790,13 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
                 HIPCHK(c, chd::ps_steps_tape(a, s->aux, len - 1, tape));
             }
             else {
-                HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux, true));
+                HIPCHK(c, tape_steps(s, tape, 0, len - 1, s->aux));
+                if(adj_split(s)) {
+                    // mu and the mobilities of the window's last state
+                    char *last = field_at(tape, len - 1, tape_slot(s));
+                    HIPCHK(c, chd::split3d_mu(s->p, s->d.precision, s->d.integrator, s->d.model, s->d.mobility, last,
+                                              last + fb, last + 2 * fb, s->aux));
+                }
             }
             HIPCHK(c, hipEventRecord(s->ev_rec[b], s->aux));
             HIPCHK(c, hipStreamWaitEvent(ms, s->ev_rec[b], 0));
@@ -1797,10 +1810,11 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
             }
         }
         if(adj_split(s)) {
-            void *const w[4] = {s->stage2, s->split_mob, s->adj_split[0], s->adj_split[1]};
             for(int64_t t = len - 1; t >= 0; t--) {
+                char *slot = field_at(tape, t, tape_slot(s));
+                void *const w[4] = {slot + fb, slot + 2 * fb, s->adj_split[0], s->adj_split[1]};
                 HIPCHK(c, chd::adjoint_split3d(s->p, s->d.precision, s->d.model, s->d.mobility, s->lam[li],
-                                               s->lam[li ^ 1], field_at(tape, t, fb), w, s->gacc, ms));
+                                               s->lam[li ^ 1], slot, w, true, s->gacc, ms));
                 li ^= 1;
             }
         }

@@ -20,14 +20,19 @@ bool split3d_ok(const Params &p, int integrator, int mob);
 bool split3d_needs_mobility(int integrator, int mob);
 hipError_t stencil_split3d(const Params &p, int precision, int integrator, int model, int mob, const void *in,
                            void *out, void *work, void *mob_work, hipStream_t s);
+// the first launch alone: mu (and the mobilities when split3d_needs_mobility)
+// of `in`
+hipError_t split3d_mu(const Params &p, int precision, int integrator, int model, int mob, const void *in, void *mu,
+                      void *mob_out, hipStream_t s);
 // the 3D euler_mobility reverse step as three launches (Params::tile3d == 3,
 // mobility not free_energy): lambda_in -> lambda_out with psi = psi_n; work =
-// four N^3 fields (mu, mobility, nu, eta); gradient partials into the first
-// adjoint_split3d_blocks(p) slot groups of gacc
+// four N^3 fields (mu, mobility, nu, eta), the first two already holding mu_n
+// and the mobilities of psi_n when have_mu (then two launches); gradient
+// partials into the first adjoint_split3d_blocks(p) slot groups of gacc
 bool adjoint_split3d_ok(const Params &p, int mob);
 int adjoint_split3d_blocks(const Params &p);
 hipError_t adjoint_split3d(const Params &p, int precision, int model, int mob, const void *lam_in, void *lam_out,
-                           const void *psi, void *const work[4], double *gacc, hipStream_t s);
+                           const void *psi, void *const work[4], bool have_mu, double *gacc, hipStream_t s);
 inline int stencil_max_k(int dims, int N) { return (dims == 2 && N >= 64) ? 8 : 1; }
 // one euler_mobility step with conserved noise (noise.hip), in -> out; the
 // step index is *step_base + step_off.  noise_step_advance adds n to *step_base.

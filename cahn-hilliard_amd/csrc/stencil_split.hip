@@ -263,13 +263,13 @@ constexpr int kAdjZNu = 8, kAdjZUpd = 2;
 
 template <typename T, int MODEL, int MOB>
 static hipError_t adj_split_t(const Params &p, const void *lam_in, void *lam_out, const void *psi, void *const w[4],
-                              double *gacc, hipStream_t s) {
+                              bool have_mu, double *gacc, hipStream_t s) {
     constexpr int ZM = kSplitZMu, ZN = kAdjZNu, ZU = kAdjZUpd;
     const int64_t bins = (int64_t)1 << (3 * p.log2N);
     const int nbm = (int)(bins / (256 * ZM)), nbn = (int)(bins / (256 * ZN)), nbu = (int)(bins / (256 * ZU));
     const T *ps = (const T *)psi;
     T *mu = (T *)w[0], *mob = (T *)w[1], *nu = (T *)w[2], *eta = (T *)w[3];
-    hipLaunchKernelGGL((k_split_mu<T, MODEL, MOB, ZM>), dim3(nbm), dim3(256), 0, s, ps, mu, mob, p);
+    if(!have_mu) hipLaunchKernelGGL((k_split_mu<T, MODEL, MOB, ZM>), dim3(nbm), dim3(256), 0, s, ps, mu, mob, p);
     hipLaunchKernelGGL((k_adj_split_nu<T, MOB, ZN>), dim3(nbn), dim3(256), 0, s, (const T *)lam_in, mu, mob, nu, eta, p);
     hipLaunchKernelGGL((k_adj_split_upd<T, MODEL, MOB, ZU>), dim3(nbu), dim3(256), 0, s, (const T *)lam_in, nu, eta,
                        ps, (T *)lam_out, p, gacc);
@@ -278,13 +278,45 @@ static hipError_t adj_split_t(const Params &p, const void *lam_in, void *lam_out
 
 template <typename T, int MODEL>
 static hipError_t adj_split_mob(const Params &p, int mob, const void *lam_in, void *lam_out, const void *psi,
-                                void *const w[4], double *gacc, hipStream_t s) {
+                                void *const w[4], bool have_mu, double *gacc, hipStream_t s) {
     switch(mob) {
-    case CH_MOB_CONSTANT: return adj_split_t<T, MODEL, CH_MOB_CONSTANT>(p, lam_in, lam_out, psi, w, gacc, s);
-    case CH_MOB_REGULARISED: return adj_split_t<T, MODEL, CH_MOB_REGULARISED>(p, lam_in, lam_out, psi, w, gacc, s);
-    case CH_MOB_LINEAR: return adj_split_t<T, MODEL, CH_MOB_LINEAR>(p, lam_in, lam_out, psi, w, gacc, s);
+    case CH_MOB_CONSTANT:
+        return adj_split_t<T, MODEL, CH_MOB_CONSTANT>(p, lam_in, lam_out, psi, w, have_mu, gacc, s);
+    case CH_MOB_REGULARISED:
+        return adj_split_t<T, MODEL, CH_MOB_REGULARISED>(p, lam_in, lam_out, psi, w, have_mu, gacc, s);
+    case CH_MOB_LINEAR: return adj_split_t<T, MODEL, CH_MOB_LINEAR>(p, lam_in, lam_out, psi, w, have_mu, gacc, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <typename T, int MODEL>
+static hipError_t mu_only_t(const Params &p, int integrator, int mob, const void *in, void *mu, void *mob_out,
+                            hipStream_t s) {
+    constexpr int ZM = kSplitZMu;
+    const int nbm = (int)(((int64_t)1 << (3 * p.log2N)) / (256 * ZM));
+    const T *a = (const T *)in;
+    T *w = (T *)mu, *mw = (T *)mob_out;
+    const bool varm = split3d_needs_mobility(integrator, mob);
+    if(varm && mob == CH_MOB_REGULARISED)
+        hipLaunchKernelGGL((k_split_mu<T, MODEL, CH_MOB_REGULARISED, ZM>), dim3(nbm), dim3(256), 0, s, a, w, mw, p);
+    else if(varm && mob == CH_MOB_LINEAR)
+        hipLaunchKernelGGL((k_split_mu<T, MODEL, CH_MOB_LINEAR, ZM>), dim3(nbm), dim3(256), 0, s, a, w, mw, p);
+    else if(!varm)
+        hipLaunchKernelGGL((k_split_mu<T, MODEL, CH_MOB_CONSTANT, ZM>), dim3(nbm), dim3(256), 0, s, a, w, nullptr, p);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t split3d_mu(const Params &p, int precision, int integrator, int model, int mob, const void *in, void *mu,
+                      void *mob_out, hipStream_t s) {
+    if(!split3d_ok(p, integrator, mob)) return hipErrorInvalidValue;
+    if(precision == CH_FP64) {
+        if(model == CH_LANDAU) return mu_only_t<double, CH_LANDAU>(p, integrator, mob, in, mu, mob_out, s);
+        return mu_only_t<double, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, mu, mob_out, s);
+    }
+    if(model == CH_LANDAU) return mu_only_t<float, CH_LANDAU>(p, integrator, mob, in, mu, mob_out, s);
+    return mu_only_t<float, CH_SIMPLE_WERTHEIM>(p, integrator, mob, in, mu, mob_out, s);
 }
 
 bool adjoint_split3d_ok(const Params &p, int mob) {
@@ -294,14 +326,15 @@ bool adjoint_split3d_ok(const Params &p, int mob) {
 int adjoint_split3d_blocks(const Params &p) { return (int)(((int64_t)1 << (3 * p.log2N)) / (256 * kAdjZUpd)); }
 
 hipError_t adjoint_split3d(const Params &p, int precision, int model, int mob, const void *lam_in, void *lam_out,
-                           const void *psi, void *const work[4], double *gacc, hipStream_t s) {
+                           const void *psi, void *const work[4], bool have_mu, double *gacc, hipStream_t s) {
     if(!adjoint_split3d_ok(p, mob)) return hipErrorInvalidValue;
     if(precision == CH_FP64) {
-        if(model == CH_LANDAU) return adj_split_mob<double, CH_LANDAU>(p, mob, lam_in, lam_out, psi, work, gacc, s);
-        return adj_split_mob<double, CH_SIMPLE_WERTHEIM>(p, mob, lam_in, lam_out, psi, work, gacc, s);
+        if(model == CH_LANDAU)
+            return adj_split_mob<double, CH_LANDAU>(p, mob, lam_in, lam_out, psi, work, have_mu, gacc, s);
+        return adj_split_mob<double, CH_SIMPLE_WERTHEIM>(p, mob, lam_in, lam_out, psi, work, have_mu, gacc, s);
     }
-    if(model == CH_LANDAU) return adj_split_mob<float, CH_LANDAU>(p, mob, lam_in, lam_out, psi, work, gacc, s);
-    return adj_split_mob<float, CH_SIMPLE_WERTHEIM>(p, mob, lam_in, lam_out, psi, work, gacc, s);
+    if(model == CH_LANDAU) return adj_split_mob<float, CH_LANDAU>(p, mob, lam_in, lam_out, psi, work, have_mu, gacc, s);
+    return adj_split_mob<float, CH_SIMPLE_WERTHEIM>(p, mob, lam_in, lam_out, psi, work, have_mu, gacc, s);
 }
 
 }  // namespace chd

@@ -288,10 +288,12 @@ def test_euler_mobility_adjoint_against_oracle(ch, ctx, case):
 @pytest.mark.parametrize("case", ["wertheim_linear", "landau_regularised", "landau_constant"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_euler_mobility_3d_split_adjoint_equals_tile(ch, ctx, monkeypatch, case, precision):
-    """The 3D euler_mobility reverse step in three launches (mu and mobilities,
-    nu and eta, the update; the default for SimpleWertheim) and its two-launch
-    window recompute against the LDS tile adjoint: dL/dpsi0 bit for bit, the
-    parameter gradients (FP64 sums in another order) within 1e-12."""
+    """The 3D euler_mobility reverse step in launches of one cell per thread (nu
+    and eta, then the update, with mu and the mobilities of each state kept on
+    the tape; the default for SimpleWertheim), with a checkpointed record (the
+    two-launch window recompute) and with the whole tape, against the LDS tile
+    adjoint: dL/dpsi0 bit for bit, the parameter gradients (FP64 sums in
+    another order) within 1e-12."""
     N, steps, ckpt = 32, 24, 5
     mk, mob, init, dt = {
         "wertheim_linear": (dict(free_energy="simple_wertheim", B2=1.0, delta=1e4, valence=4),
@@ -301,21 +303,23 @@ def test_euler_mobility_3d_split_adjoint_equals_tile(ch, ctx, monkeypatch, case,
     psi0 = O.init_density(3, N, init[0], init[1], 0, 777)
     w = np.random.default_rng(5).normal(size=N ** 3)
     out = {}
-    for t3 in ("1", "3"):
+    for t3, ck in (("1", ckpt), ("3", ckpt), ("3", 0)):  # 0: the whole tape recorded, no recompute
         monkeypatch.setenv("CH_TILE3D", t3)
         s = ch.System(ctx, ch.make_desc(dims=3, N=N, integrator="euler_mobility", dt=dt, k=1.1, M=0.9,
                                         precision=precision, **mk, **mob))
         s.upload(psi0)
-        s.record(steps, ckpt)
+        s.record(steps, ck)
         psiT = s.download()
         lam, g = s.adjoint(w)
         s.close()
-        out[t3] = (psiT, lam, np.asarray(g))
-    print(f"{case} {precision}: grads {out['3'][2]} vs tile {out['1'][2]}")
-    assert np.all(np.isfinite(out["3"][1]))
-    assert np.array_equal(out["1"][0], out["3"][0])
-    assert np.array_equal(out["1"][1], out["3"][1])
-    np.testing.assert_allclose(out["3"][2], out["1"][2], rtol=1e-12, atol=1e-14 * np.abs(out["1"][2]).max())
+        out[t3, ck] = (psiT, lam, np.asarray(g))
+    ref = out["1", ckpt]
+    for key in (("3", ckpt), ("3", 0)):
+        print(f"{case} {precision} {key}: grads {out[key][2]} vs tile {ref[2]}")
+        assert np.all(np.isfinite(out[key][1]))
+        assert np.array_equal(ref[0], out[key][0]), key
+        assert np.array_equal(ref[1], out[key][1]), key
+        np.testing.assert_allclose(out[key][2], ref[2], rtol=1e-12, atol=1e-14 * np.abs(ref[2]).max())
 
 
 def test_euler_mobility_checkpoint_invariance(ch, ctx):
