@@ -881,7 +881,6 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     const Params &p = a.p;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1;
     constexpr int LX = LOG2N - 1;                     // x pass: complex length N / 2
-    using GX = FftGeom<LX, PackedLines<LX>::value>;
     const int dims = p.dims;
     const SpecLayout L = spec_layout(dims, N);
     const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
@@ -889,7 +888,6 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     const T2 *tw2 = (const T2 *)a.twiddles;
     T *rho = (T *)a.rho;
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
-    const int bx = (rows + GX::NL - 1) / GX::NL;
     const bool landau = a.model == CH_LANDAU;
     const size_t bins = (size_t)rows * N;
     if(init) {
@@ -973,11 +971,9 @@ static hipError_t spec_t(const SpectralArgs &a, const void *real_in, void *spec,
     using T2 = typename Complex<T>::type;
     const Params &p = a.p;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
-    using GX = FftGeom<LX, PackedLines<LX>::value>;
     const int dims = p.dims;
     const SpecLayout L = spec_layout(dims, N);
     const int rows = dims == 1 ? 1 : (dims == 2 ? N : N * N);
-    const int bx = (rows + GX::NL - 1) / GX::NL;
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)spec;
     const int lines = dims == 3 ? N * Nh : Nh;
@@ -1070,7 +1066,6 @@ template <typename T, int LOG2N>
 static hipError_t fft2_inverse_t(const SpectralArgs &a, void *out, hipStream_t s) {
     using T2 = typename Complex<T>::type;
     constexpr int N = 1 << LOG2N, Nh = N / 2 + 1, LX = LOG2N - 1;
-    using GX = FftGeom<LX, PackedLines<LX>::value>;
     const SpecLayout L = spec_layout(2, N);
     const T2 *tw2 = (const T2 *)a.twiddles;
     T2 *W = (T2 *)a.work;
