@@ -21,7 +21,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // adds cell i's density and free-energy density (bulk + k sum_d (grad_d+)^2,
 // forward differences as CahnHilliard.cpp:256-276, 300-319) to m and f
-template <typename T, int MODEL>
+template <typename T, int MODEL, int DIMS>
 __device__ __forceinline__ void obs_cell(const T *__restrict__ rho, const Params &p, int64_t i, double &m, double &f) {
     const int mask = p.mask, lg = p.log2N;
     const double r = (double)rho[i];
@@ -33,13 +33,13 @@ __device__ __forceinline__ void obs_cell(const T *__restrict__ rho, const Params
         const double g = ((double)rho[ip] - r) * p.inv_dx;
         inter += p.k * g * g;
     }
-    if(p.dims >= 2) {
+    if constexpr(DIMS >= 2) {
         const int y = (int)((i >> lg) & mask);
         const int64_t ip = i + ((int64_t)(((y + 1) & mask) - y) << lg);
         const double g = ((double)rho[ip] - r) * p.inv_dx;
         inter += p.k * g * g;
     }
-    if(p.dims == 3) {
+    if constexpr(DIMS == 3) {
         const int z = (int)(i >> (2 * lg));
         const int64_t ip = i + ((int64_t)(((z + 1) & mask) - z) << (2 * lg));
         const double g = ((double)rho[ip] - r) * p.inv_dx;
@@ -48,13 +48,26 @@ __device__ __forceinline__ void obs_cell(const T *__restrict__ rho, const Params
     f += bulk<MODEL>(p, r) + inter;
 }
 
-// the sums of n (mass, f) pairs, one block, in a fixed order
+// the sums of n (mass, f) pairs, one block, in a fixed order.  Each thread
+// adds pairs t, t + 256, ... in that order, four per round with all their
+// loads issued together (a missing pair adds +0.0, which changes no sum).
 __device__ __forceinline__ void sum_pairs_block(const double *__restrict__ partials, int n, double &m, double &f) {
     m = 0;
     f = 0;
-    for(int i = threadIdx.x; i < n; i += 256) {
-        m += partials[2 * i];
-        f += partials[2 * i + 1];
+    for(int i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {
+        double pm[4], pf[4];
+#pragma unroll
+        for(int u = 0; u < 4; u++) {
+            const int i = i0 + 256 * u;
+            const bool in = i < n;
+            pm[u] = in ? partials[2 * i] : 0.0;
+            pf[u] = in ? partials[2 * i + 1] : 0.0;
+        }
+#pragma unroll
+        for(int u = 0; u < 4; u++) {
+            m += pm[u];
+            f += pf[u];
+        }
     }
     __shared__ double sm[4], sf[4];
     m = wave_sum(m);
@@ -66,12 +79,23 @@ __device__ __forceinline__ void sum_pairs_block(const double *__restrict__ parti
     f = (sf[0] + sf[1]) + (sf[2] + sf[3]);
 }
 
-template <typename T, int MODEL>
+// a grid-stride loop over the cells, four strides per round so that the
+// loads of all four cells issue before the first is used; the cells are
+// added in the plain grid-stride order
+template <typename T, int MODEL, int DIMS>
 __global__ __launch_bounds__(256) void k_observables(const T *__restrict__ rho, Params p, double *__restrict__ partials) {
     const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
+    const int64_t stride = (int64_t)gridDim.x * 256;
     double m = 0, f = 0;
-    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < bins; i += (int64_t)gridDim.x * 256)
-        obs_cell<T, MODEL>(rho, p, i, m, f);
+    for(int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x; i0 < bins; i0 += 4 * stride) {
+        if(i0 + 3 * stride < bins) {
+#pragma unroll
+            for(int u = 0; u < 4; u++) obs_cell<T, MODEL, DIMS>(rho, p, i0 + u * stride, m, f);
+        }
+        else {
+            for(int64_t i = i0; i < bins; i += stride) obs_cell<T, MODEL, DIMS>(rho, p, i, m, f);
+        }
+    }
     __shared__ double sm[4], sf[4];
     m = wave_sum(m);
     f = wave_sum(f);
@@ -111,15 +135,23 @@ __global__ __launch_bounds__(256) void k_finalize_ring(const double *__restrict_
     }
 }
 
+template <typename T, int MODEL>
+static void obs_launch(const Params &p, const void *rho, double *partials, int nb, hipStream_t s) {
+    const T *r = (const T *)rho;
+    if(p.dims == 1) hipLaunchKernelGGL((k_observables<T, MODEL, 1>), dim3(nb), dim3(256), 0, s, r, p, partials);
+    else if(p.dims == 2) hipLaunchKernelGGL((k_observables<T, MODEL, 2>), dim3(nb), dim3(256), 0, s, r, p, partials);
+    else hipLaunchKernelGGL((k_observables<T, MODEL, 3>), dim3(nb), dim3(256), 0, s, r, p, partials);
+}
+
 static hipError_t obs_partials(const Params &p, int precision, int model, const void *rho, double *partials, int nb,
                                hipStream_t s) {
     if(precision == CH_FP64) {
-        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<double, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
-        else hipLaunchKernelGGL((k_observables<double, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const double *)rho, p, partials);
+        if(model == CH_LANDAU) obs_launch<double, CH_LANDAU>(p, rho, partials, nb, s);
+        else obs_launch<double, CH_SIMPLE_WERTHEIM>(p, rho, partials, nb, s);
     }
     else {
-        if(model == CH_LANDAU) hipLaunchKernelGGL((k_observables<float, CH_LANDAU>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
-        else hipLaunchKernelGGL((k_observables<float, CH_SIMPLE_WERTHEIM>), dim3(nb), dim3(256), 0, s, (const float *)rho, p, partials);
+        if(model == CH_LANDAU) obs_launch<float, CH_LANDAU>(p, rho, partials, nb, s);
+        else obs_launch<float, CH_SIMPLE_WERTHEIM>(p, rho, partials, nb, s);
     }
     return hipGetLastError();
 }
@@ -192,7 +224,7 @@ __global__ __launch_bounds__(256) void k_batch_obs(const T *__restrict__ buf, co
     const T *rho = buf + reps[blockIdx.x].off;
     const int64_t bins = (int64_t)1 << (p.log2N * p.dims);
     double m = 0, f = 0;
-    for(int64_t i = threadIdx.x; i < bins; i += 256) obs_cell<T, MODEL>(rho, p, i, m, f);
+    for(int64_t i = threadIdx.x; i < bins; i += 256) obs_cell<T, MODEL, 2>(rho, p, i, m, f);  // batches are 2D
     __shared__ double sm[4], sf[4];
     m = wave_sum(m);
     f = wave_sum(f);
