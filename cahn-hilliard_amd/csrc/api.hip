@@ -603,6 +603,8 @@ static chd::SpectralArgs spectral_args(ch_sys *s) {
     a.work = s->work;
     a.rho_hat = s->rho_hat;
     a.twiddles = s->twiddles;
+    a.rh_lines = s->d.integrator == CH_PSEUDOSPECTRAL && !is_species(s) && s->d.dims == 2 &&
+                 (s->d.N == 1024 || s->d.N == 2048);
     return a;
 }
 

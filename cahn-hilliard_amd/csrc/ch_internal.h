@@ -49,6 +49,10 @@ struct SpectralArgs {
     void *work;           // half spectrum scratch (T2)
     void *rho_hat;        // half spectrum state (T2)
     const void *twiddles; // W_N^m, m = 0..N-1 (T2)
+    // rho_hat stored line-major (entry j of column line l at l N + j) where the
+    // 2D column pass runs the two-wave kernel (N = 1024, 2048): set for the
+    // pseudospectral integrator, whose chain alone reads rho_hat
+    bool rh_lines = false;
 };
 // init = true: rho_hat = r2c(rho); otherwise one time step
 hipError_t ps_run(const SpectralArgs &a, hipStream_t s, bool init);

@@ -480,12 +480,16 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
 // 3 the gradient factor (i k_gaxis) deal on load, then inverse (src -> dst);
 // 4 forward, then rho_hat (the divergence) =|+= (i k_gaxis) deal (transform),
 // no dst.  Modes 3 and 4 use the per-entry operations of psm_ops.h.
+// rhl (modes 0 and 2): the spectral state is stored line-major, entry j of line
+// l at l N + j, so that its loads and stores are whole 64-entry rows of the
+// line; mode 2 reads and writes rho_hat so, mode 0 writes its output so (the
+// initial transform into rho_hat).
 template <typename T, int MODE, int LOG2N>
 __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *__restrict__ src,
                                               typename Complex<T>::type *__restrict__ dst,
                                               typename Complex<T>::type *__restrict__ rho_hat,
                                               const typename Complex<T>::type *tw, Params p, int lines, long P,
-                                              long ls, long stride, int axis, int gaxis = 0) {
+                                              long ls, long stride, int axis, int gaxis = 0, int rhl = 0) {
     constexpr bool INV_IN = MODE == 1 || MODE == 3;  // input entries k + h n2 (else 2 i + h)
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N - 1>;
@@ -526,9 +530,11 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             v[m] = psm_grad_term<T>(p, ix, iy, iz, gaxis, v[m]);
         }
     }
+    // entry j of the line in the spectral state
+    auto rh_at = [&](int j) -> size_t { return rhl ? (size_t)l * p.N + j : base + (size_t)j * stride; };
     if constexpr(MODE == 2) {
 #pragma unroll
-        for(int m = 0; m < E; m++) rh[m] = rho_hat[base + (size_t)(lane + 64 * m + h * n2) * stride];
+        for(int m = 0; m < E; m++) rh[m] = rho_hat[rh_at(lane + 64 * m + h * n2)];
     }
     auto forward = [&]() {
         wave_transform<T2, LOG2N - 1, false>(lane, v, own, tw, ts);
@@ -574,7 +580,7 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
             else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
             v[m] = spectral_update<T, T2>(p, ix, iy, iz, v[m], rh[m], inv_bins);
-            rho_hat[base + (size_t)j * stride] = rh[m];
+            rho_hat[rh_at(j)] = rh[m];
         }
         inverse();
     }
@@ -593,7 +599,8 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
 #pragma unroll
     for(int m = 0; m < E; m++) {
         const int i = lane + 64 * m;
-        dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride] = v[m];
+        if(MODE == 0 && rhl) dst[rh_at(i + h * n2)] = v[m];
+        else dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride] = v[m];
     }
 }
 
@@ -829,16 +836,18 @@ static SpecLayout spec_layout(int dims, int N) {
     return SpecLayout{Nh, 1, (long)N * Nh, 1, Nh, Nh, 1, (long)N * Nh, false};
 }
 
+// rhl: the line-major spectral state of the two-wave kernel (N = 1024, 2048)
 template <typename T, int MODE, int LOG2N>
 static void launch_col(const typename Complex<T>::type *src, typename Complex<T>::type *dst,
                        typename Complex<T>::type *rh, const typename Complex<T>::type *tw, const Params &p, int lines,
-                       long P, long ls, long stride, int axis, bool contig, hipStream_t s) {
+                       long P, long ls, long stride, int axis, bool contig, hipStream_t s, bool rhl = false) {
     if constexpr(LOG2N == 10 || LOG2N == 11) {
         (void)contig;
         hipLaunchKernelGGL((k_colw2<T, MODE, LOG2N>), dim3((lines + 7) / 8 * 8), dim3(128), 0, s, src, dst, rh, tw, p,
-                           lines, P, ls, stride, axis);
+                           lines, P, ls, stride, axis, 0, rhl ? 1 : 0);
         return;
     }
+    (void)rhl;
     if constexpr(wave_len<LOG2N>()) {
         (void)contig;
         hipLaunchKernelGGL((k_colw<T, MODE, LOG2N>), dim3((lines + 7) / 8 * 8), dim3(64), 0, s, src, dst, rh, tw, p, lines,
@@ -860,8 +869,8 @@ static void launch_col(const typename Complex<T>::type *src, typename Complex<T>
 template <typename T, int MODE, int LOG2N>
 static void pass_y(const SpecLayout &L, typename Complex<T>::type *W, typename Complex<T>::type *RH,
                    const typename Complex<T>::type *tw, const Params &p, int lines, hipStream_t s,
-                   typename Complex<T>::type *dst = nullptr) {
-    launch_col<T, MODE, LOG2N>(W, dst ? dst : W, RH, tw, p, lines, L.yP, L.yls, L.ystride, 1, L.ycontig, s);
+                   typename Complex<T>::type *dst = nullptr, bool rhl = false) {
+    launch_col<T, MODE, LOG2N>(W, dst ? dst : W, RH, tw, p, lines, L.yP, L.yls, L.ystride, 1, L.ycontig, s, rhl);
 }
 template <typename T, int MODE, int LOG2N>
 static void pass_z(const SpecLayout &L, typename Complex<T>::type *W, typename Complex<T>::type *RH,
@@ -890,9 +899,10 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
     T2 *W = (T2 *)a.work, *RH = (T2 *)a.rho_hat;
     const bool landau = a.model == CH_LANDAU;
     const size_t bins = (size_t)rows * N;
+    const bool rhl = a.rh_lines && dims == 2 && (LOG2N == 10 || LOG2N == 11);
     if(init) {
         launch_xfwd<T, CH_LANDAU, false, LX>(rho, dims == 1 ? RH : W, tw2, p, rows, L.rs, L.ks, s);
-        if(dims == 2) pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH);
+        if(dims == 2) pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH, rhl);
         if(dims == 3) {
             pass_y<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s);
             pass_z<T, 0, LOG2N>(L, W, RH, tw2, p, lines, s, RH);
@@ -909,7 +919,7 @@ static hipError_t ps_chain_t(const SpectralArgs &a, hipStream_t s, int64_t nstep
             hipLaunchKernelGGL((k_update1d<T>), dim3((Nh + 255) / 256), dim3(256), 0, s, W, RH, p, Nh);
         }
         else if(dims == 2) {
-            pass_y<T, 2, LOG2N>(L, W, RH, tw2, p, lines, s);
+            pass_y<T, 2, LOG2N>(L, W, RH, tw2, p, lines, s, nullptr, rhl);
         }
         else {
             // y forward (line l: plane l / Nh, column l % Nh), the fused z pass, y inverse
