@@ -275,6 +275,17 @@ __device__ __forceinline__ T mobility_update(const Params &p, T psi, T mu, const
     return fma_t(res, (T)p.dt, psi);
 }
 
+// A workgroup barrier that orders LDS only: the release and acquire fences are
+// restricted to the local address space, so it waits for the wave's LDS
+// operations (lgkmcnt) but not for its outstanding global loads and stores, as
+// __syncthreads does (vmcnt(0)).  For exchanges through LDS whose global
+// accesses need no ordering between the block's threads.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // XCD-aware block order: gfx950 hands workgroups to its 8 XCDs round-robin by
 // id, so block b runs on XCD b % 8.  Mapping XCD x to the x-th contiguous
 // range of tiles keeps a band of the domain in one XCD's L2 from step to step.

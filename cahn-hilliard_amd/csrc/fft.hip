@@ -353,11 +353,17 @@ __global__ __launch_bounds__((FftGeom<LOG2N, NLC>::NT)) void k_col(
                 else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
                 const int q = pidx(lb * n + j);
                 buf[q] = spectral_update<T, T2>(p, ix, iy, iz, buf[q], rh[i], inv_bins);
-                rho_hat[gi[i]] = rh[i];
             }
         }
         __syncthreads();
         fft.template run<true>(buf);
+        // after the transform's barriers (each barrier waits for earlier global stores)
+#pragma unroll
+        for(int i = 0; i < 4; i++) {
+            const int e = threadIdx.x + i * NT;
+            const int lb = CONTIG ? e / n : e % nl;
+            if(l0 + lb < lines) rho_hat[gi[i]] = rh[i];
+        }
     }
 #pragma unroll
     for(int i = 0; i < 4; i++) {
@@ -380,6 +386,10 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
 }
 
 // ---- one wave per line (wave_fft.h), for lines of 512, 1024 or 2048 points.
+// Their barriers order LDS only (lds_barrier): a line's global loads stay in
+// flight across them and its stores need not complete before the next one.
+// Every global load of these kernels is consumed before the first barrier or
+// rewritten later by its own lane only, so no other lane can overwrite it early.
 // Fewer LDS passes than LineFFT (two exchanges per transform, wave-local) and
 // the line's entries stay in registers between the forward transform, the
 // spectral update and the inverse transform.
@@ -388,13 +398,13 @@ template <typename T2, int LOG2N, bool INV, int E>
 __device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, const T2 *tw, int tstride) {
     static_assert(E == WaveFFT<T2, LOG2N>::E, "entries per lane");
     using F = WaveFFT<T2, LOG2N>;
-    __syncthreads();
+    lds_barrier();
     F::template pass1_write<INV>(lane, v, lds);
-    __syncthreads();
+    lds_barrier();
     F::pass_read(lane, v, lds);
-    __syncthreads();
+    lds_barrier();
     F::template pass2_write<INV>(lane, v, lds, tw, tstride);
-    __syncthreads();
+    lds_barrier();
     F::pass_read(lane, v, lds);
     F::template pass3_final<INV>(lane, v, tw, tstride);
 }
@@ -402,19 +412,41 @@ __device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, co
 // The system's twiddle table (N entries, W_N^m) staged in LDS when N <= 1024:
 // one wave runs per SIMD at most (a line per wave), so nothing hides a global
 // load, and every transform pass after the first barrier would otherwise wait
-// on its twiddles.  The copy's loads issue together with the line's.
+// on its twiddles.  load() fetches a thread's share of the table into
+// registers and store() writes it to LDS: a kernel issues load(), then the
+// line's own loads, then store(), so that the two sets of loads are in flight
+// together (a store straight after each load would wait out the table's
+// latency before the line's loads issue).
 template <typename T2, int TN>
 struct TwiddleStage {
     static constexpr bool on = TN <= 1024;
     T2 t[on ? TN : 1];
-    __device__ __forceinline__ const T2 *fill(const T2 *__restrict__ tw, int lane) {
+    // entries per thread of NT threads (the register array of load / store)
+    template <int NT>
+    static constexpr int per() { return on ? TN / NT : 1; }
+    template <int NT>
+    __device__ __forceinline__ void load(const T2 *__restrict__ tw, int tid, T2 (&r)[per<NT>()]) const {
         if constexpr(on) {
 #pragma unroll
-            for(int i = 0; i < TN / 64; i++) t[lane + 64 * i] = tw[lane + 64 * i];
-            return t;  // visible after the transform's first barrier
+            for(int i = 0; i < TN / NT; i++) r[i] = tw[tid + NT * i];
         }
         else {
-            (void)lane;
+            (void)tw;
+            (void)tid;
+            (void)r;
+        }
+    }
+    // the table to read from: the LDS copy, visible after the next barrier
+    template <int NT>
+    __device__ __forceinline__ const T2 *store(const T2 *tw, int tid, const T2 (&r)[per<NT>()]) {
+        if constexpr(on) {
+#pragma unroll
+            for(int i = 0; i < TN / NT; i++) t[tid + NT * i] = r[i];
+            return t;
+        }
+        else {
+            (void)tid;
+            (void)r;
             return tw;
         }
     }
@@ -431,11 +463,13 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E;
     __shared__ T2 lds[F::LDS];
-    __shared__ TwiddleStage<T2, (1 << LOG2N)> stage;  // table length N = line length
+    using Stage = TwiddleStage<T2, (1 << LOG2N)>;  // table length N = line length
+    __shared__ Stage stage;
     const int l = xcd_tile(blockIdx.x, gridDim.x);
     if(l >= lines) return;
     const int lane = threadIdx.x;
-    tw = stage.fill(tw, lane);
+    T2 twr[Stage::template per<64>()];
+    stage.template load<64>(tw, lane, twr);
     const int Nh = p.N / 2 + 1;
     const size_t base = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls;
     const int tstride = p.N >> LOG2N;
@@ -446,6 +480,7 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
 #pragma unroll
         for(int m = 0; m < E; m++) rh[m] = rho_hat[base + (size_t)(lane + 64 * m) * stride];
     }
+    tw = stage.template store<64>(tw, lane, twr);
     wave_transform<T2, LOG2N, MODE == 1>(lane, v, lds, tw, tstride);
     if constexpr(MODE == 2) {
         const T inv_bins = (T)p.inv_bins;
@@ -457,9 +492,11 @@ __global__ __launch_bounds__(64) void k_colw(const typename Complex<T>::type *__
             if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
             else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
             v[m] = spectral_update<T, T2>(p, ix, iy, iz, v[m], rh[m], inv_bins);
-            rho_hat[base + (size_t)j * stride] = rh[m];
         }
         wave_transform<T2, LOG2N, true>(lane, v, lds, tw, tstride);
+        // after the transform's barriers (each barrier waits for earlier global stores)
+#pragma unroll
+        for(int m = 0; m < E; m++) rho_hat[base + (size_t)(lane + 64 * m) * stride] = rh[m];
     }
 #pragma unroll
     for(int m = 0; m < E; m++) dst[base + (size_t)(lane + 64 * m) * stride] = v[m];
@@ -495,15 +532,21 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
     using F = WaveFFT<T2, LOG2N - 1>;
     constexpr int E = F::E, n2 = F::n;
     __shared__ T2 lds[2][F::LDS];
-    __shared__ TwiddleStage<T2, 2 * n2> stage;  // table length N = line length
+    using Stage = TwiddleStage<T2, 2 * n2>;  // table length N = line length
+    __shared__ Stage stage;
     const int l = xcd_tile(blockIdx.x, gridDim.x);
     if(l >= lines) return;
     const int lane = threadIdx.x & 63;
     const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if constexpr(TwiddleStage<T2, 2 * n2>::on) {
+    constexpr int TWR = Stage::on ? 2 * n2 / 128 : 1;
+    T twx[TWR], twy[TWR];  // this thread's share of the table, stored to LDS after the line's loads issue
+    if constexpr(Stage::on) {
 #pragma unroll
-        for(int i = 0; i < 2 * n2 / 128; i++) stage.t[threadIdx.x + 128 * i] = tw[threadIdx.x + 128 * i];
-        tw = stage.t;  // visible after the first barrier
+        for(int i = 0; i < TWR; i++) {
+            const T2 w = tw[threadIdx.x + 128 * i];
+            twx[i] = w.x;
+            twy[i] = w.y;
+        }
     }
     const int Nh = p.N / 2 + 1;
     const size_t base = (size_t)(l / Nh) * P + (size_t)(l % Nh) * ls;
@@ -536,6 +579,11 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
 #pragma unroll
         for(int m = 0; m < E; m++) rh[m] = rho_hat[rh_at(lane + 64 * m + h * n2)];
     }
+    if constexpr(Stage::on) {
+#pragma unroll
+        for(int i = 0; i < TWR; i++) stage.t[threadIdx.x + 128 * i] = T2{twx[i], twy[i]};
+        tw = stage.t;  // visible after the first barrier
+    }
     auto forward = [&]() {
         wave_transform<T2, LOG2N - 1, false>(lane, v, own, tw, ts);
         // wave 1 turns O_k into W_n^k O_k; both publish, then combine
@@ -544,10 +592,10 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             const int k = lane + 64 * m;
             if(h == 1) v[m] = cmul(v[m], tw[k * (ts / 2)]);
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for(int m = 0; m < E; m++) own[lane + 64 * m] = v[m];
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for(int m = 0; m < E; m++) {
             const T2 o = other[lane + 64 * m];
@@ -555,10 +603,10 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
         }
     };
     auto inverse = [&]() {
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for(int m = 0; m < E; m++) own[lane + 64 * m] = v[m];
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for(int m = 0; m < E; m++) {
             const int k = lane + 64 * m;
@@ -580,9 +628,12 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             if(axis == 1) { iy = j; iz = l / Nh; }  // y pass (2D last axis)
             else { iy = l / Nh; iz = j; }            // z pass (3D last axis)
             v[m] = spectral_update<T, T2>(p, ix, iy, iz, v[m], rh[m], inv_bins);
-            rho_hat[rh_at(j)] = rh[m];
         }
         inverse();
+        // the new state after the inverse transform's barriers: a barrier's
+        // fence waits for every global store issued before it
+#pragma unroll
+        for(int m = 0; m < E; m++) rho_hat[rh_at(lane + 64 * m + h * n2)] = rh[m];
     }
     if constexpr(MODE == 4) {
 #pragma unroll
@@ -645,10 +696,10 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
                                          T2 *__restrict__ out, long ks) {
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
-    __syncthreads();
+    lds_barrier();
     auto emit = [&](int k, T2 zk, T2 znk) { out[(size_t)k * ks] = r2c_entry<T>(zk, znk, tw[k]); };
 #pragma unroll
     for(int m = 0; m < E; m++) {
@@ -667,18 +718,21 @@ __global__ __launch_bounds__(64) void k_xfwdw(const T *__restrict__ in, typename
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
-    __shared__ TwiddleStage<T2, 2 * n> stage;  // table length N = 2 n
+    using Stage = TwiddleStage<T2, 2 * n>;  // table length N = 2 n
+    __shared__ Stage stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
-    tw = stage.fill(tw, lane);
+    T2 twr[Stage::template per<64>()];
+    stage.template load<64>(tw, lane, twr);
     const T2 *rin = reinterpret_cast<const T2 *>(in + (size_t)row * 2 * n);
     T2 v[E];
 #pragma unroll
-    for(int m = 0; m < E; m++) {
-        T2 x = rin[lane + 64 * m];
-        if constexpr(FPRIME) x = T2{fprime<T, MODEL>(p, x.x), fprime<T, MODEL>(p, x.y)};
-        v[m] = x;
+    for(int m = 0; m < E; m++) v[m] = rin[lane + 64 * m];
+    tw = stage.template store<64>(tw, lane, twr);
+    if constexpr(FPRIME) {
+#pragma unroll
+        for(int m = 0; m < E; m++) v[m] = T2{fprime<T, MODEL>(p, v[m].x), fprime<T, MODEL>(p, v[m].y)};
     }
     wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
     r2c_post<T, LOG2N>(v, lds, lane, tw, out + (size_t)row * rs, ks);
@@ -693,13 +747,16 @@ __global__ __launch_bounds__(64) void k_xinvw(const typename Complex<T>::type *_
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
-    __shared__ TwiddleStage<T2, 2 * n> stage;
+    using Stage = TwiddleStage<T2, 2 * n>;
+    __shared__ Stage stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
+    T2 twr[Stage::template per<64>()];
+    stage.template load<64>(tw, lane, twr);
     T2 v[E];
     c2r_pre<T, E>(in + (size_t)row * rs, ks, lane, tw, v);  // before the first barrier: global table
-    const T2 *twl = stage.fill(tw, lane);
+    const T2 *twl = stage.template store<64>(tw, lane, twr);
     wave_transform<T2, LOG2N, true>(lane, v, lds, twl, 2);
     T2 *rout = reinterpret_cast<T2 *>(out + (size_t)row * 2 * n);
 #pragma unroll
@@ -715,14 +772,17 @@ __global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restr
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
-    __shared__ TwiddleStage<T2, 2 * n> stage;
+    using Stage = TwiddleStage<T2, 2 * n>;
+    __shared__ Stage stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
+    T2 twr[Stage::template per<64>()];
+    stage.template load<64>(tw, lane, twr);
     T2 *w = W + (size_t)row * rs;
     T2 v[E];
     c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
-    tw = stage.fill(tw, lane);
+    tw = stage.template store<64>(tw, lane, twr);
     wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
     if(rho_out) {
         T2 *ro = reinterpret_cast<T2 *>(rho_out + (size_t)row * 2 * n);
@@ -746,18 +806,21 @@ __global__ __launch_bounds__(64) void k_xfluxw(typename Complex<T>::type *__rest
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
     __shared__ T2 lds[F::LDS];
-    __shared__ TwiddleStage<T2, 2 * n> stage;
+    using Stage = TwiddleStage<T2, 2 * n>;
+    __shared__ Stage stage;
     const int row = xcd_tile(blockIdx.x, gridDim.x);
     if(row >= rows) return;
     const int lane = threadIdx.x;
+    T2 twr[Stage::template per<64>()];
+    stage.template load<64>(tw, lane, twr);
     T2 *w = W + (size_t)row * rs;
     const T2 *mrow = reinterpret_cast<const T2 *>(mob + (size_t)row * 2 * n);
     const T M0 = sub ? (T)*M0p : (T)0, ib = (T)p.inv_bins;
     T2 v[E], mv[E];
-    c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
 #pragma unroll
     for(int m = 0; m < E; m++) mv[m] = mrow[lane + 64 * m];
-    tw = stage.fill(tw, lane);
+    c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
+    tw = stage.template store<64>(tw, lane, twr);
     wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
 #pragma unroll
     for(int m = 0; m < E; m++) v[m] = T2{psm_flux<T>(v[m].x, mv[m].x, M0, ib), psm_flux<T>(v[m].y, mv[m].y, M0, ib)};
