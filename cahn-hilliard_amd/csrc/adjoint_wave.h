@@ -90,7 +90,10 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
     if constexpr(AHEAD2)
         if(KA > 1 && reads_tape(1)) load(tape + (size_t)(s_top - 1) * NN, nxt2);
     const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
-    double ge = 0, gk = 0, gm = 0, gd = 0;
+    // gradient terms of this thread's cells over the launch's steps (at most
+    // KA C RPW of them), summed in FP64 across threads and launches: in T, so
+    // that an FP32 sweep spends no FP64 instructions per cell on them
+    T ge = 0, gk = 0, gm = 0, gd = 0;
 #pragma unroll 1
     for(int st = 0; st < KA; st++) {
         if constexpr(AHEAD2) {
@@ -155,13 +158,17 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
 #pragma unroll
                 for(int c = 0; c < C; c++) {
                     const T lw = laplacian<T, 2>(wv[c][j], m[c], q[c], inv_dx2);
-                    const T fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
+                    // f'' in T for Landau FP32 fields (-eps + 3 psi^2), else FP64
+                    T fpp;
+                    if constexpr(sizeof(T) == 4 && MODEL == CH_LANDAU) fpp = (T)3 * psi[c][j] * psi[c][j] - (T)p.eps;
+                    else fpp = (T)fsecond<MODEL>(p, (double)psi[c][j]);
                     if(rin && col_in(c)) {
-                        const double wd = (double)wv[c][j], ps = (double)psi[c][j], lwd = (double)lw;
+                        const T wd = wv[c][j], ps = psi[c][j];
                         ge += wd * ps;
-                        gk += lwd * ps;
-                        gm += wd * (double)fprime<T, MODEL>(p, psi[c][j]) - p.two_k * (lwd * ps);
-                        if constexpr(MODEL == CH_SIMPLE_WERTHEIM) gd += wd * dfprime_ddelta<MODEL>(p, ps);
+                        gk += lw * ps;
+                        gm += wd * fprime<T, MODEL>(p, ps) - two_k * (lw * ps);
+                        if constexpr(MODEL == CH_SIMPLE_WERTHEIM)
+                            gd += (T)((double)wd * dfprime_ddelta<MODEL>(p, (double)ps));
                     }
                     lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
                 }
@@ -178,7 +185,7 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
             if(col_in(c)) lam_out[row + gx0 + C * lane + c] = lam[c][j];
     }
 
-    const double g[kGradSlots] = {ge, gk, gm, gd};
+    const double g[kGradSlots] = {(double)ge, (double)gk, (double)gm, (double)gd};
 #pragma unroll
     for(int i = 0; i < kGradSlots; i++) {
         const double v = wave_sum64(g[i]);
