@@ -41,8 +41,8 @@ struct ch_sys {
     void *stage = nullptr;   // bins doubles: host transfers of FP32 fields, mobility
     void *stage2 = nullptr;  // bins elements of T: mobility
     void *split_mob = nullptr;  // bins elements of T: the cells' mobilities of the two-launch 3D step
-    // the three-launch 3D euler_mobility reverse step: nu and eta
-    void *adj_split[2] = {nullptr, nullptr};
+    // the 3D euler_mobility reverse step of one cell per thread: nu
+    void *adj_split[1] = {nullptr};
     double *partials = nullptr, *result = nullptr;
     // reverse mode: checkpoints, the window tape, lambda ping-pong, gradient slots
     void *ckpt = nullptr, *tape = nullptr, *lam[2] = {nullptr, nullptr};
@@ -345,7 +345,7 @@ static void free_sys(ch_sys *s) {
     void *bufs[] = {s->noise_step, s->work, s->rho_hat, s->twiddles, s->stage, s->stage2, s->split_mob, s->partials, s->result,
                     s->ckpt, s->tape, s->lam[0], s->lam[1], s->gacc, s->ckpt_hat, s->ps_rh, s->ps_w,
                     s->ps_spec[0], s->ps_spec[1], s->ps_spec[2], s->ps_real[0], s->ps_real[1],
-                    s->adj_split[0], s->adj_split[1]};
+                    s->adj_split[0]};
     for(void *b : bufs)
         if(b) (void)hipFree(b);
     for(void *b : s->psm_spec)
@@ -1814,7 +1814,7 @@ static ch_status adjoint_core(ch_sys *s, double *grads) {
         if(adj_split(s)) {
             for(int64_t t = len - 1; t >= 0; t--) {
                 char *slot = field_at(tape, t, tape_slot(s));
-                void *const w[4] = {slot + fb, slot + 2 * fb, s->adj_split[0], s->adj_split[1]};
+                void *const w[4] = {slot + fb, slot + 2 * fb, s->adj_split[0], nullptr};
                 HIPCHK(c, chd::adjoint_split3d(s->p, s->d.precision, s->d.model, s->d.mobility, s->lam[li],
                                                s->lam[li ^ 1], slot, w, true, s->gacc, ms));
                 li ^= 1;

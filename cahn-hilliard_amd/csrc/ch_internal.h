@@ -26,8 +26,8 @@ hipError_t split3d_mu(const Params &p, int precision, int integrator, int model,
                       void *mob_out, hipStream_t s);
 // the 3D euler_mobility reverse step as three launches (Params::tile3d == 3,
 // mobility not free_energy): lambda_in -> lambda_out with psi = psi_n; work =
-// four N^3 fields (mu, mobility, nu, eta), the first two already holding mu_n
-// and the mobilities of psi_n when have_mu (then two launches); gradient
+// three N^3 fields (mu, mobility, nu; work[3] unused), the first two already
+// holding mu_n and the mobilities of psi_n when have_mu (then two launches); gradient
 // partials into the first adjoint_split3d_blocks(p) slot groups of gacc
 bool adjoint_split3d_ok(const Params &p, int mob);
 int adjoint_split3d_blocks(const Params &p);

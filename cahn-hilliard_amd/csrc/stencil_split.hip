@@ -193,60 +193,59 @@ hipError_t stencil_split3d(const Params &p, int precision, int integrator, int m
 // ---- the euler_mobility reverse step in three launches (the reverse of the
 // two-launch forward; the LDS tile form is k_adjoint_em in adjoint.hip):
 //   k_split_mu       mu_n and the cells' mobilities from psi_n
-//   k_adj_split_nu   nu = D(Mob, lambda) and eta of every cell
-//   k_adj_split_upd  lambda_n from lambda, nu (with its Laplacian), eta and
-//                    psi_n, and the block sums of the parameter gradients
+//   k_adj_split_nu   nu = D(Mob, lambda) of every cell
+//   k_adj_split_upd  eta from mu_n and lambda, then lambda_n from lambda, nu
+//                    (with its Laplacian), eta and psi_n, and the block sums of
+//                    the parameter gradients
 // Each cell's f', mobility, f'' and mobility derivatives are evaluated once,
 // where the tile evaluates f' and the mobility on its halo-extended box; the
 // per-cell arithmetic is the tile's (adjoint_math.h), so lambda has the same
 // bits, and only the order of the FP64 gradient sums differs.
 template <typename T, int MOB, int Z>
-__global__ __launch_bounds__(256) void k_adj_split_nu(const T *__restrict__ lam, const T *__restrict__ mu,
-                                                      const T *__restrict__ mob, T *__restrict__ nu,
-                                                      T *__restrict__ eta, Params p) {
+__global__ __launch_bounds__(256) void k_adj_split_nu(const T *__restrict__ lam, const T *__restrict__ mob,
+                                                      T *__restrict__ nu, Params p) {
     const ZCol<Z> g(p);
     constexpr bool VARM = MOB != CH_MOB_CONSTANT;
-    T lc[Z + 2], ln[Z][4], uc[Z + 2], un[Z][4];
+    T lc[Z + 2], ln[Z][4];
     T mc[VARM ? Z + 2 : 1], mn[VARM ? Z : 1][4];
     zcol_load<T, Z>(lam, g, lc, ln);
-    zcol_load<T, Z>(mu, g, uc, un);
     if constexpr(VARM) zcol_load<T, Z>(mob, g, mc, mn);
     const T inv_dx2 = (T)p.inv_dx2, Mc = (T)p.M;
 #pragma unroll
     for(int j = 0; j < Z; j++) {
-        T lm[3], lq[3], um[3], uq[3], Mm[3] = {Mc, Mc, Mc}, Mq[3] = {Mc, Mc, Mc};
+        T lm[3], lq[3], Mm[3] = {Mc, Mc, Mc}, Mq[3] = {Mc, Mc, Mc};
         zcol_nb<T, Z>(lc, ln, j, lm, lq);
-        zcol_nb<T, Z>(uc, un, j, um, uq);
         if constexpr(VARM) zcol_nb<T, Z>(mc, mn, j, Mm, Mq);
-        const T l = lc[j + 1];
-        const uint32_t at = g.plane(j) + g.pc;
-        nu[at] = adj_nu<T, 3>(l, VARM ? mc[j + 1] : Mc, lm, lq, Mm, Mq, inv_dx2);
-        eta[at] = adj_eta<T, 3>(uc[j + 1], l, um, uq, lm, lq, inv_dx2);
+        nu[g.plane(j) + g.pc] = adj_nu<T, 3>(lc[j + 1], VARM ? mc[j + 1] : Mc, lm, lq, Mm, Mq, inv_dx2);
     }
 }
 
 template <typename T, int MODEL, int MOB, int Z>
 __global__ __launch_bounds__(256) void k_adj_split_upd(const T *__restrict__ lam, const T *__restrict__ nu,
-                                                       const T *__restrict__ eta, const T *__restrict__ psi,
+                                                       const T *__restrict__ mu, const T *__restrict__ psi,
                                                        T *__restrict__ lam_out, Params p, double *__restrict__ gacc) {
     __shared__ double red[kGradSlots][4];
     const ZCol<Z> g(p);
-    T nc[Z + 2], nn[Z][4], pc[Z + 2], pn[Z][4];
+    T nc[Z + 2], nn[Z][4], pc[Z + 2], pn[Z][4], lc[Z + 2], ln[Z][4], uc[Z + 2], un[Z][4];
     zcol_load<T, Z>(nu, g, nc, nn);
     zcol_load<T, Z>(psi, g, pc, pn);
+    zcol_load<T, Z>(lam, g, lc, ln);
+    zcol_load<T, Z>(mu, g, uc, un);
     const T inv_dx2 = (T)p.inv_dx2;
     double ge = 0, gk = 0, gm = 0, gd = 0;
 #pragma unroll
     for(int j = 0; j < Z; j++) {
-        T nm[3], nq[3], pm[3], pq[3];
+        T nm[3], nq[3], pm[3], pq[3], lm[3], lq[3], um[3], uq[3];
         zcol_nb<T, Z>(nc, nn, j, nm, nq);
         zcol_nb<T, Z>(pc, pn, j, pm, pq);
+        zcol_nb<T, Z>(lc, ln, j, lm, lq);
+        zcol_nb<T, Z>(uc, un, j, um, uq);
         const uint32_t at = g.plane(j) + g.pc;
-        const T nv = nc[j + 1], et = eta[at];
+        const T nv = nc[j + 1], et = adj_eta<T, 3>(uc[j + 1], lc[j + 1], um, uq, lm, lq, inv_dx2);
         const T lnu = laplacian<T, 3>(nv, nm, nq, inv_dx2);
         const double ps = (double)pc[j + 1];
         const MobDeriv<MOB> md = mob_deriv<MOB>(p, ps);
-        lam_out[at] = adj_em_step<T>(p, lam[at], (T)fsecond<MODEL>(p, ps), nv, lnu, (T)md.drho, et);
+        lam_out[at] = adj_em_step<T>(p, lc[j + 1], (T)fsecond<MODEL>(p, ps), nv, lnu, (T)md.drho, et);
         const double lps = (double)laplacian<T, 3>(pc[j + 1], pm, pq, inv_dx2);
         ge += (double)nv * ps;
         gk += (double)nv * lps;
@@ -268,10 +267,10 @@ static hipError_t adj_split_t(const Params &p, const void *lam_in, void *lam_out
     const int64_t bins = (int64_t)1 << (3 * p.log2N);
     const int nbm = (int)(bins / (256 * ZM)), nbn = (int)(bins / (256 * ZN)), nbu = (int)(bins / (256 * ZU));
     const T *ps = (const T *)psi;
-    T *mu = (T *)w[0], *mob = (T *)w[1], *nu = (T *)w[2], *eta = (T *)w[3];
+    T *mu = (T *)w[0], *mob = (T *)w[1], *nu = (T *)w[2];
     if(!have_mu) hipLaunchKernelGGL((k_split_mu<T, MODEL, MOB, ZM>), dim3(nbm), dim3(256), 0, s, ps, mu, mob, p);
-    hipLaunchKernelGGL((k_adj_split_nu<T, MOB, ZN>), dim3(nbn), dim3(256), 0, s, (const T *)lam_in, mu, mob, nu, eta, p);
-    hipLaunchKernelGGL((k_adj_split_upd<T, MODEL, MOB, ZU>), dim3(nbu), dim3(256), 0, s, (const T *)lam_in, nu, eta,
+    hipLaunchKernelGGL((k_adj_split_nu<T, MOB, ZN>), dim3(nbn), dim3(256), 0, s, (const T *)lam_in, mob, nu, p);
+    hipLaunchKernelGGL((k_adj_split_upd<T, MODEL, MOB, ZU>), dim3(nbu), dim3(256), 0, s, (const T *)lam_in, nu, mu,
                        ps, (T *)lam_out, p, gacc);
     return hipGetLastError();
 }
