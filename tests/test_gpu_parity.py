@@ -343,6 +343,33 @@ def test_pointwise_model_and_mobility(ch, ctx, precision, tol):
         assert np.abs(out.double().cpu().numpy() - ref).max() <= tol * max(1.0, np.abs(ref).max()) * 4, mob
 
 
+def test_wertheim_fprime_wide_range(ch, ctx):
+    """SimpleWertheim f' = 2 B2 rho + ln(rho X^valence) over rho from 1e-9 to 1e3
+    (log-uniform) against the oracle's libm-based f', element by element: the
+    device logarithm (series of atanh((m - 1)/(m + 1)) with the exponent split
+    off) on arguments rho X^4 spanning ~14 decades, within 2e-15 of the terms'
+    magnitude.  delta = 1e10 keeps 2 valence delta rho >= 80, where X(rho) =
+    (sqrt(1 + 2 valence delta rho) - 1) / (valence delta rho) has no
+    cancellation (below ~1e-2 the subtraction loses digits in any
+    implementation, and a one-ulp square-root difference shows in X)."""
+    import torch
+    rng = np.random.default_rng(11)
+    rho = 10.0 ** rng.uniform(-9.0, 3.0, 1 << 16)
+    r = torch.tensor(rho, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(r)
+    om = O.Model(O.SIMPLE_WERTHEIM, B2=1.0, delta=1e10, valence=4)
+    d = ch.make_desc(dims=1, N=1 << 16, free_energy="simple_wertheim", B2=1.0, delta=1e10, valence=4)
+    torch.cuda.synchronize()
+    ch.der_bulk_free_energy(ctx, d, r.data_ptr(), out.data_ptr(), r.numel())
+    ctx.synchronize()
+    got = out.cpu().numpy()
+    ref = O.der_bulk(om, rho)
+    scale = np.abs(2.0 * rho) + np.abs(ref - 2.0 * rho) + 1.0
+    err = np.abs(got - ref) / scale
+    print(f"max scaled error {err.max():.2e} at rho = {rho[err.argmax()]:.3e}")
+    assert np.all(np.isfinite(got)) and err.max() < 2e-15
+
+
 def test_linear_mobility_against_oracle(ch, ctx):
     """mobility.type = "linear" (M rho / rho_ref, new) with SimpleWertheim in 3D (C5 shape, small N)."""
     kw = dict(dims=3, N=16, dt=1e-4, dx=1.0, k=1.0, M=1.0, rho_ref=0.05)
