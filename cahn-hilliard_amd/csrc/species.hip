@@ -9,7 +9,8 @@
 //       (at most 10000 sweeps), warm-started from the cell's X of the
 //       previous evaluation (X lives in HBM, patch-major, n_patches per cell).
 // A species whose density is exactly 0 gets f' = 0 (the reference leaves its
-// previous value in a static buffer).
+// previous value in a static buffer).  f' takes its logarithms from
+// log_series (ch_device.h, within 2 ulp); the free energy keeps the library's.
 //
 // Integrators over species-major fields rho[s * bins + i]:
 //   euler / euler_mobility: k_sp_mu (mobility from the previous X, X update,
@@ -57,7 +58,7 @@ __device__ double saleh_contribution(const SpModel &m, const double *rhos, int s
     const double delta = species == 0 ? m.dAA : m.dBB;
     const double rho_factor = delta * (m.val[species] * rhos[species] + m.lh * rhos[2]);
     const double X = (-1.0 + sqrt(1.0 + 4.0 * rho_factor)) / (2.0 * rho_factor);
-    return rho_factor >= 0 ? log(X) : 0.0;
+    return rho_factor >= 0 ? log_series(X) : 0.0;
 }
 
 // f' of every species at one cell; X (Generic) in/out
@@ -67,7 +68,7 @@ __device__ void sp_fprime(const SpModel &m, const double *rhos, double *X, doubl
     double logX[kSpMaxPatches];
     if(m.model == CH_GENERIC_WERTHEIM) {
         generic_update_X(m, rhos, X);
-        for(int i = 0; i < m.P; i++) logX[i] = log(X[i]);
+        for(int i = 0; i < m.P; i++) logX[i] = log_series(X[i]);
     }
     for(int s = 0; s < S; s++) {
         out[s] = 0.0;
@@ -76,30 +77,30 @@ __device__ void sp_fprime(const SpModel &m, const double *rhos, double *X, doubl
         for(int t = 0; t < S; t++) rho_tot += rhos[t];
         double ref, bond = 0.0;
         if(m.model == CH_SALEH_WERTHEIM) {
-            ref = log(rhos[s]) + 2.0 * m.sB2 * rho_tot + 3.0 * m.sB3 * (rho_tot * rho_tot);
+            ref = log_series(rhos[s]) + 2.0 * m.sB2 * rho_tot + 3.0 * m.sB3 * (rho_tot * rho_tot);
             if(s == 0) bond = m.val[0] * saleh_contribution(m, rhos, 0);
             else if(s == 1) bond = m.val[1] * saleh_contribution(m, rhos, 1);
             else bond = m.lh * (saleh_contribution(m, rhos, 0) + saleh_contribution(m, rhos, 1));
         }
         else if(m.model == CH_RICCI_WERTHEIM) {
-            ref = log(rhos[s]) + 2.0 * m.rB2 * rho_tot;
+            ref = log_series(rhos[s]) + 2.0 * m.rB2 * rho_tot;
             const double A = rhos[1] * m.d12, B = rhos[0] * m.d12;
             if(s == 0) {
                 const double X0 = (-1 + sqrt((1 + 12 * rhos[0] * m.d00))) / (6 * rhos[0] * m.d00);
                 const double u = 1 + 2 * A - B;
                 const double X1 = (B - 1 - 2 * A + sqrt(u * u + 4 * B)) / (2 * B);
-                bond = 3 * log(X0) + log(X1);
+                bond = 3 * log_series(X0) + log_series(X1);
             }
             else {
                 const double v = B - 2 * A + 1;
                 const double X2 = (-B + 2 * A - 1 + sqrt(v * v + 8 * A)) / (4 * A);
-                bond = 2 * log(X2);
+                bond = 2 * log_series(X2);
             }
         }
         else {
             double B2c = 0.;
             for(int o = 0; o < S; o++) B2c += 2.0 * rhos[o] * m.B2[s][o];
-            ref = log(rhos[s]) + B2c;
+            ref = log_series(rhos[s]) + B2c;
             for(int q = 0; q < m.P; q++)
                 if(m.pc[s][q] > 0) bond += m.pc[s][q] * logX[q];
         }
