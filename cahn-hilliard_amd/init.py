@@ -61,3 +61,30 @@ def replica_batch(sizes=(64, 128, 256, 512), per_size=64, seed=123456):
         eps.append(0.5 + 0.5 * float(r[0]))
         fields.append(-0.3 + 0.6 * r[1] + 0.02 * (r[2:] - 0.5))
     return Ns, eps, fields
+
+
+def write_replicas(directory, Ns, eps, fields, k=1.0, dt=0.01, steps=5000, print_every=100):
+    """Writes replica r of a batch as `replica_<r>.dat`, in the native text
+    format that `load_from` reads (src/CahnHilliard.cpp:72-132: a '#' header
+    line, then one line of N values per row y, x fastest; 17 significant digits
+    so the state is exact), and `replica_<r>.toml`, an input of the reference's
+    driver for the same system (Landau 2D euler, epsilon = eps[r], k, dt, M = 1,
+    dx = 1, load_from = the .dat file).  Returns the TOML paths."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    tomls = []
+    for r, (N, e, f) in enumerate(zip(Ns, eps, fields)):
+        f = np.asarray(f, dtype=np.float64).reshape(N, N)
+        dat = os.path.join(directory, f"replica_{r}.dat")
+        with open(dat, "w") as out:
+            out.write(f"# step = 0, t = 0, size = {N}x{N}x1 | density\n")
+            for row in f:
+                out.write(" ".join(repr(float(v)) for v in row) + " \n")
+        toml = os.path.join(directory, f"replica_{r}.toml")
+        with open(toml, "w") as out:
+            out.write(f"steps = {steps}\nprint_every = {print_every}\nN = {N}\ndt = {dt}\ndx = 1.0\nk = {k}\nM = 1.0\n"
+                      f'free_energy = "landau"\nintegrator = "euler"\nload_from = "{os.path.abspath(dat)}"\n'
+                      f"\n[landau]\nepsilon = {e!r}\n")
+        tomls.append(toml)
+    return tomls
+

@@ -335,3 +335,53 @@ def test_pressure_strategy_rejected(tmp_path):
     text = "steps = 1\n" + LANDAU_BASE + "[output]\nprint_pressure = true\nprint_pressure_strategy = \"cubic\"\n"
     r = run_bin(2, text, tmp_path, check=False)
     assert r.returncode != 0 and "Unsupported pressure printing strategy 'cubic'" in (r.stdout + r.stderr)
+
+
+def _load_from_rows(path, N):
+    """The rows a 2D load_from reads (src/CahnHilliard.cpp:72-132): lines not
+    starting with '#', N whitespace-separated values each, N of them."""
+    rows = [ln.split() for ln in open(path).read().split("\n") if ln.strip() and not ln.lstrip().startswith("#")]
+    assert len(rows) == N and all(len(r) == N for r in rows)
+    return np.array([[float(v) for v in r] for r in rows])
+
+
+def test_replica_files_round_trip(ch, tmp_path):
+    """write_replicas: each replica's field as a native file that load_from
+    reads back exactly (17 digits), and an input naming it with the replica's
+    epsilon."""
+    import tomli
+    from cahn_hilliard_amd.init import replica_batch, write_replicas
+    Ns, eps, fields = replica_batch(sizes=(64, 128), per_size=2)
+    tomls = write_replicas(str(tmp_path), Ns, eps, fields, steps=10, print_every=0)
+    assert len(tomls) == 4
+    for r, t in enumerate(tomls):
+        cfg = tomli.loads(open(t).read())
+        assert cfg["N"] == Ns[r] and cfg["landau"]["epsilon"] == eps[r] and cfg["integrator"] == "euler"
+        got = _load_from_rows(cfg["load_from"], Ns[r])
+        assert np.array_equal(got.ravel(), fields[r])
+
+
+@pytest.mark.gpu
+def test_replica_file_runs_on_dropin_binary(ch, tmp_path):
+    """A C3 replica written by write_replicas, run by the drop-in driver for 50
+    steps (lossless output), equals the same replica advanced in an FP64 batch,
+    bit for bit."""
+    from cahn_hilliard_amd.init import replica_batch, write_replicas
+    Ns, eps, fields = replica_batch(sizes=(64, 128), per_size=1)
+    tomls = write_replicas(str(tmp_path / "in"), Ns, eps, fields, steps=50, print_every=0)
+    ctx = ch.Context(0)
+    descs = [ch.make_desc(dims=2, N=N, integrator="euler", epsilon=e, k=1.0, dt=0.01) for N, e in zip(Ns, eps)]
+    b = ch.Batch(ctx, descs)
+    for r, f in enumerate(fields):
+        b.upload(r, f)
+    b.evolve(50)
+    want = [b.download(r) for r in range(len(Ns))]
+    b.close()
+    ctx.close()
+    for r, t in enumerate(tomls):
+        run = tmp_path / f"run{r}"
+        run.mkdir()
+        run_bin(2, open(t).read() + "\n[output]\nlossless = true\n", run)
+        got = np.loadtxt(run / "last_0.dat")
+        assert np.array_equal(got.ravel(), want[r]), r
+
