@@ -286,6 +286,16 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders the LDS accesses of one wave's lanes among themselves, for a buffer
+// no other wave touches meanwhile: a wave's LDS instructions execute in issue
+// order, so no wait and no s_barrier is needed, only that the compiler keep
+// the accesses on their side of this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // XCD-aware block order: gfx950 hands workgroups to its 8 XCDs round-robin by
 // id, so block b runs on XCD b % 8.  Mapping XCD x to the x-th contiguous
 // range of tiles keeps a band of the domain in one XCD's L2 from step to step.

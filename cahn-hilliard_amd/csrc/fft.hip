@@ -394,17 +394,21 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
 // the line's entries stay in registers between the forward transform, the
 // spectral update and the inverse transform.
 
+// The first barrier orders the block's earlier LDS accesses (the staged
+// twiddle table, another wave's reads of this buffer) before the transform;
+// the buffer is the wave's own from there on, so the passes within the
+// transform are ordered by wave_lds_sync alone.
 template <typename T2, int LOG2N, bool INV, int E>
 __device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, const T2 *tw, int tstride) {
     static_assert(E == WaveFFT<T2, LOG2N>::E, "entries per lane");
     using F = WaveFFT<T2, LOG2N>;
     lds_barrier();
     F::template pass1_write<INV>(lane, v, lds);
-    lds_barrier();
+    wave_lds_sync();
     F::pass_read(lane, v, lds);
-    lds_barrier();
+    wave_lds_sync();
     F::template pass2_write<INV>(lane, v, lds, tw, tstride);
-    lds_barrier();
+    wave_lds_sync();
     F::pass_read(lane, v, lds);
     F::template pass3_final<INV>(lane, v, tw, tstride);
 }
@@ -592,7 +596,7 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
             const int k = lane + 64 * m;
             if(h == 1) v[m] = cmul(v[m], tw[k * (ts / 2)]);
         }
-        lds_barrier();
+        wave_lds_sync();  // the other wave has not read this buffer since the transform's barrier
 #pragma unroll
         for(int m = 0; m < E; m++) own[lane + 64 * m] = v[m];
         lds_barrier();
@@ -696,10 +700,10 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
                                          T2 *__restrict__ out, long ks) {
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
-    lds_barrier();
+    wave_lds_sync();  // one-wave blocks: the buffer is the wave's own
 #pragma unroll
     for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
-    lds_barrier();
+    wave_lds_sync();
     auto emit = [&](int k, T2 zk, T2 znk) { out[(size_t)k * ks] = r2c_entry<T>(zk, znk, tw[k]); };
 #pragma unroll
     for(int m = 0; m < E; m++) {
