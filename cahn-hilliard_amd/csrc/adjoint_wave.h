@@ -9,18 +9,24 @@
 // (stencil_wave.h): lane l owns C adjacent columns, wave w owns RPW rows, x
 // neighbours through DPP wave shifts, y neighbours from registers and, for the
 // first / last row of a wave, through LDS.  KA backward steps run per launch on
-// a region carrying a 2 KA halo of lambda; psi_n comes from the tape, the next
-// step's rows fetched while the current step computes.  As in the forward
-// kernel, a phase computes only the rows later phases read (lambda after
-// backward step st within 2(KA-1-st) rows of the interior, w of step st within
-// 2(KA-st)-1): a wave whose rows all lie further out skips the phase and the
-// tape rows it would have read.
+// a region carrying a 2 KA halo of lambda; psi_n comes from the tape, the rows
+// of later steps in flight while a step computes.  As in the forward kernel, a
+// phase computes only the rows later phases read (lambda after backward step
+// st within 2(KA-1-st) rows of the interior, w of step st within 2(KA-st)-1):
+// a wave whose rows all lie further out skips the phase.
 #pragma once
 
 #include "adjoint_math.h"
 #include "stencil_wave.h"
 
 namespace chd {
+
+// tape buffers of the reverse sweep's ring (Landau tiles): up to 16 tape
+// values per lane in flight, at least two buffers, at most one per step
+constexpr int adj_ring_depth(int KA, int cells) {
+    const int d = 16 / cells < 2 ? 2 : 16 / cells;
+    return d < KA ? d : KA;
+}
 
 template <typename T, int MODEL, int KA, class CFG, bool WHOLE>
 __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__restrict__ lam_in, T *__restrict__ lam_out,
@@ -70,50 +76,23 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
         if constexpr(WHOLE) return true;
         else return w * RPW + RPW > H - d && w * RPW < RH - H + d;
     };
+
     // backward step st reads psi where its lambda update is needed
     auto reads_tape = [&](int st) { return needed(2 * (KA - 1 - st)); };
 
-    // psi_n of the next step (and, with <= 4 cells per lane, of the one after)
-    // is in flight while a step computes
-    constexpr bool AHEAD2 = C * RPW <= 4;
-    T lam[C][RPW], psi[C][RPW], nxt[C][RPW], nxt2[AHEAD2 ? C : 1][AHEAD2 ? RPW : 1], wv[C][RPW];
+    T lam[C][RPW], wv[C][RPW];
     load(lam_in, lam);
 #pragma unroll
     for(int j = 0; j < RPW; j++)
 #pragma unroll
-        for(int c = 0; c < C; c++) {
-            wv[c][j] = 0;
-            nxt[c][j] = 0;
-            if constexpr(AHEAD2) nxt2[c][j] = 0;
-        }
-    if(reads_tape(0)) load(tape + (size_t)s_top * NN, nxt);
-    if constexpr(AHEAD2)
-        if(KA > 1 && reads_tape(1)) load(tape + (size_t)(s_top - 1) * NN, nxt2);
+        for(int c = 0; c < C; c++) wv[c][j] = 0;
     const T inv_dx2 = (T)p.inv_dx2, two_k = (T)p.two_k, Mdt = (T)p.Mdt;
     // gradient terms of this thread's cells over the launch's steps (at most
     // KA C RPW of them), summed in FP64 across threads and launches: in T, so
     // that an FP32 sweep spends no FP64 instructions per cell on them
     T ge = 0, gk = 0, gm = 0, gd = 0;
-#pragma unroll 1
-    for(int st = 0; st < KA; st++) {
-        if constexpr(AHEAD2) {
-#pragma unroll
-            for(int j = 0; j < RPW; j++)
-#pragma unroll
-                for(int c = 0; c < C; c++) {
-                    psi[c][j] = nxt[c][j];
-                    nxt[c][j] = nxt2[c][j];
-                }
-            if(st + 2 < KA && reads_tape(st + 2)) load(tape + (size_t)(s_top - st - 2) * NN, nxt2);
-        }
-        else {
-#pragma unroll
-            for(int j = 0; j < RPW; j++)
-#pragma unroll
-                for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
-            if(st + 1 < KA && reads_tape(st + 1)) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
-        }
-
+    // backward step st with psi_n in psi
+    auto step = [&](int st, const T(&psi)[C][RPW]) {
         // ---- phase 1: w = L lambda
 #pragma unroll
         for(int c = 0; c < C; c++) {
@@ -173,6 +152,67 @@ __global__ __launch_bounds__(64 * CFG::NW) void k_adjoint2d_wave(const T *__rest
                     lam[c][j] = lam[c][j] + Mdt * (fpp * wv[c][j] - two_k * lw);
                 }
             }
+        }
+    };
+
+    if constexpr(MODEL == CH_LANDAU && !WHOLE) {
+        // psi_n of backward step st sits in ring buffer st % D: the loads of
+        // the first D steps issue together with lambda's, and a step's buffer
+        // is refilled with the rows of step st + D once the step has read it.
+        // The step loop is unrolled, so every buffer index is a constant: no
+        // register copies between steps, and a step waits only for its own
+        // rows (a wave's loads complete in issue order), D - 1 steps after
+        // they were issued.  Every wave loads its rows of every step, also
+        // those of a phase it skips (rows wrap, so all addresses lie in the
+        // field): a load under a per-wave branch would leave the compiler
+        // unable to count the loads in flight, and it would wait for all of
+        // them at the first barrier.
+        constexpr int D = adj_ring_depth(KA, C * RPW);
+        T ring[D][C][RPW];
+#pragma unroll
+        for(int b = 0; b < D; b++) load(tape + (size_t)(s_top - b) * NN, ring[b]);
+#pragma unroll
+        for(int st = 0; st < KA; st++) {
+            step(st, ring[st % D]);
+            if(st + D < KA) load(tape + (size_t)(s_top - st - D) * NN, ring[st % D]);
+        }
+    }
+    else {
+        // SimpleWertheim and whole-grid regions, whose registers an unrolled
+        // loop would overflow: psi_n of the next step (and, with <= 4 cells
+        // per lane, of the one after) is in flight while a step computes
+        constexpr bool AHEAD2 = C * RPW <= 4;
+        T psi[C][RPW], nxt[C][RPW], nxt2[AHEAD2 ? C : 1][AHEAD2 ? RPW : 1];
+#pragma unroll
+        for(int j = 0; j < RPW; j++)
+#pragma unroll
+            for(int c = 0; c < C; c++) {
+                nxt[c][j] = 0;
+                if constexpr(AHEAD2) nxt2[c][j] = 0;
+            }
+        if(reads_tape(0)) load(tape + (size_t)s_top * NN, nxt);
+        if constexpr(AHEAD2)
+            if(KA > 1 && reads_tape(1)) load(tape + (size_t)(s_top - 1) * NN, nxt2);
+#pragma unroll 1
+        for(int st = 0; st < KA; st++) {
+            if constexpr(AHEAD2) {
+#pragma unroll
+                for(int j = 0; j < RPW; j++)
+#pragma unroll
+                    for(int c = 0; c < C; c++) {
+                        psi[c][j] = nxt[c][j];
+                        nxt[c][j] = nxt2[c][j];
+                    }
+                if(st + 2 < KA && reads_tape(st + 2)) load(tape + (size_t)(s_top - st - 2) * NN, nxt2);
+            }
+            else {
+#pragma unroll
+                for(int j = 0; j < RPW; j++)
+#pragma unroll
+                    for(int c = 0; c < C; c++) psi[c][j] = nxt[c][j];
+                if(st + 1 < KA && reads_tape(st + 1)) load(tape + (size_t)(s_top - st - 1) * NN, nxt);
+            }
+            step(st, psi);
         }
     }
 
