@@ -35,6 +35,17 @@ __device__ __forceinline__ T2 csub(T2 a, T2 b) { return T2{a.x - b.x, a.y - b.y}
 template <typename T2>
 __device__ __forceinline__ T2 conj_(T2 a) { return T2{a.x, -a.y}; }
 
+// streaming store of one complex entry (written once, read by the next pass)
+template <typename T2>
+__device__ __forceinline__ void nt_store(T2 *p, T2 v) {
+    using T = decltype(T2{}.x);
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    V2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<V2 *>(p));
+}
+
 // LDS index with one pad element every 32: the Stockham writes of the first
 // passes (stride 4 elements) then spread over every bank group.
 __device__ __forceinline__ constexpr int pidx(int i) { return i + (i >> 5); }
@@ -655,7 +666,7 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
     for(int m = 0; m < E; m++) {
         const int i = lane + 64 * m;
         if(MODE == 0 && rhl) dst[rh_at(i + h * n2)] = v[m];
-        else dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride] = v[m];
+        else nt_store(&dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride], v[m]);
     }
 }
 
@@ -704,7 +715,7 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
 #pragma unroll
     for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
     wave_lds_sync();
-    auto emit = [&](int k, T2 zk, T2 znk) { out[(size_t)k * ks] = r2c_entry<T>(zk, znk, tw[k]); };
+    auto emit = [&](int k, T2 zk, T2 znk) { nt_store(&out[(size_t)k * ks], r2c_entry<T>(zk, znk, tw[k])); };
 #pragma unroll
     for(int m = 0; m < E; m++) {
         const int k = lane + 64 * m;
