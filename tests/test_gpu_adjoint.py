@@ -95,10 +95,13 @@ def test_adjoint_finite_differences(ch, ctx):
         assert g[i] == pytest.approx(fd, rel=1e-6), name
 
 
-def test_fp32_adjoint_against_fp64(ch, ctx):
-    kw = dict(dims=2, N=64, dt=0.01, dx=1.0, k=1.0, M=1.0)
-    psi0 = O.init_density(2, 64, 0.0, 0.3, 0, 9)
-    w = np.random.default_rng(3).normal(size=64 * 64)
+# N = 64 is one whole-grid region; N = 256 runs the tiled reverse kernel (64
+# blocks, 8 backward steps per launch) with its ring of tape buffers
+@pytest.mark.parametrize("N", [64, 256])
+def test_fp32_adjoint_against_fp64(ch, ctx, N):
+    kw = dict(dims=2, N=N, dt=0.01, dx=1.0, k=1.0, M=1.0)
+    psi0 = O.init_density(2, N, 0.0, 0.3, 0, 9)
+    w = np.random.default_rng(3).normal(size=N * N)
     _, lam64, g64 = gpu_adjoint(ch, ctx, kw, psi0, 100, w, 25)
     _, lam32, g32 = gpu_adjoint(ch, ctx, kw, psi0, 100, w, 25, precision="fp32")
     err = rel_linf(lam32, lam64)
