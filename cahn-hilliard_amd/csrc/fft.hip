@@ -408,12 +408,15 @@ __global__ void k_update1d(typename Complex<T>::type *__restrict__ W, typename C
 // The first barrier orders the block's earlier LDS accesses (the staged
 // twiddle table, another wave's reads of this buffer) before the transform;
 // the buffer is the wave's own from there on, so the passes within the
-// transform are ordered by wave_lds_sync alone.
-template <typename T2, int LOG2N, bool INV, int E>
+// transform are ordered by wave_lds_sync alone.  BLOCK_FIRST = false: the
+// caller has ordered the block's accesses already and the buffer is the
+// wave's own, so the first sync is the wave's too.
+template <typename T2, int LOG2N, bool INV, bool BLOCK_FIRST = true, int E>
 __device__ __forceinline__ void wave_transform(int lane, T2 (&v)[E], T2 *lds, const T2 *tw, int tstride) {
     static_assert(E == WaveFFT<T2, LOG2N>::E, "entries per lane");
     using F = WaveFFT<T2, LOG2N>;
-    lds_barrier();
+    if constexpr(BLOCK_FIRST) lds_barrier();
+    else wave_lds_sync();
     F::template pass1_write<INV>(lane, v, lds);
     wave_lds_sync();
     F::pass_read(lane, v, lds);
@@ -778,27 +781,30 @@ __global__ __launch_bounds__(64) void k_xinvw(const typename Complex<T>::type *_
     for(int m = 0; m < E; m++) rout[lane + 64 * m] = v[m];
 }
 
-// x pass between two steps (as k_xrow): c2r, f', r2c, in place on W
-template <typename T, int MODEL, int LOG2N>
-__global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restrict__ W,
-                                              const typename Complex<T>::type *tw, Params p, int rows,
-                                              long rs, long ks, T *__restrict__ rho_out) {
+// x pass between two steps (as k_xrow): c2r, f', r2c, in place on W.  NR rows
+// per block, one per wave, share one staged twiddle table (rows % NR == 0)
+template <typename T, int MODEL, int LOG2N, int NR = 1>
+__global__ __launch_bounds__(64 * NR) void k_xroww(typename Complex<T>::type *__restrict__ W,
+                                                   const typename Complex<T>::type *tw, Params p, int rows,
+                                                   long rs, long ks, T *__restrict__ rho_out) {
     using T2 = typename Complex<T>::type;
     using F = WaveFFT<T2, LOG2N>;
     constexpr int E = F::E, n = F::n;
-    __shared__ T2 lds[F::LDS];
+    __shared__ T2 lds[NR][F::LDS];
     using Stage = TwiddleStage<T2, 2 * n>;
     __shared__ Stage stage;
-    const int row = xcd_tile(blockIdx.x, gridDim.x);
-    if(row >= rows) return;
-    const int lane = threadIdx.x;
-    T2 twr[Stage::template per<64>()];
-    stage.template load<64>(tw, lane, twr);
+    const int row = xcd_tile(blockIdx.x, gridDim.x) * NR + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if(row - (int)(threadIdx.x >> 6) >= rows) return;  // the whole block lies past the last row
+    const int lane = threadIdx.x & 63;
+    T2 twr[Stage::template per<64 * NR>()];
+    stage.template load<64 * NR>(tw, threadIdx.x, twr);
     T2 *w = W + (size_t)row * rs;
     T2 v[E];
     c2r_pre<T, E>(w, ks, lane, tw, v);  // before the first barrier: global table
-    tw = stage.template store<64>(tw, lane, twr);
-    wave_transform<T2, LOG2N, true>(lane, v, lds, tw, 2);
+    tw = stage.template store<64 * NR>(tw, threadIdx.x, twr);
+    T2 *own = lds[threadIdx.x >> 6];
+    lds_barrier();  // the staged table; every buffer below is one wave's own
+    wave_transform<T2, LOG2N, true, false>(lane, v, own, tw, 2);
     if(rho_out) {
         T2 *ro = reinterpret_cast<T2 *>(rho_out + (size_t)row * 2 * n);
 #pragma unroll
@@ -806,8 +812,8 @@ __global__ __launch_bounds__(64) void k_xroww(typename Complex<T>::type *__restr
     }
 #pragma unroll
     for(int m = 0; m < E; m++) v[m] = T2{fprime<T, MODEL>(p, v[m].x), fprime<T, MODEL>(p, v[m].y)};
-    wave_transform<T2, LOG2N, false>(lane, v, lds, tw, 2);
-    r2c_post<T, LOG2N>(v, lds, lane, tw, w, ks);
+    wave_transform<T2, LOG2N, false, false>(lane, v, own, tw, 2);
+    r2c_post<T, LOG2N>(v, own, lane, tw, w, ks);
 }
 
 // x pass of the pseudospectral_mobility flux (as k_xroww with the flux in place
@@ -878,8 +884,13 @@ template <typename T, int MODEL, int LX>
 static void launch_xrow(typename Complex<T>::type *W, const typename Complex<T>::type *tw, const Params &p, int rows,
                         long rs, long ks, T *rho_out, hipStream_t s) {
     if constexpr(wave_len<LX>()) {
-        hipLaunchKernelGGL((k_xroww<T, MODEL, LX>), dim3((rows + 7) / 8 * 8), dim3(64), 0, s, W, tw, p, rows, rs, ks,
-                           rho_out);
+        constexpr int NR = 4;  // rows per block sharing one staged table
+        if(rows % NR == 0)
+            hipLaunchKernelGGL((k_xroww<T, MODEL, LX, NR>), dim3((rows / NR + 7) / 8 * 8), dim3(64 * NR), 0, s, W, tw,
+                               p, rows, rs, ks, rho_out);
+        else
+            hipLaunchKernelGGL((k_xroww<T, MODEL, LX>), dim3((rows + 7) / 8 * 8), dim3(64), 0, s, W, tw, p, rows, rs, ks,
+                               rho_out);
     }
     else {
         using GX = FftGeom<LX, PackedLines<LX>::value>;
