@@ -669,7 +669,8 @@ __global__ __launch_bounds__(128) void k_colw2(const typename Complex<T>::type *
     for(int m = 0; m < E; m++) {
         const int i = lane + 64 * m;
         if(MODE == 0 && rhl) dst[rh_at(i + h * n2)] = v[m];
-        else nt_store(&dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride], v[m]);
+        else if constexpr(MODE == 2) nt_store(&dst[base + (size_t)(2 * i + h) * stride], v[m]);
+        else dst[base + (size_t)(MODE == 0 ? i + h * n2 : 2 * i + h) * stride] = v[m];
     }
 }
 
@@ -708,8 +709,9 @@ __device__ __forceinline__ void c2r_pre(const typename Complex<T>::type *__restr
     for(int m = 0; m < E; m++) z[m] = c2r_entry(in, ks, lane + 64 * m, n, tw);
 }
 
-// r2c post-processing: Z (distribution D) -> n + 1 half-spectrum entries
-template <typename T, int LOG2N, typename T2>
+// r2c post-processing: Z (distribution D) -> n + 1 half-spectrum entries;
+// NT: streaming stores (the pseudospectral chain's work field)
+template <typename T, int LOG2N, bool NT = false, typename T2>
 __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T2 *__restrict__ tw,
                                          T2 *__restrict__ out, long ks) {
     using F = WaveFFT<T2, LOG2N>;
@@ -718,7 +720,10 @@ __device__ __forceinline__ void r2c_post(const T2 *z, T2 *lds, int lane, const T
 #pragma unroll
     for(int m = 0; m < E; m++) lds[F::pad(lane + 64 * m)] = z[m];
     wave_lds_sync();
-    auto emit = [&](int k, T2 zk, T2 znk) { nt_store(&out[(size_t)k * ks], r2c_entry<T>(zk, znk, tw[k])); };
+    auto emit = [&](int k, T2 zk, T2 znk) {
+        if constexpr(NT) nt_store(&out[(size_t)k * ks], r2c_entry<T>(zk, znk, tw[k]));
+        else out[(size_t)k * ks] = r2c_entry<T>(zk, znk, tw[k]);
+    };
 #pragma unroll
     for(int m = 0; m < E; m++) {
         const int k = lane + 64 * m;
@@ -813,7 +818,7 @@ __global__ __launch_bounds__(64 * NR) void k_xroww(typename Complex<T>::type *__
 #pragma unroll
     for(int m = 0; m < E; m++) v[m] = T2{fprime<T, MODEL>(p, v[m].x), fprime<T, MODEL>(p, v[m].y)};
     wave_transform<T2, LOG2N, false, false>(lane, v, own, tw, 2);
-    r2c_post<T, LOG2N>(v, own, lane, tw, w, ks);
+    r2c_post<T, LOG2N, true>(v, own, lane, tw, w, ks);
 }
 
 // x pass of the pseudospectral_mobility flux (as k_xroww with the flux in place
