@@ -163,4 +163,45 @@ struct WaveFFT {
     }
 };
 
+// 256-point transform on one wave, 4 entries per lane (distribution D: lane
+// l holds entries l + 64 m), as four radix-4 Stockham passes: pass s (stride
+// Ns = 4^s) reads inputs j + 64 r of butterfly j = lane, applies W_{4 Ns}^{jm r}
+// (jm = j mod Ns) and writes outputs (j - jm) 4 + jm + r Ns; the last pass's
+// outputs lane + 64 r are distribution D again.  The wave's own LDS buffer
+// (LDS entries, padded by one every four) carries the three exchanges; the
+// caller orders the block's earlier accesses to it.  tstride: table index of
+// W_256^1 in the table of W_Ntab^m.
+template <typename T2>
+struct WaveFFT4 {
+    static constexpr int n = 256, E = 4, LDS = 320;
+    CH_HD static int pad(int i) { return i + (i >> 2); }
+    template <bool INV>
+    CH_HD static T2 tw(const T2 *table, int m, int tstride) {
+        const T2 w = table[m * tstride];
+        return INV ? T2{w.x, -w.y} : w;
+    }
+    template <bool INV>
+    __device__ static void run(int lane, T2 (&v)[4], T2 *lds, const T2 *table, int tstride) {
+        dft<4, INV>(v);
+#pragma unroll
+        for(int r = 0; r < 4; r++) lds[pad(4 * lane + r)] = v[r];
+#pragma unroll
+        for(int s = 1; s < 4; s++) {
+            const int Ns = 1 << (2 * s);
+            wave_lds_sync();
+#pragma unroll
+            for(int r = 0; r < 4; r++) v[r] = lds[pad(lane + 64 * r)];
+            const int jm = lane & (Ns - 1);
+#pragma unroll
+            for(int r = 1; r < 4; r++) v[r] = w_mul(v[r], tw<INV>(table, jm * r * (64 >> (2 * s)), tstride));
+            dft<4, INV>(v);
+            if(s < 3) {
+                wave_lds_sync();
+#pragma unroll
+                for(int r = 0; r < 4; r++) lds[pad((lane - jm) * 4 + jm + r * Ns)] = v[r];
+            }
+        }
+    }
+};
+
 }  // namespace chd

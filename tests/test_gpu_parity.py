@@ -113,8 +113,8 @@ def test_stencil_large_grids_against_oracle(ch, ctx, dims, N, steps, integrator)
 @pytest.mark.parametrize("N,steps", [(512, 20), (1024, 20), (2048, 6)])
 def test_pseudospectral_large_grids_against_oracle(ch, ctx, N, steps):
     """2D grids whose axis passes run the wave kernels: one wave per line
-    (N = 512) and two waves per line (N = 1024, 2048), with the one-wave row
-    passes.  FP64 within 1e-10 relative L-inf of the oracle, mass within 1e-12;
+    (N = 512), two waves per line (N = 1024, 2048) and four waves per line
+    (the FP32 fused column pass at N = 1024), with the one-wave row passes.  FP64 within 1e-10 relative L-inf of the oracle, mass within 1e-12;
     FP32 within 1e-4."""
     print(f"pseudospectral 2D N={N}, {steps} steps", flush=True)
     rho0 = O.init_density(2, N, 0.0, 0.3, 0, 5)
